@@ -1,0 +1,223 @@
+"""``assemble(op, X, Y; quadstrat)`` on the GPU, through the C ABI.
+
+Host-side mirror of src/operator.jl:100-116 (``assemble``) and
+src/excitation.jl:17-30 (``assemble(field, tfs)``): operators, spaces and
+quadrature strategies are resolved here, and the native library does all the
+arithmetic.  There is no CPU path: unsupported inputs raise.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as nat
+from .operators import (DoubleNumQStrat, DoubleNumSauterQstrat, DoubleNumWiltonSauterQStrat,
+                        LinearCombinationOfOperators, MWDoubleLayer3D, MWSingleLayer3D, PlaneWaveMW,
+                        SingleNumQStrat, TangTraceMW, defaultquadstrat, resolve_quadstrat)
+from .rt import RTBasis
+
+
+class Context:
+    """One GPU and one stream (replaces the reference's Threading{:multi} driver)."""
+
+    def __init__(self, device=0):
+        lib = nat.load()
+        h = C.c_void_p()
+        nat.check(lib.bh_ctx_create(int(device), C.byref(h)))
+        self.handle = h
+        self.device = device
+        self._lib = lib
+
+    def synchronize(self):
+        nat.check(self._lib.bh_ctx_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            self._lib.bh_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class _Handle:
+    def __init__(self, h, destroy, keep):
+        self.h = h
+        self._destroy = destroy
+        self._keep = keep
+
+    def __del__(self):
+        try:
+            if self.h:
+                self._destroy(self.h)
+        except Exception:
+            pass
+
+
+def mesh_handle(mesh, ctx):
+    cache = mesh.__dict__.setdefault("_bh", {})
+    key = id(ctx)
+    if key not in cache:
+        lib = nat.load()
+        V = nat.as_f64(mesh.vertices)
+        Cc = nat.as_i32(mesh.cells)
+        h = C.c_void_p()
+        nat.check(lib.bh_mesh_create(ctx.handle, nat.dptr(V), V.shape[0], nat.i32ptr(Cc), Cc.shape[0], C.byref(h)))
+        cache[key] = _Handle(h, lib.bh_mesh_destroy, ctx)
+    return cache[key].h
+
+
+def space_handle(space, ctx):
+    cache = space.__dict__.setdefault("_bh", {})
+    key = id(ctx)
+    if key not in cache:
+        lib = nat.load()
+        mh = mesh_handle(space.geo, ctx)
+        h = C.c_void_p()
+        nat.check(lib.bh_space_create(mh, space.numfunctions(), nat.i32ptr(space.shape_ptr),
+                                      nat.i32ptr(space.shape_cell), nat.i32ptr(space.shape_ref),
+                                      nat.dptr(space.shape_coeff), C.byref(h)))
+        cache[key] = _Handle(h, lib.bh_space_destroy, (ctx, mh))
+    return cache[key].h
+
+
+def op_struct(op):
+    s = nat.bh_operator()
+    if isinstance(op, MWSingleLayer3D):
+        s.kind = nat.OP_SINGLELAYER
+        s.alpha[:] = [op.alpha.real, op.alpha.imag]
+        s.beta[:] = [op.beta.real, op.beta.imag]
+    elif isinstance(op, MWDoubleLayer3D):
+        s.kind = nat.OP_DOUBLELAYER
+        s.alpha[:] = [op.alpha.real, op.alpha.imag]
+        s.beta[:] = [0.0, 0.0]
+    else:
+        raise nat.UnsupportedOnGPU(nat.BH_ERR_UNSUPPORTED, f"operator {type(op).__name__} has no GPU path")
+    g = complex(op.gamma)
+    s.gamma[:] = [g.real, g.imag]
+    return s
+
+
+def qs_struct(qs):
+    s = nat.bh_quadstrat()
+    if isinstance(qs, (DoubleNumWiltonSauterQStrat, DoubleNumSauterQstrat, DoubleNumQStrat)):
+        s.kind = qs.kind
+        for i, v in enumerate(qs.params):
+            s.params[i] = int(v)
+    else:
+        raise nat.UnsupportedOnGPU(nat.BH_ERR_UNSUPPORTED, f"quadrature strategy {qs!r} has no GPU path")
+    return s
+
+
+def _terms(op):
+    if isinstance(op, LinearCombinationOfOperators):
+        return op.terms
+    return [(1.0, op)]
+
+
+def assemble(op, X, Y=None, quadstrat=defaultquadstrat, precision="fp64", ctx=None, threading="gpu", out=None,
+             row_offset=0, col_offset=0, **kwargs):
+    """Dense matrix of ``op`` tested with ``X`` and trial ``Y`` (column-major complex).
+
+    With ``Y`` omitted and ``op`` a functional, assembles the right-hand side
+    (``assemble(e, X)``).  ``out``/offsets give the ``assemble!`` store semantics:
+    contributions are added into ``out[row_offset:, col_offset:]``.
+    """
+    ctx = ctx or default_context()
+    if Y is None:
+        return assemble_rhs(op, X, quadstrat=quadstrat, ctx=ctx)
+    if not isinstance(X, RTBasis) or not isinstance(Y, RTBasis):
+        raise nat.UnsupportedOnGPU(nat.BH_ERR_UNSUPPORTED, "GPU assembly supports raviartthomas spaces only")
+    prec = {"fp64": nat.FP64, "fp32": nat.FP32}[precision]
+    dt = np.complex128 if prec == nat.FP64 else np.complex64
+    M, N = X.numfunctions(), Y.numfunctions()
+    if out is None:
+        out = np.zeros((M, N), dtype=dt, order="F")
+        row_offset = col_offset = 0
+    if out.dtype != dt or not out.flags.f_contiguous:
+        raise ValueError("out must be a Fortran-ordered array of the assembly precision")
+    if M == 0 or N == 0:
+        return out
+    lib = nat.load()
+    xh, yh = space_handle(X, ctx), space_handle(Y, ctx)
+    for coef, term in _terms(op):
+        qs = resolve_quadstrat(quadstrat, term, X, Y)
+        sc = (C.c_double * 2)(complex(coef).real, complex(coef).imag)
+        nat.check(lib.bh_assemble(ctx.handle, C.byref(op_struct(term)), C.byref(qs_struct(qs)), prec, xh, yh, sc, 1,
+                                  out.ctypes.data_as(C.c_void_p), out.shape[0], row_offset, col_offset))
+    return out
+
+
+def assemble_rhs(field, X, quadstrat=defaultquadstrat, ctx=None):
+    """``assemble((n x E) x n, X)`` for a plane wave E (excitation.jl:17-115)."""
+    ctx = ctx or default_context()
+    if not isinstance(field, TangTraceMW) or not isinstance(field.field, PlaneWaveMW):
+        raise nat.UnsupportedOnGPU(nat.BH_ERR_UNSUPPORTED, "GPU right-hand side supports (n x E) x n of a plane wave")
+    qs = resolve_quadstrat(quadstrat, field, X, None)
+    if not isinstance(qs, SingleNumQStrat):
+        raise ValueError("right-hand sides take a SingleNumQStrat")
+    E = field.field
+    lib = nat.load()
+    b = np.zeros(X.numfunctions(), dtype=np.complex128)
+    d = nat.as_f64(E.direction)
+    pr = nat.as_f64(E.polarisation.real)
+    pi = nat.as_f64(E.polarisation.imag)
+    g = np.array([E.gamma.real, E.gamma.imag])
+    a = np.array([E.amplitude.real, E.amplitude.imag])
+    nat.check(lib.bh_assemble_planewave(ctx.handle, space_handle(X, ctx), nat.dptr(d), nat.dptr(pr), nat.dptr(pi),
+                                        nat.dptr(g), nat.dptr(a), int(qs.quad_rule), 1,
+                                        b.ctypes.data_as(nat._D)))
+    return b
+
+
+def classify(op, X, Y, quadstrat=defaultquadstrat, ctx=None):
+    """Pair classes [nf_test, nf_trial] (0 far, 1 near, 2 CV, 3 CE, 4 CF)."""
+    ctx = ctx or default_context()
+    qs = resolve_quadstrat(quadstrat, op, X, Y)
+    lib = nat.load()
+    mx, my = X.geo if isinstance(X, RTBasis) else X, Y.geo if isinstance(Y, RTBasis) else Y
+    out = np.zeros((mx.numcells, my.numcells), dtype=np.int8)
+    nat.check(lib.bh_classify(ctx.handle, C.byref(op_struct(op)), C.byref(qs_struct(qs)), mesh_handle(mx, ctx),
+                              mesh_handle(my, ctx), out.ctypes.data_as(C.POINTER(C.c_int8))))
+    return out
+
+
+def local_blocks(op, mesh_t, mesh_s, pairs, quadstrat=defaultquadstrat, ctx=None):
+    """3x3 local blocks z[i, j] for (test cell, trial cell) pairs, as the GPU computes them."""
+    ctx = ctx or default_context()
+    qs = resolve_quadstrat(quadstrat, op, mesh_t, mesh_s)
+    pairs = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+    tc, sc = nat.as_i32(pairs[:, 0]), nat.as_i32(pairs[:, 1])
+    out = np.zeros((len(pairs), 18))
+    lib = nat.load()
+    nat.check(lib.bh_local_blocks(ctx.handle, C.byref(op_struct(op)), C.byref(qs_struct(qs)),
+                                  mesh_handle(mesh_t, ctx), mesh_handle(mesh_s, ctx), len(pairs), nat.i32ptr(tc),
+                                  nat.i32ptr(sc), nat.dptr(out)))
+    z = out[:, 0::2] + 1j * out[:, 1::2]
+    return z.reshape(-1, 3, 3).transpose(0, 2, 1)   # column-major 3x3 -> [i, j]
+
+
+def element_to_dof(X, ctx=None):
+    """The element-to-dof map the library built from X (dof[nf, 3, kmax], coeff)."""
+    ctx = ctx or default_context()
+    lib = nat.load()
+    h = space_handle(X, ctx)
+    k = C.c_int32()
+    nat.check(lib.bh_space_e2d(h, C.byref(k), None, None, 0))
+    n = X.geo.numcells * 3 * k.value
+    dof = np.zeros(n, dtype=np.int32)
+    cf = np.zeros(n)
+    nat.check(lib.bh_space_e2d(h, C.byref(k), nat.i32ptr(dof), nat.dptr(cf), n))
+    return dof.reshape(X.geo.numcells, 3, k.value), cf.reshape(X.geo.numcells, 3, k.value)

@@ -1,0 +1,193 @@
+// C-ABI glue: errors, contexts, meshes and spaces.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <new>
+
+#include "bh_internal.h"
+
+namespace bh {
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+const char *get_error() { return g_err; }
+
+}  // namespace bh
+
+using namespace bh;
+
+extern "C" int bh_version(void) { return BH_VERSION; }
+
+extern "C" const char *bh_last_error(void) { return bh::get_error(); }
+
+extern "C" int bh_ctx_create(int device, bh_ctx **out)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(out != nullptr, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    BH_CHECK_HIP(hipGetDeviceCount(&n));
+    BH_REQUIRE(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
+    BH_CHECK_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    BH_CHECK_HIP(hipGetDeviceProperties(&prop, device));
+    std::unique_ptr<bh_ctx> c(new bh_ctx());
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    c->arch = prop.gcnArchName;
+    BH_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+    BH_API_END
+}
+
+extern "C" int bh_ctx_destroy(bh_ctx *ctx)
+{
+    BH_API_BEGIN
+    if (!ctx) return BH_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    BH_API_END
+}
+
+extern "C" int bh_ctx_synchronize(bh_ctx *ctx)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctx != nullptr, "ctx is NULL");
+    BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    BH_API_END
+}
+
+extern "C" int bh_mesh_create(bh_ctx *ctx, const double *vertices, int64_t nv, const int32_t *cells, int64_t nf,
+                              bh_mesh **out)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctx && out, "NULL argument");
+    *out = nullptr;
+    BH_REQUIRE(nv >= 0 && nf >= 0 && nf < (int64_t(1) << 31), "bad mesh sizes");
+    BH_REQUIRE(nv == 0 || vertices, "vertices is NULL");
+    BH_REQUIRE(nf == 0 || cells, "cells is NULL");
+    BH_CHECK_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<bh_mesh> m(new bh_mesh());
+    m->ctx = ctx;
+    m->nv = nv;
+    m->nf = nf;
+    m->V.assign(vertices, vertices + 3 * nv);
+    m->C.assign(cells, cells + 3 * nf);
+    for (int64_t k = 0; k < 3 * nv; ++k) BH_REQUIRE(std::isfinite(m->V[k]), "vertex coordinate not finite");
+    m->geo.resize((size_t)nf * Mesh::kGeo);
+    for (int64_t c = 0; c < nf; ++c) {
+        double *g = &m->geo[(size_t)c * Mesh::kGeo];
+        for (int a = 0; a < 3; ++a) {
+            const int32_t v = m->C[3 * c + a];
+            BH_REQUIRE(v >= 0 && v < nv, "cell %lld references vertex %d out of range", (long long)c, v);
+            for (int k = 0; k < 3; ++k) g[3 * a + k] = m->V[3 * v + k];
+        }
+        for (int k = 0; k < 3; ++k) g[9 + k] = (g[k] + g[3 + k] + g[6 + k]) / 3.0;
+        // area as 0.5 |(v1 - v3) x (v2 - v3)| (chart tangents, rtlocal.jl:8-27)
+        const double t1[3] = {g[0] - g[6], g[1] - g[7], g[2] - g[8]};
+        const double t2[3] = {g[3] - g[6], g[4] - g[7], g[5] - g[8]};
+        const double n[3] = {t1[1] * t2[2] - t1[2] * t2[1], t1[2] * t2[0] - t1[0] * t2[2],
+                             t1[0] * t2[1] - t1[1] * t2[0]};
+        const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        BH_REQUIRE(nn > 0, "cell %lld is degenerate (zero area)", (long long)c);
+        g[12] = 0.5 * nn;
+        for (int k = 0; k < 3; ++k) g[13 + k] = n[k] / nn;
+    }
+    m->d_geo.upload(m->geo.data(), m->geo.size(), ctx->stream);
+    BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    *out = m.release();
+    BH_API_END
+}
+
+extern "C" int bh_mesh_destroy(bh_mesh *mesh)
+{
+    BH_API_BEGIN
+    if (mesh) {
+        (void)hipSetDevice(mesh->ctx->device);
+        delete mesh;
+    }
+    BH_API_END
+}
+
+extern "C" int bh_space_create(bh_mesh *mesh, int64_t ndofs, const int32_t *shape_ptr, const int32_t *shape_cell,
+                               const int32_t *shape_ref, const double *shape_coeff, bh_space **out)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(mesh && out, "NULL argument");
+    *out = nullptr;
+    BH_REQUIRE(ndofs >= 0 && ndofs < (int64_t(1) << 31), "bad ndofs");
+    BH_REQUIRE(ndofs == 0 || shape_ptr, "shape_ptr is NULL");
+    std::unique_ptr<bh_space> s(new bh_space());
+    s->mesh = mesh;
+    s->ndofs = ndofs;
+    s->ptr.assign(shape_ptr, shape_ptr + (ndofs + 1));
+    BH_REQUIRE(ndofs == 0 || s->ptr[0] == 0, "shape_ptr[0] must be 0");
+    for (int64_t b = 0; b < ndofs; ++b) BH_REQUIRE(s->ptr[b + 1] >= s->ptr[b], "shape_ptr must be non-decreasing");
+    const int64_t ns = ndofs ? s->ptr[ndofs] : 0;
+    if (ns) BH_REQUIRE(shape_cell && shape_ref && shape_coeff, "shape arrays are NULL");
+    s->cell.assign(shape_cell, shape_cell + ns);
+    s->ref.assign(shape_ref, shape_ref + ns);
+    s->coeff.assign(shape_coeff, shape_coeff + ns);
+    for (int64_t k = 0; k < ns; ++k) {
+        BH_REQUIRE(s->cell[k] >= 0 && s->cell[k] < mesh->nf, "shape %lld: cell out of range", (long long)k);
+        BH_REQUIRE(s->ref[k] >= 0 && s->ref[k] < 3, "shape %lld: refid out of range", (long long)k);
+        BH_REQUIRE(std::isfinite(s->coeff[k]), "shape %lld: coefficient not finite", (long long)k);
+    }
+    // element-to-dof map (assemblydata, basis.jl:235-294)
+    std::vector<int> cnt((size_t)mesh->nf * 3, 0);
+    for (int64_t k = 0; k < ns; ++k) cnt[3 * s->cell[k] + s->ref[k]]++;
+    int kmax = 1;
+    for (int c : cnt) kmax = std::max(kmax, c);
+    s->kmax = kmax;
+    s->e2d.assign((size_t)mesh->nf * 3 * kmax, -1);
+    s->e2c.assign((size_t)mesh->nf * 3 * kmax, 0.0);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int64_t b = 0; b < ndofs; ++b)
+        for (int k = s->ptr[b]; k < s->ptr[b + 1]; ++k) {
+            const int slot = 3 * s->cell[k] + s->ref[k];
+            const size_t at = (size_t)slot * kmax + cnt[slot]++;
+            s->e2d[at] = (int32_t)b;
+            s->e2c[at] = s->coeff[k];
+        }
+    hipStream_t st = mesh->ctx->stream;
+    s->d_e2d.upload(s->e2d.data(), s->e2d.size(), st);
+    s->d_e2c.upload(s->e2c.data(), s->e2c.size(), st);
+    BH_CHECK_HIP(hipStreamSynchronize(st));
+    *out = s.release();
+    BH_API_END
+}
+
+extern "C" int bh_space_destroy(bh_space *space)
+{
+    BH_API_BEGIN
+    delete space;
+    BH_API_END
+}
+
+extern "C" int bh_space_e2d(const bh_space *space, int32_t *kmax, int32_t *dof, double *coeff, int64_t cap)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(space && kmax, "NULL argument");
+    *kmax = space->kmax;
+    const int64_t need = (int64_t)space->e2d.size();
+    if (dof && coeff) {
+        BH_REQUIRE(cap >= need, "capacity %lld < %lld", (long long)cap, (long long)need);
+        std::memcpy(dof, space->e2d.data(), sizeof(int32_t) * need);
+        std::memcpy(coeff, space->e2c.data(), sizeof(double) * need);
+    }
+    BH_API_END
+}

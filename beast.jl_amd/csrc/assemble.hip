@@ -1,0 +1,1206 @@
+// Galerkin assembly of MWSingleLayer3D / MWDoubleLayer3D on RT x RT on gfx950.
+//
+// Reference path (src/integralop.jl:69-184): for every test x trial cell pair,
+// quadrule() picks a rule (doublenumwiltonsauterqstrat.jl:32-69), momintegrals!
+// integrates the 3x3 local block, and the block is scattered with the signed
+// element-to-dof map into Z.  Here that becomes:
+//
+//   K0  k_classify_count / k_classify_fill: FP64 pair classification, bit-exact
+//       with the reference, compacting the non-far pairs of every test cell
+//       into a CSR list in ascending trial order (deterministic).
+//   K2  k_sauter_schwab: Sauter-Schwab blocks of touching pairs, one wavefront
+//       per pair, launched per class so no wavefront diverges on the class.
+//   K3  k_wilton: near pairs by singularity extraction (regular part by double
+//       Gauss, singular part by analytic potential integrals), one wavefront per pair.
+//   K1+K4 k_far_tiles: one workgroup per 64x64 tile of Z.  It computes the
+//       double-Gauss blocks of every (row cell, column cell) pair the tile needs,
+//       chunk by chunk into LDS (non-far pairs read their K2/K3 block), and
+//       gathers them through the signed dof map into registers; each Z entry is
+//       written once, coalesced, without atomics.  The accumulation order of an
+//       entry depends only on its own mesh and space, so a request gives the
+//       same bits alone or batched with others.
+//
+// Local blocks use coordinates relative to cell centroids, so the expanded
+// moment formulas below do not cancel catastrophically (and the FP32 variant
+// keeps ~1e-7 relative accuracy).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "bh_device.h"
+#include "bh_internal.h"
+
+namespace bh {
+
+struct OpParams {
+    int kind;
+    double gr, gi;   // gamma
+    double ar, ai;   // alpha
+    double br, bi;   // beta
+};
+
+// ============================================================ K0 classification
+__global__ __launch_bounds__(256) void k_classify_count(const double *__restrict__ tgeo, int nft,
+                                                        const double *__restrict__ sgeo, int nfs, double k2,
+                                                        int mode, int *__restrict__ counts, int *__restrict__ err)
+{
+    __shared__ int wsum[4];
+    const int p = blockIdx.x;
+    double tv[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tv[k] = tgeo[(size_t)p * kGeo + k];
+    int cnt = 0, bad = 0;
+    for (int q = threadIdx.x; q < nfs; q += blockDim.x) {
+        const double *s = sgeo + (size_t)q * kGeo;
+        double sv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sv[k] = s[k];
+        int pk;
+        const int c = classify_pair(tv, sv, s[kGeoA], k2, mode, &pk);
+        if (c < 0) bad = 1;
+        else cnt += (c != kFar);
+    }
+    if (bad) atomicOr(err, 1);
+    for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[p] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(256) void k_classify_fill(const double *__restrict__ tgeo, int nft,
+                                                       const double *__restrict__ sgeo, int nfs, double k2, int mode,
+                                                       const int *__restrict__ offsets, int4 *__restrict__ entries)
+{
+    __shared__ int wtot[4];
+    const int p = blockIdx.x;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double tv[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tv[k] = tgeo[(size_t)p * kGeo + k];
+    int base = offsets[p];
+    for (int q0 = 0; q0 < nfs; q0 += 256) {
+        const int q = q0 + threadIdx.x;
+        int c = kFar, pk = 0;
+        if (q < nfs) {
+            const double *s = sgeo + (size_t)q * kGeo;
+            double sv[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) sv[k] = s[k];
+            c = classify_pair(tv, sv, s[kGeoA], k2, mode, &pk);
+            if (c < 0) c = kFar;   // reported by the count pass
+        }
+        const bool flag = (c != kFar);
+        const unsigned long long m = __ballot(flag);
+        const int pre = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wtot[wid] = __popcll(m);
+        __syncthreads();
+        int off = 0;
+        for (int w = 0; w < wid; ++w) off += wtot[w];
+        const int tot = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        if (flag) entries[base + off + pre] = make_int4(p, q, c, pk);
+        base += tot;
+        __syncthreads();
+    }
+}
+
+__global__ void k_class_lists(const int4 *__restrict__ entries, int n, int *__restrict__ counters,
+                              int *__restrict__ lists, int cap)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int c = entries[e].z;   // 1..4
+    const int slot = atomicAdd(&counters[c], 1);
+    lists[(size_t)c * cap + slot] = e;
+}
+
+// Full class matrix (debug / bit-exact test entry).
+__global__ void k_classify_matrix(const double *__restrict__ tgeo, int nft, const double *__restrict__ sgeo, int nfs,
+                                  double k2, int mode, int8_t *__restrict__ out)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)nft * nfs) return;
+    const int p = (int)(idx / nfs), q = (int)(idx % nfs);
+    double tv[9], sv[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        tv[k] = tgeo[(size_t)p * kGeo + k];
+        sv[k] = sgeo[(size_t)q * kGeo + k];
+    }
+    int pk;
+    out[idx] = (int8_t)classify_pair(tv, sv, sgeo[(size_t)q * kGeo + kGeoA], k2, mode, &pk);
+}
+
+// ============================================================ shared helpers
+// Complex Green's function G = exp(-gamma R) / (4 pi R).
+template <typename R>
+__device__ __forceinline__ cx<R> green(R gr, R gi, R dist, R idist)
+{
+    R s, c;
+    if constexpr (sizeof(R) == 8) sincos(gi * dist, &s, &c);
+    else sincosf(gi * dist, &s, &c);
+    R amp = idist * (R)kInv4Pi;
+    if (gr != (R)0) {
+        if constexpr (sizeof(R) == 8) amp *= exp(-gr * dist);
+        else amp *= expf(-gr * dist);
+    }
+    return {c * amp, -s * amp};
+}
+
+// Block from EFIE moments: z_ij = s1 [alpha (Mxy - a_i.My - b_j.Mx + (a_i.b_j) M0) + 4 beta M0]
+template <typename R>
+__device__ __forceinline__ void efie_block(const OpParams &op, R s1, const R *a, const R *b, cx<R> M0, const cx<R> *Mx,
+                                          const cx<R> *My, cx<R> Mxy, cx<R> *z, int stride)
+{
+    const cx<R> al{(R)op.ar, (R)op.ai}, be{(R)op.br, (R)op.bi};
+    const cx<R> bM0 = cmul(be, M0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const R *ai = a + 3 * i, *bj = b + 3 * j;
+            const R ab = dot3(ai, bj);
+            cx<R> t;
+            t.r = Mxy.r - (ai[0] * My[0].r + ai[1] * My[1].r + ai[2] * My[2].r) -
+                  (bj[0] * Mx[0].r + bj[1] * Mx[1].r + bj[2] * Mx[2].r) + ab * M0.r;
+            t.i = Mxy.i - (ai[0] * My[0].i + ai[1] * My[1].i + ai[2] * My[2].i) -
+                  (bj[0] * Mx[0].i + bj[1] * Mx[1].i + bj[2] * Mx[2].i) + ab * M0.i;
+            const cx<R> at = cmul(al, t);
+            z[(i + 3 * j) * stride] = {s1 * (at.r + (R)4 * bM0.r), s1 * (at.i + (R)4 * bM0.i)};
+        }
+}
+
+// Block from MFIE-K moments (common origin): z_ij = s1 [(b_j - a_i).Kc + (b_j x a_i).Kr]
+template <typename R>
+__device__ __forceinline__ void mfie_block(R s1, const R *a, const R *b, const cx<R> *Kc, const cx<R> *Kr, cx<R> *z,
+                                          int stride)
+{
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const R *ai = a + 3 * i, *bj = b + 3 * j;
+            R d[3] = {bj[0] - ai[0], bj[1] - ai[1], bj[2] - ai[2]};
+            R c[3];
+            cross3(c, bj, ai);
+            cx<R> t;
+            t.r = d[0] * Kc[0].r + d[1] * Kc[1].r + d[2] * Kc[2].r + c[0] * Kr[0].r + c[1] * Kr[1].r + c[2] * Kr[2].r;
+            t.i = d[0] * Kc[0].i + d[1] * Kc[1].i + d[2] * Kc[2].i + c[0] * Kr[0].i + c[1] * Kr[1].i + c[2] * Kr[2].i;
+            z[(i + 3 * j) * stride] = {s1 * t.r, s1 * t.i};
+        }
+}
+
+// ============================================================ K2 Sauter-Schwab
+struct GLRule {
+    int n;
+    double x[kMaxGL];
+    double w[kMaxGL];
+};
+
+// Region parameterisations of Sauter & Schwab, "Boundary Element Methods"
+// (2011) Sec. 5.2, on the reference element {0 <= x2 <= x1 <= 1}.
+__device__ __forceinline__ double ss_region(int cls, int r, double xi, double e1, double e2, double e3, double *xh,
+                                            double *yh)
+{
+    if (cls == kCF) {
+        switch (r) {
+        case 0: xh[0] = xi; xh[1] = xi * (1 - e1 + e1 * e2); yh[0] = xi * (1 - e1 * e2 * e3); yh[1] = xi * (1 - e1); break;
+        case 1: xh[0] = xi * (1 - e1 * e2 * e3); xh[1] = xi * (1 - e1); yh[0] = xi; yh[1] = xi * (1 - e1 + e1 * e2); break;
+        case 2: xh[0] = xi; xh[1] = xi * e1 * (1 - e2 + e2 * e3); yh[0] = xi * (1 - e1 * e2); yh[1] = xi * e1 * (1 - e2); break;
+        case 3: xh[0] = xi * (1 - e1 * e2); xh[1] = xi * e1 * (1 - e2); yh[0] = xi; yh[1] = xi * e1 * (1 - e2 + e2 * e3); break;
+        case 4: xh[0] = xi * (1 - e1 * e2 * e3); xh[1] = xi * e1 * (1 - e2 * e3); yh[0] = xi; yh[1] = xi * e1 * (1 - e2); break;
+        default: xh[0] = xi; xh[1] = xi * e1 * (1 - e2); yh[0] = xi * (1 - e1 * e2 * e3); yh[1] = xi * e1 * (1 - e2 * e3); break;
+        }
+        return xi * xi * xi * e1 * e1 * e2;
+    }
+    if (cls == kCE) {
+        switch (r) {
+        case 0: xh[0] = xi; xh[1] = xi * e1 * e3; yh[0] = xi * (1 - e1 * e2); yh[1] = xi * e1 * (1 - e2);
+            return xi * xi * xi * e1 * e1;
+        case 1: xh[0] = xi; xh[1] = xi * e1; yh[0] = xi * (1 - e1 * e2 * e3); yh[1] = xi * e1 * e2 * (1 - e3); break;
+        case 2: xh[0] = xi * (1 - e1 * e2); xh[1] = xi * e1 * (1 - e2); yh[0] = xi; yh[1] = xi * e1 * e2 * e3; break;
+        case 3: xh[0] = xi * (1 - e1 * e2 * e3); xh[1] = xi * e1 * e2 * (1 - e3); yh[0] = xi; yh[1] = xi * e1; break;
+        default: xh[0] = xi * (1 - e1 * e2 * e3); xh[1] = xi * e1 * (1 - e2 * e3); yh[0] = xi; yh[1] = xi * e1 * e2; break;
+        }
+        return xi * xi * xi * e1 * e1 * e2;
+    }
+    if (r == 0) { xh[0] = xi; xh[1] = xi * e1; yh[0] = xi * e2; yh[1] = xi * e2 * e3; }
+    else { xh[0] = xi * e2; xh[1] = xi * e2 * e3; yh[0] = xi; yh[1] = xi * e1; }
+    return xi * xi * xi * e2;
+}
+
+__device__ __forceinline__ int ss_nregions(int cls) { return cls == kCF ? 6 : (cls == kCE ? 5 : 2); }
+
+// Vertex order for the Sauter-Schwab parameterisation: shared vertices first
+// (matched, in the order found with the test vertex outermost), then the
+// remaining vertices in ascending local order.
+__device__ __forceinline__ void ss_order(int cls, int pk, int *pi, int *pj)
+{
+    const int nh = cls == kCF ? 3 : (cls == kCE ? 2 : 1);
+    int ui = 0, uj = 0;
+    for (int h = 0; h < nh; ++h) {
+        pi[h] = (pk >> (2 * h)) & 3;
+        pj[h] = (pk >> (6 + 2 * h)) & 3;
+        ui |= 1 << pi[h];
+        uj |= 1 << pj[h];
+    }
+    int a = nh, b = nh;
+    for (int k = 0; k < 3; ++k) {
+        if (!(ui & (1 << k))) pi[a++] = k;
+        if (!(uj & (1 << k))) pj[b++] = k;
+    }
+}
+
+template <int OPK>
+__global__ __launch_bounds__(256) void k_sauter_schwab(const double *__restrict__ tgeo, const double *__restrict__ sgeo,
+                                                       const int4 *__restrict__ entries, const int *__restrict__ list,
+                                                       int nlist, int cls, GLRule gl, OpParams op,
+                                                       double *__restrict__ blocks)
+{
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= nlist) return;
+    const int lane = threadIdx.x & 63;
+    const int e = list[wv];
+    const int4 en = entries[e];
+    const double *tg = tgeo + (size_t)en.x * kGeo, *sg = sgeo + (size_t)en.y * kGeo;
+    const double c[3] = {tg[kGeoC], tg[kGeoC + 1], tg[kGeoC + 2]};
+    double a[9], b[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        a[k] = tg[k] - c[k % 3];
+        b[k] = sg[k] - c[k % 3];
+    }
+    int pi[3], pj[3];
+    ss_order(cls, en.w, pi, pj);
+    double P0[3], dP1[3], dP2[3], Q0[3], dQ1[3], dQ2[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        P0[k] = a[3 * pi[0] + k];
+        dP1[k] = a[3 * pi[1] + k] - P0[k];
+        dP2[k] = a[3 * pi[2] + k] - a[3 * pi[1] + k];
+        Q0[k] = b[3 * pj[0] + k];
+        dQ1[k] = b[3 * pj[1] + k] - Q0[k];
+        dQ2[k] = b[3 * pj[2] + k] - b[3 * pj[1] + k];
+    }
+    const int n = gl.n, n2 = n * n, n4 = n2 * n2;
+    const int nreg = ss_nregions(cls);
+    // moments: EFIE (M0, Mx, My, Mxy) / MFIE (Kc, Kr)
+    double m[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = 0.0;
+    for (int r = 0; r < nreg; ++r) {
+        for (int idx = lane; idx < n4; idx += 64) {
+            const int i0 = idx % n, i1 = (idx / n) % n, i2 = (idx / n2) % n, i3 = idx / (n2 * n);
+            const double w4 = gl.w[i0] * gl.w[i1] * gl.w[i2] * gl.w[i3];
+            double xh[2], yh[2];
+            const double J = ss_region(cls, r, gl.x[i0], gl.x[i1], gl.x[i2], gl.x[i3], xh, yh);
+            double x[3], y[3], rr[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                x[k] = P0[k] + xh[0] * dP1[k] + xh[1] * dP2[k];
+                y[k] = Q0[k] + yh[0] * dQ1[k] + yh[1] * dQ2[k];
+                rr[k] = x[k] - y[k];
+            }
+            const double R = sqrt(dot3(rr, rr));
+            const double iR = 1.0 / R;
+            const cx<double> G = green<double>(op.gr, op.gi, R, iR);
+            const double wj = w4 * J;
+            if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
+                const cx<double> wg{wj * G.r, wj * G.i};
+                const double xy = dot3(x, y);
+                m[0] += wg.r; m[1] += wg.i;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    m[2 + 2 * k] += wg.r * x[k]; m[3 + 2 * k] += wg.i * x[k];
+                    m[8 + 2 * k] += wg.r * y[k]; m[9 + 2 * k] += wg.i * y[k];
+                }
+                m[14] += wg.r * xy; m[15] += wg.i * xy;
+            } else {
+                // h = -(gamma + 1/R) G / R
+                const cx<double> gp{op.gr + iR, op.gi};
+                cx<double> h = cmul(gp, G);
+                h.r *= -iR * wj;
+                h.i *= -iR * wj;
+                double xy[3];
+                cross3(xy, x, y);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    m[2 * k] += h.r * xy[k]; m[2 * k + 1] += h.i * xy[k];
+                    m[6 + 2 * k] += h.r * rr[k]; m[7 + 2 * k] += h.i * rr[k];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = wave_sum(m[k]);
+    if (lane != 0) return;
+    cx<double> z[9];
+    if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
+        const cx<double> M0{m[0], m[1]};
+        const cx<double> Mx[3] = {{m[2], m[3]}, {m[4], m[5]}, {m[6], m[7]}};
+        const cx<double> My[3] = {{m[8], m[9]}, {m[10], m[11]}, {m[12], m[13]}};
+        const cx<double> Mxy{m[14], m[15]};
+        efie_block<double>(op, 1.0, a, b, M0, Mx, My, Mxy, z, 1);
+    } else {
+        const cx<double> Kc[3] = {{m[0], m[1]}, {m[2], m[3]}, {m[4], m[5]}};
+        const cx<double> Kr[3] = {{m[6], m[7]}, {m[8], m[9]}, {m[10], m[11]}};
+        mfie_block<double>(1.0, a, b, Kc, Kr, z, 1);
+    }
+    double *out = blocks + (size_t)e * 18;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { out[2 * k] = z[k].r; out[2 * k + 1] = z[k].i; }
+}
+
+// ============================================================ K3 Wilton near
+// Analytic integrals over the triangle (b0, b1, b2) seen from x (all in one
+// frame): S[-1,0,1] = int R^k, V[-1,0,1] = int (y - rho) R^k, G[-1] = int grad_y 1/R,
+// G[1] = int grad_y R; rho is the projection of x on the triangle plane.
+// Formulation: D. R. Wilton et al., IEEE TAP 32 (1984) 276-281 and R. D. Graglia,
+// IEEE TAP 41 (1993) 1448-1455: edge integrals I_e^k = int_e R^k dl and the
+// recursions sum_e p_e I_e^q = (q+2) int R^q - q d^2 int R^(q-2),
+// int (y-rho) R^q = 1/(q+2) sum_e m_e I_e^(q+2).
+struct WiltonOut {
+    double Sm1, S0, S1;
+    double Vm1[3], V0[3], V1[3];
+    double Gm1[3], G1[3];
+    double rho[3];
+};
+
+__device__ void wilton_integrals(const double *b, const double *x, double area, WiltonOut &o)
+{
+    double e13[3] = {b[0] - b[6], b[1] - b[7], b[2] - b[8]};
+    double e23[3] = {b[3] - b[6], b[4] - b[7], b[5] - b[8]};
+    double n[3];
+    cross3(n, e13, e23);
+    const double nn = sqrt(dot3(n, n));
+    n[0] /= nn; n[1] /= nn; n[2] /= nn;
+    const double xb[3] = {x[0] - b[0], x[1] - b[1], x[2] - b[2]};
+    const double d = dot3(xb, n), D = fabs(d);
+    for (int k = 0; k < 3; ++k) o.rho[k] = x[k] - d * n[k];
+    double Sm1 = 0, S1 = 0, Om = 0;
+    for (int k = 0; k < 3; ++k) { o.Vm1[k] = 0; o.V0[k] = 0; o.V1[k] = 0; o.Gm1[k] = 0; }
+    for (int e = 0; e < 3; ++e) {
+        const double *pa = b + 3 * e, *pb = b + 3 * ((e + 1) % 3);
+        double t[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+        const double l = sqrt(dot3(t, t));
+        t[0] /= l; t[1] /= l; t[2] /= l;
+        double m[3];
+        cross3(m, t, n);
+        const double ar[3] = {pa[0] - o.rho[0], pa[1] - o.rho[1], pa[2] - o.rho[2]};
+        const double sm = dot3(ar, t), sp = sm + l;
+        const double p = dot3(ar, m);
+        const double R02 = p * p + d * d;
+        const double Rm = sqrt(sm * sm + R02), Rp = sqrt(sp * sp + R02);
+        double Im1;
+        if (sm >= 0) Im1 = log((sp + Rp) / (sm + Rm));
+        else if (sp <= 0) Im1 = log((Rm - sm) / (Rp - sp));
+        else Im1 = log((sp + Rp) * (Rm - sm) / R02);
+        const double I1 = 0.5 * (sp * Rp - sm * Rm) + 0.5 * R02 * Im1;
+        const double I2 = (sp * sp * sp - sm * sm * sm) / 3.0 + R02 * (sp - sm);
+        const double I3 = 0.25 * (sp * Rp * Rp * Rp - sm * Rm * Rm * Rm) + 0.375 * R02 * (sp * Rp - sm * Rm) +
+                          0.375 * R02 * R02 * Im1;
+        const double np = p * sp, nm = p * sm;
+        const double bp = np == 0.0 ? 0.0 : atan(np / (R02 + D * Rp));
+        const double bm = nm == 0.0 ? 0.0 : atan(nm / (R02 + D * Rm));
+        Om += bp - bm;
+        Sm1 += p * Im1;
+        S1 += p * I1;
+        for (int k = 0; k < 3; ++k) {
+            o.Vm1[k] += m[k] * I1;
+            o.V0[k] += m[k] * (0.5 * I2);
+            o.V1[k] += m[k] * (I3 / 3.0);
+            o.Gm1[k] += m[k] * Im1;
+        }
+    }
+    Sm1 -= D * Om;
+    o.Sm1 = Sm1;
+    o.S0 = area;
+    o.S1 = (S1 + d * d * Sm1) / 3.0;
+    const double sgnOm = d > 0 ? Om : (d < 0 ? -Om : 0.0);
+    for (int k = 0; k < 3; ++k) {
+        o.Gm1[k] += n[k] * sgnOm;
+        o.G1[k] = o.Vm1[k] - d * n[k] * Sm1;
+    }
+}
+
+// exp(z) - 1 for complex z, accurate for small |z|.
+__device__ __forceinline__ cx<double> cexpm1(double a, double b)
+{
+    const double sb2 = sin(0.5 * b);
+    return {expm1(a) * cos(b) - 2.0 * sb2 * sb2, exp(a) * sin(b)};
+}
+
+template <int OPK>
+__global__ __launch_bounds__(256) void k_wilton(const double *__restrict__ tgeo, const double *__restrict__ sgeo,
+                                                const int4 *__restrict__ entries, const int *__restrict__ list,
+                                                int nlist, const double *__restrict__ tq, int Qo,
+                                                const double *__restrict__ sq, int Qi, OpParams op,
+                                                double *__restrict__ blocks)
+{
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= nlist) return;
+    const int lane = threadIdx.x & 63;
+    const int e = list[wv];
+    const int4 en = entries[e];
+    const double *tg = tgeo + (size_t)en.x * kGeo, *sg = sgeo + (size_t)en.y * kGeo;
+    const double c[3] = {tg[kGeoC], tg[kGeoC + 1], tg[kGeoC + 2]};
+    double a[9], b[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        a[k] = tg[k] - c[k % 3];
+        b[k] = sg[k] - c[k % 3];
+    }
+    const double dcs[3] = {sg[kGeoC] - c[0], sg[kGeoC + 1] - c[1], sg[kGeoC + 2] - c[2]};
+    const double At = tg[kGeoA], As = sg[kGeoA];
+    const double *xq = tq + (size_t)en.x * Qo * 4, *yq = sq + (size_t)en.y * Qi * 4;
+    double m[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = 0.0;
+    // regular part: double Gauss with G - (1/R - gamma + gamma^2 R / 2) / (4 pi)
+    for (int idx = lane; idx < Qo * Qi; idx += 64) {
+        const int p = idx / Qi, q = idx % Qi;
+        const double x[3] = {xq[4 * p], xq[4 * p + 1], xq[4 * p + 2]};
+        const double y[3] = {yq[4 * q] + dcs[0], yq[4 * q + 1] + dcs[1], yq[4 * q + 2] + dcs[2]};
+        const double w = xq[4 * p + 3] * yq[4 * q + 3];
+        const double rr[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+        const double R = sqrt(dot3(rr, rr));
+        const double gRr = op.gr * R, gRi = op.gi * R;
+        if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
+            // (expm1(-gR) + gR - (gR)^2/2) / (4 pi R)
+            const cx<double> em = cexpm1(-gRr, -gRi);
+            const double sq2r = gRr * gRr - gRi * gRi, sq2i = 2.0 * gRr * gRi;
+            const double f = w / (4.0 * kPi * R);
+            const cx<double> wg{(em.r + gRr - 0.5 * sq2r) * f, (em.i + gRi - 0.5 * sq2i) * f};
+            const double xy = dot3(x, y);
+            m[0] += wg.r; m[1] += wg.i;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                m[2 + 2 * k] += wg.r * x[k]; m[3 + 2 * k] += wg.i * x[k];
+                m[8 + 2 * k] += wg.r * y[k]; m[9 + 2 * k] += wg.i * y[k];
+            }
+            m[14] += wg.r * xy; m[15] += wg.i * xy;
+        } else {
+            // (-(gR + 1) exp(-gR) + 1 - (gR)^2/2) / (4 pi R^3)
+            double s, co;
+            sincos(gRi, &s, &co);
+            const double ea = exp(-gRr);
+            const cx<double> ex{ea * co, -ea * s};
+            const cx<double> t = cmul(cx<double>{gRr + 1.0, gRi}, ex);
+            const double sq2r = gRr * gRr - gRi * gRi, sq2i = 2.0 * gRr * gRi;
+            const double f = w / (4.0 * kPi * R * R * R);
+            const cx<double> h{(-t.r + 1.0 - 0.5 * sq2r) * f, (-t.i - 0.5 * sq2i) * f};
+            double xy[3];
+            cross3(xy, x, y);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                m[2 * k] += h.r * xy[k]; m[2 * k + 1] += h.i * xy[k];
+                m[6 + 2 * k] += h.r * rr[k]; m[7 + 2 * k] += h.i * rr[k];
+            }
+        }
+    }
+    // singular part: analytic inner integrals at each outer point
+    if (lane < Qo) {
+        const double x[3] = {xq[4 * lane], xq[4 * lane + 1], xq[4 * lane + 2]};
+        const double w = xq[4 * lane + 3];
+        WiltonOut o;
+        wilton_integrals(b, x, As, o);
+        const cx<double> g{op.gr, op.gi};
+        const cx<double> g2 = cmul(g, g);
+        if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
+            const double f = w / (4.0 * kPi);
+            const cx<double> IG{(o.Sm1 - g.r * o.S0 + 0.5 * g2.r * o.S1) * f, (-g.i * o.S0 + 0.5 * g2.i * o.S1) * f};
+            cx<double> Y[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Y[k].r = (o.Vm1[k] - g.r * o.V0[k] + 0.5 * g2.r * o.V1[k]) * f + o.rho[k] * IG.r;
+                Y[k].i = (-g.i * o.V0[k] + 0.5 * g2.i * o.V1[k]) * f + o.rho[k] * IG.i;
+            }
+            m[0] += IG.r; m[1] += IG.i;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                m[2 + 2 * k] += IG.r * x[k]; m[3 + 2 * k] += IG.i * x[k];
+                m[8 + 2 * k] += Y[k].r; m[9 + 2 * k] += Y[k].i;
+            }
+            m[14] += x[0] * Y[0].r + x[1] * Y[1].r + x[2] * Y[2].r;
+            m[15] += x[0] * Y[0].i + x[1] * Y[1].i + x[2] * Y[2].i;
+        } else {
+            const double f = w / (4.0 * kPi);
+            cx<double> IdG[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                IdG[k].r = (-o.Gm1[k] - 0.5 * g2.r * o.G1[k]) * f;
+                IdG[k].i = (-0.5 * g2.i * o.G1[k]) * f;
+            }
+            // Kc += IdG x x, Kr += IdG
+            const double cr[3] = {IdG[1].r * x[2] - IdG[2].r * x[1], IdG[2].r * x[0] - IdG[0].r * x[2],
+                                  IdG[0].r * x[1] - IdG[1].r * x[0]};
+            const double ci[3] = {IdG[1].i * x[2] - IdG[2].i * x[1], IdG[2].i * x[0] - IdG[0].i * x[2],
+                                  IdG[0].i * x[1] - IdG[1].i * x[0]};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                m[2 * k] += cr[k]; m[2 * k + 1] += ci[k];
+                m[6 + 2 * k] += IdG[k].r; m[7 + 2 * k] += IdG[k].i;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = wave_sum(m[k]);
+    if (lane != 0) return;
+    const double s1 = 1.0 / (4.0 * At * As);
+    cx<double> z[9];
+    if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
+        const cx<double> M0{m[0], m[1]};
+        const cx<double> Mx[3] = {{m[2], m[3]}, {m[4], m[5]}, {m[6], m[7]}};
+        const cx<double> My[3] = {{m[8], m[9]}, {m[10], m[11]}, {m[12], m[13]}};
+        const cx<double> Mxy{m[14], m[15]};
+        efie_block<double>(op, s1, a, b, M0, Mx, My, Mxy, z, 1);
+    } else {
+        const cx<double> Kc[3] = {{m[0], m[1]}, {m[2], m[3]}, {m[4], m[5]}};
+        const cx<double> Kr[3] = {{m[6], m[7]}, {m[8], m[9]}, {m[10], m[11]}};
+        mfie_block<double>(s1, a, b, Kc, Kr, z, 1);
+    }
+    double *out = blocks + (size_t)e * 18;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { out[2 * k] = z[k].r; out[2 * k + 1] = z[k].i; }
+}
+
+// ============================================================ K1 far field
+struct FarArgs {
+    const double *tgeo, *sgeo;
+    const void *tq, *sq;          // [nf][Q][4] in R: x', y', z' (rel. centroid), weight
+    int Qo, Qi;
+    int mode;
+    double k2;
+    const int *nptr;              // CSR of non-far pairs per test cell
+    const int4 *nent;
+    const double *nblk;           // 18 doubles per entry
+    OpParams op;
+};
+
+// Far-field 3x3 block of (test p, trial q) by double Gauss, written to z[k*stride].
+template <typename R, int OPK>
+__device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> *z, int stride)
+{
+    const double *tg = A.tgeo + (size_t)p * kGeo, *sg = A.sgeo + (size_t)q * kGeo;
+    if (A.mode != kModeDoubleNum) {
+        int pk;
+        const int cls = classify_pair(tg, sg, sg[kGeoA], A.k2, A.mode, &pk);
+        if (cls != kFar) {
+            const int lo = A.nptr[p], hi = A.nptr[p + 1];
+            int hit = -1;
+            for (int e = lo; e < hi; ++e)
+                if (A.nent[e].y == q) { hit = e; break; }
+            if (hit < 0) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) z[k * stride] = {(R)0, (R)0};
+            } else {
+                const double *blk = A.nblk + (size_t)hit * 18;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) z[k * stride] = {(R)blk[2 * k], (R)blk[2 * k + 1]};
+            }
+            return;
+        }
+    }
+    R a[9], b[9], dc[3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        a[k] = (R)(tg[k] - tg[kGeoC + k % 3]);
+        b[k] = (R)(sg[k] - sg[kGeoC + k % 3]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dc[k] = (R)(tg[kGeoC + k] - sg[kGeoC + k]);
+    const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
+    const R gr = (R)A.op.gr, gi = (R)A.op.gi;
+    const R *xq = (const R *)A.tq + (size_t)p * A.Qo * 4;
+    const R *yq = (const R *)A.sq + (size_t)q * A.Qi * 4;
+    if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
+        cx<R> T0{0, 0}, T1{0, 0}, Tx[3] = {{0, 0}, {0, 0}, {0, 0}}, Ty[3] = {{0, 0}, {0, 0}, {0, 0}};
+        for (int ip = 0; ip < A.Qo; ++ip) {
+            const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
+            const R wp = xq[4 * ip + 3];
+            const R xd[3] = {x[0] + dc[0], x[1] + dc[1], x[2] + dc[2]};
+            cx<R> S0{0, 0}, S1[3] = {{0, 0}, {0, 0}, {0, 0}};
+            for (int iq = 0; iq < A.Qi; ++iq) {
+                const R y[3] = {yq[4 * iq], yq[4 * iq + 1], yq[4 * iq + 2]};
+                const R wq = yq[4 * iq + 3];
+                const R r[3] = {xd[0] - y[0], xd[1] - y[1], xd[2] - y[2]};
+                const R R2 = dot3(r, r);
+                R dist, idist;
+                if constexpr (sizeof(R) == 8) { dist = sqrt(R2); idist = 1.0 / dist; }
+                else { idist = rsqrtf(R2); dist = R2 * idist; }
+                const cx<R> G = green<R>(gr, gi, dist, idist);
+                const cx<R> wg{wq * G.r, wq * G.i};
+                S0.r += wg.r; S0.i += wg.i;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) { S1[k].r += wg.r * y[k]; S1[k].i += wg.i * y[k]; }
+            }
+            T0.r += wp * S0.r; T0.i += wp * S0.i;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Tx[k].r += wp * x[k] * S0.r; Tx[k].i += wp * x[k] * S0.i;
+                Ty[k].r += wp * S1[k].r; Ty[k].i += wp * S1[k].i;
+            }
+            T1.r += wp * (x[0] * S1[0].r + x[1] * S1[1].r + x[2] * S1[2].r);
+            T1.i += wp * (x[0] * S1[0].i + x[1] * S1[1].i + x[2] * S1[2].i);
+        }
+        efie_block<R>(A.op, s1, a, b, T0, Tx, Ty, T1, z, stride);
+    } else {
+        // M1 = sum w x.U, MU = sum w U, MX = sum w x x Rv, MR = sum w Rv with
+        // U = xd x H1, Rv = xd H0 - H1, H0 = sum w h, H1 = sum w h y, h = -(gamma + 1/R) G / R
+        cx<R> M1{0, 0}, MU[3] = {{0, 0}, {0, 0}, {0, 0}}, MX[3] = {{0, 0}, {0, 0}, {0, 0}},
+                        MR[3] = {{0, 0}, {0, 0}, {0, 0}};
+        for (int ip = 0; ip < A.Qo; ++ip) {
+            const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
+            const R wp = xq[4 * ip + 3];
+            const R xd[3] = {x[0] + dc[0], x[1] + dc[1], x[2] + dc[2]};
+            cx<R> H0{0, 0}, H1[3] = {{0, 0}, {0, 0}, {0, 0}};
+            for (int iq = 0; iq < A.Qi; ++iq) {
+                const R y[3] = {yq[4 * iq], yq[4 * iq + 1], yq[4 * iq + 2]};
+                const R wq = yq[4 * iq + 3];
+                const R r[3] = {xd[0] - y[0], xd[1] - y[1], xd[2] - y[2]};
+                const R R2 = dot3(r, r);
+                R dist, idist;
+                if constexpr (sizeof(R) == 8) { dist = sqrt(R2); idist = 1.0 / dist; }
+                else { idist = rsqrtf(R2); dist = R2 * idist; }
+                const cx<R> G = green<R>(gr, gi, dist, idist);
+                cx<R> h = cmul(cx<R>{gr + idist, gi}, G);
+                const R f = -idist * wq;
+                h.r *= f; h.i *= f;
+                H0.r += h.r; H0.i += h.i;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) { H1[k].r += h.r * y[k]; H1[k].i += h.i * y[k]; }
+            }
+            cx<R> U[3], Rv[3];
+            U[0] = {xd[1] * H1[2].r - xd[2] * H1[1].r, xd[1] * H1[2].i - xd[2] * H1[1].i};
+            U[1] = {xd[2] * H1[0].r - xd[0] * H1[2].r, xd[2] * H1[0].i - xd[0] * H1[2].i};
+            U[2] = {xd[0] * H1[1].r - xd[1] * H1[0].r, xd[0] * H1[1].i - xd[1] * H1[0].i};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) Rv[k] = {xd[k] * H0.r - H1[k].r, xd[k] * H0.i - H1[k].i};
+            M1.r += wp * (x[0] * U[0].r + x[1] * U[1].r + x[2] * U[2].r);
+            M1.i += wp * (x[0] * U[0].i + x[1] * U[1].i + x[2] * U[2].i);
+            const cx<R> XR[3] = {{x[1] * Rv[2].r - x[2] * Rv[1].r, x[1] * Rv[2].i - x[2] * Rv[1].i},
+                                 {x[2] * Rv[0].r - x[0] * Rv[2].r, x[2] * Rv[0].i - x[0] * Rv[2].i},
+                                 {x[0] * Rv[1].r - x[1] * Rv[0].r, x[0] * Rv[1].i - x[1] * Rv[0].i}};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                MU[k].r += wp * U[k].r; MU[k].i += wp * U[k].i;
+                MX[k].r += wp * XR[k].r; MX[k].i += wp * XR[k].i;
+                MR[k].r += wp * Rv[k].r; MR[k].i += wp * Rv[k].i;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const R *ai = a + 3 * i, *bj = b + 3 * j;
+                R c[3];
+                cross3(c, bj, ai);
+                cx<R> t;
+                t.r = M1.r - (ai[0] * MU[0].r + ai[1] * MU[1].r + ai[2] * MU[2].r) -
+                      (bj[0] * MX[0].r + bj[1] * MX[1].r + bj[2] * MX[2].r) +
+                      (c[0] * MR[0].r + c[1] * MR[1].r + c[2] * MR[2].r);
+                t.i = M1.i - (ai[0] * MU[0].i + ai[1] * MU[1].i + ai[2] * MU[2].i) -
+                      (bj[0] * MX[0].i + bj[1] * MX[1].i + bj[2] * MX[2].i) +
+                      (c[0] * MR[0].i + c[1] * MR[1].i + c[2] * MR[2].i);
+                z[(i + 3 * j) * stride] = {s1 * t.r, s1 * t.i};
+            }
+    }
+}
+
+constexpr int kTile = 64;          // dofs per tile side
+constexpr int kTileCells = 128;    // max cells per tile side
+constexpr int kPairChunk = 256;    // pair blocks staged in LDS at once
+
+struct TileSide {
+    const int *ptr;       // [ntiles+1] into cells
+    const int *cells;     // global cell ids
+    const int *slot;      // [ndofs][2]: (local cell << 2 | refid), -1 none
+    const double *coef;   // [ndofs][2]
+    int ndofs;
+};
+
+struct TileArgs {
+    FarArgs far;
+    TileSide row, col;
+    void *Z;              // output, complex R
+    long long ld;
+    int rowmajor;
+    int accumulate;
+    double scale_r, scale_i;
+};
+
+template <typename R, int OPK, bool ROWMAJOR>
+__global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
+{
+    __shared__ int s_rc[kTileCells], s_cc[kTileCells];
+    __shared__ cx<R> s_blk[kPairChunk * 9];
+    const int tr = blockIdx.y, tc = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int r0 = A.row.ptr[tr], nr = A.row.ptr[tr + 1] - r0;
+    const int c0 = A.col.ptr[tc], nc = A.col.ptr[tc + 1] - c0;
+    for (int k = tid; k < nr; k += 256) s_rc[k] = A.row.cells[r0 + k];
+    for (int k = tid; k < nc; k += 256) s_cc[k] = A.col.cells[c0 + k];
+
+    // ownership: ROWMAJOR -> one column, 16 rows; else one row, 16 columns
+    const int own = tid & 63, grp = tid >> 6;
+    int rs[ROWMAJOR ? 32 : 2], cs_[ROWMAJOR ? 2 : 32];
+    R rcf[ROWMAJOR ? 32 : 2], ccf[ROWMAJOR ? 2 : 32];
+    const int mbase = tr * kTile, nbase = tc * kTile;
+    if constexpr (ROWMAJOR) {
+        const int n = nbase + own;
+        for (int s = 0; s < 2; ++s) {
+            cs_[s] = n < A.col.ndofs ? A.col.slot[2 * n + s] : -1;
+            ccf[s] = n < A.col.ndofs ? (R)A.col.coef[2 * n + s] : (R)0;
+        }
+        for (int e = 0; e < 16; ++e) {
+            const int m = mbase + grp * 16 + e;
+            for (int s = 0; s < 2; ++s) {
+                rs[2 * e + s] = m < A.row.ndofs ? A.row.slot[2 * m + s] : -1;
+                rcf[2 * e + s] = m < A.row.ndofs ? (R)A.row.coef[2 * m + s] : (R)0;
+            }
+        }
+    } else {
+        const int m = mbase + own;
+        for (int s = 0; s < 2; ++s) {
+            rs[s] = m < A.row.ndofs ? A.row.slot[2 * m + s] : -1;
+            rcf[s] = m < A.row.ndofs ? (R)A.row.coef[2 * m + s] : (R)0;
+        }
+        for (int e = 0; e < 16; ++e) {
+            const int n = nbase + grp * 16 + e;
+            for (int s = 0; s < 2; ++s) {
+                cs_[2 * e + s] = n < A.col.ndofs ? A.col.slot[2 * n + s] : -1;
+                ccf[2 * e + s] = n < A.col.ndofs ? (R)A.col.coef[2 * n + s] : (R)0;
+            }
+        }
+    }
+    cx<R> acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = {(R)0, (R)0};
+    __syncthreads();
+
+    const int CC = nr > 0 ? max(1, kPairChunk / nr) : 1;
+    for (int cb = 0; cb < nc; cb += CC) {
+        const int cn = min(CC, nc - cb);
+        // phase 1: pair blocks of (all row cells) x (column cells cb .. cb+cn)
+        if (tid < nr * cn) {
+            const int rc = tid % nr, cc = tid / nr;
+            far_block<R, OPK>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + tid, kPairChunk);
+        }
+        __syncthreads();
+        // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int *cslot = ROWMAJOR ? cs_ : cs_ + 2 * e;
+            const R *cco = ROWMAJOR ? ccf : ccf + 2 * e;
+            const int *rslot = ROWMAJOR ? rs + 2 * e : rs;
+            const R *rco = ROWMAJOR ? rcf + 2 * e : rcf;
+#pragma unroll
+            for (int sb = 0; sb < 2; ++sb) {
+                const int cslt = cslot[sb];
+                if (cslt < 0) continue;
+                const int lc = (cslt >> 2) - cb;
+                if (lc < 0 || lc >= cn) continue;
+                const int j = cslt & 3;
+                const R bc = cco[sb];
+#pragma unroll
+                for (int sa = 0; sa < 2; ++sa) {
+                    const int rslt = rslot[sa];
+                    if (rslt < 0) continue;
+                    const int i = rslt & 3;
+                    const cx<R> zz = s_blk[(lc * nr + (rslt >> 2)) + (i + 3 * j) * kPairChunk];
+                    const R ac = rco[sa];
+                    acc[e].r += ac * (zz.r * bc);
+                    acc[e].i += ac * (zz.i * bc);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // write once
+    cx<R> *Z = (cx<R> *)A.Z;
+    const R sr = (R)A.scale_r, si = (R)A.scale_i;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        int m, n;
+        if constexpr (ROWMAJOR) { m = mbase + grp * 16 + e; n = nbase + own; }
+        else { m = mbase + own; n = nbase + grp * 16 + e; }
+        if (m >= A.row.ndofs || n >= A.col.ndofs) continue;
+        const size_t idx = ROWMAJOR ? (size_t)m * A.ld + n : (size_t)m + (size_t)n * A.ld;
+        cx<R> v{sr * acc[e].r - si * acc[e].i, sr * acc[e].i + si * acc[e].r};
+        if (A.accumulate) { v.r += Z[idx].r; v.i += Z[idx].i; }
+        Z[idx] = v;
+    }
+}
+
+// Far blocks for an explicit pair list (inspection entry).
+template <typename R, int OPK>
+__global__ void k_far_pairs(FarArgs A, const int *tc, const int *sc, int n, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    cx<R> z[9];
+    far_block<R, OPK>(A, tc[k], sc[k], z, 1);
+    for (int t = 0; t < 9; ++t) { out[18 * (size_t)k + 2 * t] = (double)z[t].r; out[18 * (size_t)k + 2 * t + 1] = (double)z[t].i; }
+}
+
+// ============================================================ host side
+namespace {
+
+struct Strategy {
+    int mode;
+    int far_o, far_i, near_o, near_i;
+    int ss_face, ss_edge, ss_vert;
+};
+
+Strategy parse_qs(const bh_quadstrat *qs)
+{
+    BH_REQUIRE(qs != nullptr, "quadstrat is NULL");
+    Strategy s{};
+    const int *p = qs->params;
+    auto tri_ok = [](int d) { return d >= 1 && d <= 8; };
+    auto gl_ok = [](int n) { return n >= 1 && n <= kMaxGL; };
+    switch (qs->kind) {
+    case BH_QS_WILTON_SAUTER:
+        s = {kModeWiltonSauter, p[0], p[1], p[2], p[3], p[5], p[6], p[7]};
+        BH_REQUIRE(tri_ok(p[0]) && tri_ok(p[1]) && tri_ok(p[2]) && tri_ok(p[3]),
+                   "DoubleNumWiltonSauterQStrat: triangle rules must be degrees 1..8");
+        BH_REQUIRE(gl_ok(p[5]) && gl_ok(p[6]) && gl_ok(p[7]), "Sauter-Schwab orders must be 1..%d", kMaxGL);
+        break;
+    case BH_QS_SAUTER:
+        s = {kModeSauter, p[0], p[1], 0, 0, p[3], p[4], p[5]};
+        BH_REQUIRE(tri_ok(p[0]) && tri_ok(p[1]), "DoubleNumSauterQstrat: triangle rules must be degrees 1..8");
+        BH_REQUIRE(gl_ok(p[3]) && gl_ok(p[4]) && gl_ok(p[5]), "Sauter-Schwab orders must be 1..%d", kMaxGL);
+        break;
+    case BH_QS_DOUBLENUM:
+        s = {kModeDoubleNum, p[0], p[1], 0, 0, 0, 0, 0};
+        BH_REQUIRE(tri_ok(p[0]) && tri_ok(p[1]), "DoubleNumQStrat: triangle rules must be degrees 1..8");
+        break;
+    default:
+        BH_FAIL(BH_ERR_UNSUPPORTED, "quadrature strategy kind %d is not supported on the GPU path", qs->kind);
+    }
+    return s;
+}
+
+OpParams parse_op(const bh_operator *op)
+{
+    BH_REQUIRE(op != nullptr, "operator is NULL");
+    BH_REQUIRE(op->kind == BH_OP_MW_SINGLELAYER || op->kind == BH_OP_MW_DOUBLELAYER,
+               "operator kind %d is not supported", op->kind);
+    OpParams p{op->kind, op->gamma[0], op->gamma[1], op->alpha[0], op->alpha[1], op->beta[0], op->beta[1]};
+    for (double v : {p.gr, p.gi, p.ar, p.ai, p.br, p.bi})
+        BH_REQUIRE(std::isfinite(v), "operator parameters must be finite");
+    return p;
+}
+
+}  // namespace
+
+// Quadrature points of a rule on every cell of a mesh (cached per degree):
+// [nf][Q][4] = x - centroid (3), weight * 2 * area.  Points follow
+// quadpoints(refspace, charts, rule): x = u v1 + v v2 + (1 - u - v) v3.
+const void *quad_points(Mesh &m, int degree, int *Q, bool fp32, hipStream_t s)
+{
+    double u[kMaxTriPts], v[kMaxTriPts], w[kMaxTriPts];
+    const int nq = triangle_rule(degree, u, v, w);
+    BH_REQUIRE(nq > 0, "no triangle rule of degree %d", degree);
+    *Q = nq;
+    auto &cache64 = m.quad64[degree];
+    if (cache64.n == 0 && m.nf > 0) {
+        std::vector<double> h((size_t)m.nf * nq * 4);
+        for (int64_t c = 0; c < m.nf; ++c) {
+            const double *g = &m.geo[(size_t)c * kGeo];
+            for (int q = 0; q < nq; ++q) {
+                double *o = &h[((size_t)c * nq + q) * 4];
+                for (int k = 0; k < 3; ++k) {
+                    const double x = u[q] * g[k] + v[q] * g[3 + k] + (1.0 - u[q] - v[q]) * g[6 + k];
+                    o[k] = x - g[kGeoC + k];
+                }
+                o[3] = w[q] * 2.0 * g[kGeoA];
+            }
+        }
+        cache64.upload(h.data(), h.size(), s);
+        std::vector<float> hf(h.begin(), h.end());
+        m.quad32[degree].upload(hf.data(), hf.size(), s);
+        BH_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    return fp32 ? (const void *)m.quad32[degree].p : (const void *)cache64.p;
+}
+
+namespace {
+
+struct NearData {
+    DevBuf<int> nptr;
+    DevBuf<int4> entries;
+    DevBuf<double> blocks;
+    int nent = 0;
+    int count[5] = {0, 0, 0, 0, 0};
+};
+
+void check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) BH_FAIL(BH_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+}
+
+// K0 + K2 + K3: classification, compaction and the non-far blocks.
+void build_near(Ctx &ctx, Mesh &tm, Mesh &sm, const Strategy &st, const OpParams &op, NearData &nd)
+{
+    hipStream_t s = ctx.stream;
+    const int nft = (int)tm.nf, nfs = (int)sm.nf;
+    nd.nptr.alloc(nft + 1);
+    std::vector<int> off(nft + 1, 0);
+    if (st.mode == kModeDoubleNum || nft == 0 || nfs == 0) {
+        nd.nptr.upload(off.data(), off.size(), s);
+        nd.nent = 0;
+        return;
+    }
+    const double k2 = op.gr * op.gr + op.gi * op.gi;
+    DevBuf<int> counts(nft), err(1);
+    BH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    k_classify_count<<<nft, 256, 0, s>>>(tm.d_geo.p, nft, sm.d_geo.p, nfs, k2, st.mode, counts.p, err.p);
+    check_launch("k_classify_count");
+    std::vector<int> hc(nft);
+    int herr = 0;
+    BH_CHECK_HIP(hipMemcpyAsync(hc.data(), counts.p, sizeof(int) * nft, hipMemcpyDeviceToHost, s));
+    BH_CHECK_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    BH_CHECK_HIP(hipStreamSynchronize(s));
+    if (herr) BH_FAIL(BH_ERR_ASSERT, "AssertionError: hits <= 3 (two cells share more than 3 vertex positions)");
+    for (int p = 0; p < nft; ++p) off[p + 1] = off[p] + hc[p];
+    nd.nent = off[nft];
+    nd.nptr.upload(off.data(), off.size(), s);
+    if (nd.nent == 0) return;
+    nd.entries.alloc(nd.nent);
+    k_classify_fill<<<nft, 256, 0, s>>>(tm.d_geo.p, nft, sm.d_geo.p, nfs, k2, st.mode, nd.nptr.p, nd.entries.p);
+    check_launch("k_classify_fill");
+    DevBuf<int> counters(5), lists((size_t)5 * nd.nent);
+    BH_CHECK_HIP(hipMemsetAsync(counters.p, 0, 5 * sizeof(int), s));
+    k_class_lists<<<(nd.nent + 255) / 256, 256, 0, s>>>(nd.entries.p, nd.nent, counters.p, lists.p, nd.nent);
+    check_launch("k_class_lists");
+    BH_CHECK_HIP(hipMemcpyAsync(nd.count, counters.p, 5 * sizeof(int), hipMemcpyDeviceToHost, s));
+    BH_CHECK_HIP(hipStreamSynchronize(s));
+    nd.blocks.alloc((size_t)nd.nent * 18);
+    // Sauter-Schwab, one launch per class
+    const int cls_list[3] = {kCF, kCE, kCV};
+    for (int ci = 0; ci < 3; ++ci) {
+        const int cls = cls_list[ci];
+        const int cnt = nd.count[cls];
+        if (cnt == 0) continue;
+        GLRule gl{};
+        gl.n = cls == kCF ? st.ss_face : (cls == kCE ? st.ss_edge : st.ss_vert);
+        legendre01(gl.n, gl.x, gl.w);
+        const int grid = (cnt + 3) / 4;
+        const int *lp = lists.p + (size_t)cls * nd.nent;
+        if (op.kind == BH_OP_MW_SINGLELAYER)
+            k_sauter_schwab<BH_OP_MW_SINGLELAYER><<<grid, 256, 0, s>>>(tm.d_geo.p, sm.d_geo.p, nd.entries.p, lp, cnt,
+                                                                       cls, gl, op, nd.blocks.p);
+        else
+            k_sauter_schwab<BH_OP_MW_DOUBLELAYER><<<grid, 256, 0, s>>>(tm.d_geo.p, sm.d_geo.p, nd.entries.p, lp, cnt,
+                                                                       cls, gl, op, nd.blocks.p);
+        check_launch("k_sauter_schwab");
+    }
+    const int cnt = nd.count[kNear];
+    if (cnt > 0) {
+        int Qo = 0, Qi = 0;
+        const double *tq = (const double *)quad_points(tm, st.near_o, &Qo, false, s);
+        const double *sq = (const double *)quad_points(sm, st.near_i, &Qi, false, s);
+        BH_REQUIRE(Qo <= 64, "near outer rule too large");
+        const int grid = (cnt + 3) / 4;
+        const int *lp = lists.p + (size_t)kNear * nd.nent;
+        if (op.kind == BH_OP_MW_SINGLELAYER)
+            k_wilton<BH_OP_MW_SINGLELAYER><<<grid, 256, 0, s>>>(tm.d_geo.p, sm.d_geo.p, nd.entries.p, lp, cnt, tq, Qo,
+                                                                sq, Qi, op, nd.blocks.p);
+        else
+            k_wilton<BH_OP_MW_DOUBLELAYER><<<grid, 256, 0, s>>>(tm.d_geo.p, sm.d_geo.p, nd.entries.p, lp, cnt, tq, Qo,
+                                                                sq, Qi, op, nd.blocks.p);
+        check_launch("k_wilton");
+    }
+    BH_CHECK_HIP(hipStreamSynchronize(s));  // lists/counters go out of scope
+}
+
+// Dof tiles of a space: for each group of kTile consecutive dofs, the sorted
+// cells their shapes live on, and per dof up to two (local cell, refid, coeff).
+void build_tiles(Space &sp, hipStream_t s)
+{
+    if (sp.tiles_built) return;
+    const int64_t n = sp.ndofs;
+    const int64_t nt = (n + kTile - 1) / kTile;
+    std::vector<int> ptr(nt + 1, 0), cells, slot(2 * n, -1);
+    std::vector<double> coef(2 * n, 0.0);
+    for (int64_t t = 0; t < nt; ++t) {
+        const int64_t lo = t * kTile, hi = std::min(n, lo + kTile);
+        std::vector<int> tc;
+        for (int64_t m = lo; m < hi; ++m) {
+            const int ns = sp.ptr[m + 1] - sp.ptr[m];
+            if (ns > 2) BH_FAIL(BH_ERR_UNSUPPORTED, "dof %lld has %d shapes; the GPU path supports at most 2",
+                                (long long)m, ns);
+            for (int k = sp.ptr[m]; k < sp.ptr[m + 1]; ++k) tc.push_back(sp.cell[k]);
+        }
+        std::sort(tc.begin(), tc.end());
+        tc.erase(std::unique(tc.begin(), tc.end()), tc.end());
+        if ((int)tc.size() > kTileCells) BH_FAIL(BH_ERR_UNSUPPORTED, "dof tile touches %d cells (max %d)",
+                                                 (int)tc.size(), kTileCells);
+        for (int64_t m = lo; m < hi; ++m) {
+            int s2 = 0;
+            for (int k = sp.ptr[m]; k < sp.ptr[m + 1]; ++k) {
+                const int lc = (int)(std::lower_bound(tc.begin(), tc.end(), sp.cell[k]) - tc.begin());
+                slot[2 * m + s2] = (lc << 2) | sp.ref[k];
+                coef[2 * m + s2] = sp.coeff[k];
+                ++s2;
+            }
+        }
+        cells.insert(cells.end(), tc.begin(), tc.end());
+        ptr[t + 1] = (int)cells.size();
+    }
+    sp.t_ptr.upload(ptr.data(), ptr.size(), s);
+    sp.t_cells.upload(cells.data(), cells.size(), s);
+    sp.t_slot.upload(slot.data(), slot.size(), s);
+    sp.t_coef.upload(coef.data(), coef.size(), s);
+    sp.ntiles = nt;
+    BH_CHECK_HIP(hipStreamSynchronize(s));
+    sp.tiles_built = true;
+}
+
+struct Plan {
+    Strategy st;
+    OpParams op;
+    NearData nd;
+    FarArgs far;
+};
+
+void make_plan(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, Mesh &tm, Mesh &sm, bool fp32, Plan &P)
+{
+    P.st = parse_qs(qs);
+    P.op = parse_op(op);
+    build_near(ctx, tm, sm, P.st, P.op, P.nd);
+    FarArgs &F = P.far;
+    F.tgeo = tm.d_geo.p;
+    F.sgeo = sm.d_geo.p;
+    F.tq = quad_points(tm, P.st.far_o, &F.Qo, fp32, ctx.stream);
+    F.sq = quad_points(sm, P.st.far_i, &F.Qi, fp32, ctx.stream);
+    F.mode = P.st.mode;
+    F.k2 = P.op.gr * P.op.gr + P.op.gi * P.op.gi;
+    F.nptr = P.nd.nptr.p;
+    F.nent = P.nd.entries.p;
+    F.nblk = P.nd.blocks.p;
+    F.op = P.op;
+}
+
+template <typename R>
+void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long long ld, bool rowmajor, bool accumulate,
+                  double sr, double si)
+{
+    TileArgs A{};
+    A.far = P.far;
+    A.row = {X.t_ptr.p, X.t_cells.p, X.t_slot.p, X.t_coef.p, (int)X.ndofs};
+    A.col = {Y.t_ptr.p, Y.t_cells.p, Y.t_slot.p, Y.t_coef.p, (int)Y.ndofs};
+    A.Z = Z;
+    A.ld = ld;
+    A.rowmajor = rowmajor;
+    A.accumulate = accumulate;
+    A.scale_r = sr;
+    A.scale_i = si;
+    BH_REQUIRE(X.ntiles <= 65535 && Y.ntiles <= 65535, "too many dof tiles for one launch");
+    dim3 grid((unsigned)Y.ntiles, (unsigned)X.ntiles);
+    const bool sl = P.op.kind == BH_OP_MW_SINGLELAYER;
+    if (rowmajor) {
+        if (sl) k_far_tiles<R, BH_OP_MW_SINGLELAYER, true><<<grid, 256, 0, ctx.stream>>>(A);
+        else k_far_tiles<R, BH_OP_MW_DOUBLELAYER, true><<<grid, 256, 0, ctx.stream>>>(A);
+    } else {
+        if (sl) k_far_tiles<R, BH_OP_MW_SINGLELAYER, false><<<grid, 256, 0, ctx.stream>>>(A);
+        else k_far_tiles<R, BH_OP_MW_DOUBLELAYER, false><<<grid, 256, 0, ctx.stream>>>(A);
+    }
+    check_launch("k_far_tiles");
+}
+
+}  // namespace
+
+// Assemble into a device buffer (column-major or row-major, ld given).
+void assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ,
+                     long long ld, bool rowmajor, bool accumulate, double sr, double si)
+{
+    Plan P;
+    make_plan(ctx, op, qs, *X.mesh, *Y.mesh, fp32, P);
+    build_tiles(X, ctx.stream);
+    build_tiles(Y, ctx.stream);
+    if (X.ndofs == 0 || Y.ndofs == 0) return;
+    if (fp32) launch_tiles<float>(ctx, P, X, Y, dZ, ld, rowmajor, accumulate, sr, si);
+    else launch_tiles<double>(ctx, P, X, Y, dZ, ld, rowmajor, accumulate, sr, si);
+    BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
+}
+
+}  // namespace bh
+
+using namespace bh;
+
+extern "C" int bh_assemble(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, int precision, const bh_space *X,
+                           const bh_space *Y, const double *scale, int accumulate, void *Z, int64_t ld,
+                           int64_t row_off, int64_t col_off)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctx && X && Y, "NULL handle");
+    BH_REQUIRE(precision == BH_FP64 || precision == BH_FP32, "precision must be BH_FP64 or BH_FP32");
+    BH_REQUIRE(X->mesh->ctx == ctx && Y->mesh->ctx == ctx, "spaces belong to another context");
+    const int64_t M = X->ndofs, N = Y->ndofs;
+    if (M == 0 || N == 0) return BH_OK;   // integralop.jl:72-73
+    BH_REQUIRE(Z != nullptr, "Z is NULL");
+    BH_REQUIRE(row_off >= 0 && col_off >= 0 && ld >= row_off + M, "ld/offsets do not fit the %lld x %lld block",
+               (long long)M, (long long)N);
+    const bool fp32 = precision == BH_FP32;
+    const size_t es = fp32 ? 8 : 16;
+    DevBuf<char> dZ((size_t)M * N * es);
+    char *hz = (char *)Z + (size_t)(row_off + col_off * ld) * es;
+    if (accumulate)
+        BH_CHECK_HIP(hipMemcpy2DAsync(dZ.p, M * es, hz, ld * es, M * es, N, hipMemcpyHostToDevice, ctx->stream));
+    const double sr = scale ? scale[0] : 1.0, si = scale ? scale[1] : 0.0;
+    assemble_device(*ctx, op, qs, fp32, *const_cast<bh_space *>(X), *const_cast<bh_space *>(Y), dZ.p, M, false,
+                    accumulate != 0, sr, si);
+    BH_CHECK_HIP(hipMemcpy2DAsync(hz, ld * es, dZ.p, M * es, M * es, N, hipMemcpyDeviceToHost, ctx->stream));
+    BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    BH_API_END
+}
+
+extern "C" int bh_classify(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, const bh_mesh *test,
+                           const bh_mesh *trial, int8_t *classes)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctx && test && trial && classes, "NULL argument");
+    const Strategy st = parse_qs(qs);
+    const OpParams p = parse_op(op);
+    const size_t n = (size_t)test->nf * trial->nf;
+    if (n == 0) return BH_OK;
+    DevBuf<int8_t> d(n);
+    const double k2 = p.gr * p.gr + p.gi * p.gi;
+    k_classify_matrix<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(test->d_geo.p, (int)test->nf,
+                                                                           trial->d_geo.p, (int)trial->nf, k2,
+                                                                           st.mode, d.p);
+    check_launch("k_classify_matrix");
+    BH_CHECK_HIP(hipMemcpyAsync(classes, d.p, n, hipMemcpyDeviceToHost, ctx->stream));
+    BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    BH_API_END
+}
+
+extern "C" int bh_local_blocks(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, const bh_mesh *test,
+                               const bh_mesh *trial, int64_t npairs, const int32_t *test_cells,
+                               const int32_t *trial_cells, double *blocks)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctx && test && trial && blocks, "NULL argument");
+    if (npairs <= 0) return BH_OK;
+    for (int64_t k = 0; k < npairs; ++k)
+        BH_REQUIRE(test_cells[k] >= 0 && test_cells[k] < test->nf && trial_cells[k] >= 0 && trial_cells[k] < trial->nf,
+                   "pair %lld out of range", (long long)k);
+    Plan P;
+    make_plan(*ctx, op, qs, *const_cast<bh_mesh *>(test), *const_cast<bh_mesh *>(trial), false, P);
+    DevBuf<int> tc(npairs), sc(npairs);
+    DevBuf<double> out((size_t)npairs * 18);
+    tc.upload(test_cells, npairs, ctx->stream);
+    sc.upload(trial_cells, npairs, ctx->stream);
+    const unsigned grid = (unsigned)((npairs + 127) / 128);
+    if (P.op.kind == BH_OP_MW_SINGLELAYER)
+        k_far_pairs<double, BH_OP_MW_SINGLELAYER><<<grid, 128, 0, ctx->stream>>>(P.far, tc.p, sc.p, (int)npairs, out.p);
+    else
+        k_far_pairs<double, BH_OP_MW_DOUBLELAYER><<<grid, 128, 0, ctx->stream>>>(P.far, tc.p, sc.p, (int)npairs, out.p);
+    check_launch("k_far_pairs");
+    BH_CHECK_HIP(hipMemcpyAsync(blocks, out.p, sizeof(double) * 18 * npairs, hipMemcpyDeviceToHost, ctx->stream));
+    BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    BH_API_END
+}

@@ -1,0 +1,157 @@
+// Internal declarations shared by the HIP sources of libbeast_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/beast_hip.h"
+
+namespace bh {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+const char *get_error();
+
+struct Error {
+    int code;
+};
+
+#define BH_CHECK_HIP(expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            ::bh::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr,               \
+                            hipGetErrorString(e_));                                          \
+            throw ::bh::Error{BH_ERR_HIP};                                                   \
+        }                                                                                    \
+    } while (0)
+
+#define BH_FAIL(code, ...)                                                                   \
+    do {                                                                                     \
+        ::bh::set_error(__VA_ARGS__);                                                        \
+        throw ::bh::Error{code};                                                             \
+    } while (0)
+
+#define BH_API_BEGIN try {
+#define BH_API_END                                                                           \
+    }                                                                                        \
+    catch (const ::bh::Error &e_) { return e_.code; }                                        \
+    catch (const std::bad_alloc &) {                                                         \
+        ::bh::set_error("host allocation failed");                                           \
+        return BH_ERR_OOM;                                                                   \
+    }                                                                                        \
+    catch (...) {                                                                            \
+        ::bh::set_error("unexpected exception");                                             \
+        return BH_ERR_INVALID;                                                               \
+    }                                                                                        \
+    return BH_OK;
+
+#define BH_REQUIRE(cond, ...)                                                                \
+    do {                                                                                     \
+        if (!(cond)) BH_FAIL(BH_ERR_INVALID, __VA_ARGS__);                                   \
+    } while (0)
+
+// ---------------------------------------------------------------- device memory
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+            throw Error{BH_ERR_OOM};
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        if (count > n) alloc(count);
+        if (count) BH_CHECK_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---------------------------------------------------------------- quadrature rules
+// Dunavant triangle rules in chart parameters (u, v); weights sum to 1/2.
+int triangle_rule(int degree, double *u, double *v, double *w);  // returns npts or -1
+void legendre01(int n, double *x, double *w);                   // nodes ascending on [0,1]
+constexpr int kMaxTriPts = 16;
+constexpr int kMaxGL = 16;
+
+// ---------------------------------------------------------------- objects
+struct Space;
+struct Ctx;
+struct Mesh;
+// quadrature points of a triangle rule on all cells of a mesh (cached): [nf][Q][4]
+const void *quad_points(Mesh &m, int degree, int *Q, bool fp32, hipStream_t s);
+void assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ,
+                     long long ld, bool rowmajor, bool accumulate, double sr, double si);
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    std::string arch;
+};
+
+struct Mesh {
+    Ctx *ctx = nullptr;
+    int64_t nv = 0, nf = 0;
+    std::vector<double> V;      // [nv][3]
+    std::vector<int32_t> C;     // [nf][3]
+    std::vector<double> geo;    // per cell: vertices[9], centroid[3], area, normal[3]
+    DevBuf<double> d_geo;       // same on device
+    static constexpr int kGeo = 16;
+    // quadrature points per triangle-rule degree: [nf][Q][4] (x - centroid, weight)
+    std::map<int, DevBuf<double>> quad64;
+    std::map<int, DevBuf<float>> quad32;
+};
+
+struct Space {
+    Mesh *mesh = nullptr;
+    int64_t ndofs = 0;
+    std::vector<int32_t> ptr, cell, ref;
+    std::vector<double> coeff;
+    // element-to-dof map: per (cell, refid) up to kmax (dof, coeff) slots
+    int kmax = 1;
+    std::vector<int32_t> e2d;       // [nf][3][kmax], -1 = none
+    std::vector<double> e2c;        // [nf][3][kmax]
+    DevBuf<int32_t> d_e2d;
+    DevBuf<double> d_e2c;
+    // dof tiles for the far-field gather kernel
+    bool tiles_built = false;
+    int64_t ntiles = 0;
+    DevBuf<int> t_ptr, t_cells, t_slot;
+    DevBuf<double> t_coef;
+};
+
+}  // namespace bh
+
+// Opaque C handles are the internal objects.
+struct bh_ctx : bh::Ctx {};
+struct bh_mesh : bh::Mesh {};
+struct bh_space : bh::Space {};

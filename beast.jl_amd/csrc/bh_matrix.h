@@ -1,0 +1,15 @@
+// Resident matrix object (row-major complex in HBM) and its factors.
+#pragma once
+
+#include "bh_internal.h"
+
+struct bh_matrix {
+    bh::Ctx *ctx = nullptr;
+    int64_t n = 0;
+    bh::DevBuf<double> a64;        // FP64 matrix, row-major interleaved complex (n*n*2)
+    bh::DevBuf<double> lu64;       // FP64 LU factors (when factored in FP64)
+    bh::DevBuf<float> lu32;        // FP32 LU factors (when factored in FP32)
+    bh::DevBuf<int64_t> ipiv;      // 0-based pivot rows
+    int factored = -1;             // -1 none, BH_FP64 or BH_FP32
+    double seconds_factor = 0.0;
+};

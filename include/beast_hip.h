@@ -1,0 +1,170 @@
+/*
+ * beast_hip.h -- C ABI of the MI355X-native Galerkin BEM library.
+ *
+ * Drop-in boundary for the reference's
+ *     assemble(op, X, Y; quadstrat)           src/operator.jl:100-116
+ * on op in {Maxwell3D.singlelayer, Maxwell3D.doublelayer}
+ * (MWSingleLayer3D / MWDoubleLayer3D, src/maxwell/mwops.jl:11-64) and
+ * X, Y raviartthomas spaces (src/bases/rtspace.jl:3-105), plus the dense solve
+ * behind it (LAPACK getrf/getrs via src/solvers/lusolver.jl:26-34).
+ *
+ * Conventions: plain pointers and sizes; 0-based indices; complex numbers as
+ * interleaved (re, im) doubles; matrices in column-major order (the layout of
+ * the reference's Matrix{ComplexF64}) unless a function says otherwise.
+ * Every function returns BH_OK (0) or a negative error code; the message of
+ * the last failure on the calling thread is available from bh_last_error().
+ * Handles are not thread safe; distinct contexts may be used concurrently.
+ */
+#ifndef BEAST_HIP_H
+#define BEAST_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_VERSION 100
+
+enum bh_status {
+    BH_OK = 0,
+    BH_ERR_INVALID = -1,      /* bad argument (reference: ArgumentError / error()) */
+    BH_ERR_OOM = -2,          /* device allocation failed */
+    BH_ERR_HIP = -3,          /* HIP runtime error */
+    BH_ERR_ASSERT = -4,       /* a condition the reference @asserts failed (e.g. hits > 3) */
+    BH_ERR_UNSUPPORTED = -5,  /* valid input outside the GPU path (caller falls back) */
+    BH_ERR_SINGULAR = -6,     /* exactly singular pivot in the LU (LAPACK info > 0) */
+    BH_ERR_NOCONV = -7        /* iterative refinement did not reach the tolerance */
+};
+
+/* Operator kinds (src/maxwell/maxwell.jl:16-79). */
+enum bh_op_kind { BH_OP_MW_SINGLELAYER = 0, BH_OP_MW_DOUBLELAYER = 1 };
+
+/* Quadrature strategy kinds; the integer parameters follow the field order of
+ * the reference structs:
+ *   BH_QS_WILTON_SAUTER  DoubleNumWiltonSauterQStrat(outer_far, inner_far,
+ *                        outer_near, inner_near, ss_tetr, ss_face, ss_edge, ss_vert)
+ *                        src/quadrature/doublenumwiltonsauterqstrat.jl:1-10
+ *   BH_QS_SAUTER         DoubleNumSauterQstrat(outer, inner, ss_tetr, ss_face,
+ *                        ss_edge, ss_vert)   src/quadrature/doublenumsauterqstrat.jl:1-8
+ *   BH_QS_DOUBLENUM      DoubleNumQStrat(outer, inner)   src/quadrature/doublenumqstrat.jl:1-4
+ * Triangle rule parameters are Dunavant polynomial degrees 1..8; Sauter-Schwab
+ * parameters are Gauss-Legendre orders 1..16. */
+enum bh_qs_kind { BH_QS_WILTON_SAUTER = 0, BH_QS_SAUTER = 1, BH_QS_DOUBLENUM = 2 };
+
+/* Arithmetic of the far-field quadrature: BH_FP64 everywhere, or BH_FP32 for
+ * the far-field double-Gauss only (classification, Sauter-Schwab and Wilton
+ * blocks stay FP64 and are rounded when stored). */
+enum bh_precision { BH_FP64 = 0, BH_FP32 = 1 };
+
+typedef struct bh_ctx bh_ctx;
+typedef struct bh_mesh bh_mesh;
+typedef struct bh_space bh_space;
+typedef struct bh_matrix bh_matrix;
+
+/* Operator description: gamma, alpha, beta as in MWSingleLayer3D(gamma, alpha,
+ * beta) (mwops.jl:11-15); MWDoubleLayer3D uses gamma only (its alpha is not
+ * used by the kernel, mwops.jl:141-155). */
+typedef struct {
+    int32_t kind;          /* bh_op_kind */
+    double gamma[2];
+    double alpha[2];
+    double beta[2];
+} bh_operator;
+
+typedef struct {
+    int32_t kind;          /* bh_qs_kind */
+    int32_t params[8];
+} bh_quadstrat;
+
+int bh_version(void);
+const char *bh_last_error(void);
+
+/* Context: one GPU and one stream.  Replaces the Threading{:multi} driver of
+ * src/operator.jl:215-231 (the device is the parallel resource). */
+int bh_ctx_create(int device, bh_ctx **out);
+int bh_ctx_destroy(bh_ctx *ctx);
+int bh_ctx_synchronize(bh_ctx *ctx);
+
+/* Triangle mesh: vertices[3*nv] (x,y,z per vertex), cells[3*nf] 0-based vertex
+ * indices in chart order (CompScienceMeshes chart: r(u,v) = u v1 + v v2 + (1-u-v) v3). */
+int bh_mesh_create(bh_ctx *ctx, const double *vertices, int64_t nv, const int32_t *cells, int64_t nf,
+                   bh_mesh **out);
+int bh_mesh_destroy(bh_mesh *mesh);
+
+/* Raviart-Thomas space in the form of RTBasis.fns (rtspace.jl:3-7): the shapes
+ * of dof b are entries shape_ptr[b] .. shape_ptr[b+1]-1 of (cell, refid, coeff),
+ * refid 0-based (the local vertex opposite the edge).  From this the library
+ * builds the element-to-dof map of assemblydata (src/bases/basis.jl:235-294). */
+int bh_space_create(bh_mesh *mesh, int64_t ndofs, const int32_t *shape_ptr, const int32_t *shape_cell,
+                    const int32_t *shape_ref, const double *shape_coeff, bh_space **out);
+int bh_space_destroy(bh_space *space);
+/* Element-to-dof map as built (dof[(cell*3+ref)*kmax + k], -1 if unused). */
+int bh_space_e2d(const bh_space *space, int32_t *kmax, int32_t *dof, double *coeff, int64_t cap);
+
+/* Dense assembly into a host matrix (drop-in for assemble/assemble!,
+ * operator.jl:100-116,168-184): Z[row_off + m + ld*(col_off + n)] (+)= scale * A[m, n]
+ * with m a test dof of X and n a trial dof of Y.  With accumulate = 0 the
+ * target block is overwritten, with 1 it is added to (the store(v,m,n)
+ * semantics of assemble!).  Z is complex double (precision = BH_FP64) or complex
+ * float (precision = BH_FP32), column-major with leading dimension ld. */
+int bh_assemble(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, int precision,
+                const bh_space *X, const bh_space *Y, const double *scale, int accumulate, void *Z,
+                int64_t ld, int64_t row_off, int64_t col_off);
+
+/* Pair classes (0 far, 1 Wilton near, 2 common vertex, 3 common edge, 4 common
+ * face) for all test x trial cell pairs, row-major [nf_test][nf_trial];
+ * the classification of doublenumwiltonsauterqstrat.jl:32-69. */
+int bh_classify(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, const bh_mesh *test,
+                const bh_mesh *trial, int8_t *classes);
+
+/* Local 3x3 blocks of a list of cell pairs (column-major 3x3 complex each, i.e.
+ * 18 doubles per pair), computed by the device kernels of the given class path;
+ * debug/inspection entry for per-regime parity tests. */
+int bh_local_blocks(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, const bh_mesh *test,
+                    const bh_mesh *trial, int64_t npairs, const int32_t *test_cells,
+                    const int32_t *trial_cells, double *blocks);
+
+/* Plane-wave excitation e = (n x E) x n tested with X (excitation.jl:17-115,
+ * mwexc.jl:36-42): E(x) = amplitude * exp(-gamma d.x) * pol, rule = triangle
+ * degree (SingleNumQStrat(8) by default).  b[m] (+)= sum. */
+int bh_assemble_planewave(bh_ctx *ctx, const bh_space *X, const double *direction, const double *pol_re,
+                          const double *pol_im, const double *gamma, const double *amplitude, int rule,
+                          int accumulate, double *b);
+
+/* ---------------------------------------------------------------- resident matrices
+ * A square system matrix kept in HBM with its LU factors (no counterpart in the
+ * reference: its Matrix{ComplexF64} lives on the host and is factored by
+ * LAPACK, lusolver.jl:26-34 / BEAST.lu, lu.jl:29-47). */
+typedef struct {
+    int32_t iterations;     /* refinement iterations (outer) */
+    int32_t inner;          /* total inner (GMRES) iterations */
+    double residual;        /* final ||b - A x|| / ||b|| in FP64, max over rhs */
+    double seconds_factor;  /* device time of the last factorisation */
+    double seconds_solve;   /* device time of the solve */
+} bh_solve_stats;
+
+/* Assemble op on (X, X) into a resident FP64 matrix (row-major in HBM). */
+int bh_matrix_assemble(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, const bh_space *X,
+                       bh_matrix **out);
+/* Resident matrix from host data (column-major complex double, n x n). */
+int bh_matrix_from_host(bh_ctx *ctx, int64_t n, const double *A, int64_t ld, bh_matrix **out);
+int bh_matrix_destroy(bh_matrix *M);
+int bh_matrix_size(const bh_matrix *M, int64_t *n);
+int bh_matrix_download(bh_matrix *M, double *A, int64_t ld);
+/* LU with partial pivoting (getrf semantics) of the matrix in the given precision:
+ * BH_FP64 factors the FP64 matrix, BH_FP32 a rounded FP32 copy. */
+int bh_matrix_factor(bh_matrix *M, int precision);
+/* Solve A X = B for nrhs column-major right-hand sides (ld = n).  With FP64
+ * factors: getrs.  With FP32 factors: FP64-residual iterative refinement with
+ * GMRES (GMRES-IR), stopped at ||b - A x|| / ||b|| <= tol or maxit outer steps. */
+int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, double *X, double tol, int maxit,
+                    bh_solve_stats *stats);
+/* Pivot vector of the last factorisation (1-based like LAPACK ipiv). */
+int bh_matrix_pivots(bh_matrix *M, int64_t *ipiv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEAST_HIP_H */

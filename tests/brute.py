@@ -1,0 +1,93 @@
+"""Brute-force comparators for the oracle's pins (test utilities).
+
+``se_composite`` evaluates a pair block by singularity extraction with composite
+rules: the outer triangle and the regular part use Dunavant rules on 4^L
+subtriangles, the singular part the analytic potential integrals.  As L grows
+it converges to the exact integral for any pair, touching pairs included, so
+it checks the Sauter-Schwab blocks independently of their region formulas.
+"""
+import numpy as np
+
+
+def subtriangles(v, L):
+    tris = [np.asarray(v, float)]
+    for _ in range(L):
+        new = []
+        for a, b, c in tris:
+            ab, bc, ca = (a + b) / 2, (b + c) / 2, (c + a) / 2
+            new += [np.array([a, ab, ca]), np.array([ab, b, bc]), np.array([ca, bc, c]), np.array([ab, bc, ca])]
+        tris = new
+    return tris
+
+
+def composite_points(trgauss, v, degree, L):
+    uv, w = trgauss(degree)
+    P, W = [], []
+    for t in subtriangles(v, L):
+        A = 0.5 * np.linalg.norm(np.cross(t[0] - t[2], t[1] - t[2]))
+        for q in range(len(w)):
+            u, vv = uv[0, q], uv[1, q]
+            P.append(u * t[0] + vv * t[1] + (1 - u - vv) * t[2])
+            W.append(w[q] * 2 * A)
+    return np.array(P), np.array(W)
+
+
+def area(v):
+    return 0.5 * np.linalg.norm(np.cross(v[0] - v[2], v[1] - v[2]))
+
+
+def se_composite(orc, trgauss, kind, gam, al, be, tv, sv, L=3):
+    tv, sv = np.asarray(tv, float), np.asarray(sv, float)
+    At, As = area(tv), area(sv)
+    X, WX = composite_points(trgauss, tv, 8, L)
+    Y, WY = composite_points(trgauss, sv, 6, L)      # no shared centroid point with degree 8
+    z = np.zeros((3, 3), complex)
+    nrm = np.cross(sv[0] - sv[2], sv[1] - sv[2])
+    nrm /= np.linalg.norm(nrm)
+    gvals = (Y[:, None, :] - sv[None, :, :]) / (2 * As)
+    for x, wx in zip(X, WX):
+        r = x - Y
+        R = np.linalg.norm(r, axis=1)
+        gR = gam * R
+        f = (x - tv) / (2 * At)
+        s, v, gg = orc.wiltonints(sv[0], sv[1], sv[2], x)
+        if kind == 0:
+            Gr = (np.expm1(-gR) + gR - 0.5 * gR ** 2) / (4 * np.pi * R)
+            ff = np.einsum("ik,qjk->qij", f, gvals)
+            z += wx * np.einsum("q,q,qij->ij", WY, Gr, al * ff) + wx * np.sum(WY * Gr) * be / (At * As)
+            h = np.dot(x - sv[0], nrm)
+            rho = x - h * nrm
+            IG = (s[1] - gam * s[2] + 0.5 * gam ** 2 * s[3]) / (4 * np.pi)
+            IGy = (v[1] + s[1] * rho - gam * (v[2] + s[2] * rho) + 0.5 * gam ** 2 * (v[3] + s[3] * rho)) / (4 * np.pi)
+            for i in range(3):
+                for j in range(3):
+                    z[i, j] += wx * (al * np.dot(x - tv[i], IGy - IG * sv[j]) + be * 4 * IG) / (4 * At * As)
+        else:
+            e = np.exp(-gR)
+            hh = (-(gR + 1) * e + 1 - 0.5 * gR ** 2) / (4 * np.pi * R ** 3)
+            for j in range(3):
+                cr = np.cross(r, gvals[:, j, :])
+                for i in range(3):
+                    z[i, j] += wx * np.sum(WY * hh * (cr @ f[i]))
+            IdG = (-gg[0] - 0.5 * gam ** 2 * gg[1]) / (4 * np.pi)
+            for i in range(3):
+                for j in range(3):
+                    z[i, j] += wx * np.dot(np.cross(x - sv[j], x - tv[i]), IdG) / (4 * At * As)
+    return z
+
+
+def wilton_brute(trgauss, p1, p2, p3, x, L=4):
+    """Quadrature comparator of the analytic integrals (s, v, g as in the oracle)."""
+    tri = np.array([p1, p2, p3], float)
+    Y, W = composite_points(trgauss, tri, 8, L)
+    n = np.cross(tri[0] - tri[2], tri[1] - tri[2])
+    n /= np.linalg.norm(n)
+    d = np.dot(x - tri[0], n)
+    rho = x - d * n
+    R = np.linalg.norm(Y - x, axis=1)
+    s = np.array([np.sum(W * d / R ** 3), np.sum(W / R), np.sum(W), np.sum(W * R), np.sum(W * R ** 2),
+                  np.sum(W * R ** 3)])
+    yr = Y - rho
+    v = np.array([(W / R ** 3) @ yr, (W / R) @ yr, W @ yr, (W * R) @ yr, (W * R ** 2) @ yr])
+    g = np.array([-(v[0] - s[0] * n), v[1] - d * s[1] * n, 2 * (v[2] - d * s[2] * n), 3 * (v[3] - d * s[3] * n)])
+    return s, v, g

@@ -1,0 +1,160 @@
+"""Pins of the CPU oracle against the reference's own tests and against mathematics (CPU)."""
+import os
+
+import numpy as np
+import pytest
+
+import brute
+
+T1 = np.array([[0.180878, -0.941848, -0.283207], [0.0, -0.980785, -0.19509], [0.0, -0.92388, -0.382683]])
+T2_CV = np.array([[0.373086, -0.881524, -0.289348], [0.180878, -0.941848, -0.283207], [0.294908, -0.944921, -0.141962]])
+T2_CE = np.array([[0.180878, -0.941848, -0.283207], [0.158174, -0.881178, -0.44554], [0.0, -0.92388, -0.382683]])
+# operators of test_sauterschwabints.jl:30-32: (kind, gamma, alpha, beta)
+OP1 = (0, 1.0, 250.0, 1.0)      # MWSingleLayer3D(1.0, 250.0, 1.0)
+OP2 = (0, 1.0, -1.0, -1.0)      # MWSingleLayer3D(1.0)
+OP3 = (1, 0.0, 1.0, 0.0)        # MWDoubleLayer3D(0.0)
+
+
+# --------------------------------------------------------- Wilton integrals
+WTRI = np.array([[0.0, 0.0, 0.0], [2.0, 0.0, 0.0], [0.0, 3.0, 0.0]])
+WPTS = [WTRI[0] + 2 * WTRI[1] - WTRI[2], (WTRI[0] + WTRI[1]) / 2 + [0, 0, 0.3], [3.0, 0.0, 0.0],
+        WTRI.mean(axis=0) + [0, 0, 0.3], WTRI[1] + [0, 0, 0.3], WTRI.mean(axis=0) + [0, 0, 100.0]]
+
+
+@pytest.mark.parametrize("k", range(len(WPTS)))
+def test_wilton_vs_quadrature(bj, orc, k):
+    """test_wiltonints.jl:291-322 (there 1e-5 with trgauss(13)); composite rules give tighter pins."""
+    x = np.asarray(WPTS[k], float)
+    s, v, g = orc.wiltonints(WTRI[0], WTRI[1], WTRI[2], x)
+    sb, vb, gb = brute.wilton_brute(bj.trgauss, WTRI[0], WTRI[1], WTRI[2], x, L=5)
+    # s[1..3] (1/R, 1, R), v[1..3], g[0..1] are the quantities the kernels use
+    for i in (1, 2, 3, 4, 5):
+        assert abs(s[i] - sb[i]) < 1e-6 * max(1, abs(sb[i])), ("s", i)
+    for i in (1, 2, 3, 4):
+        assert np.abs(v[i] - vb[i]).max() < 1e-6 * max(1, np.abs(vb[i]).max()), ("v", i)
+    for i in range(4):
+        assert np.abs(g[i] - gb[i]).max() < 1e-5 * max(1, np.abs(gb[i]).max()), ("g", i)
+
+
+def test_wilton_closed_forms(orc):
+    # far away, int 1/R -> area / dist and int R -> area * dist
+    x = WTRI.mean(axis=0) + [0, 0, 1e4]
+    s, v, g = orc.wiltonints(WTRI[0], WTRI[1], WTRI[2], x)
+    assert abs(s[2] - 3.0) < 1e-12
+    assert abs(s[1] - 3.0 / 1e4) < 1e-8 * 3e-4   # next term is O(size^2 / dist^2)
+    # point in the plane, projection inside: solid-angle term vanishes, int 1/R finite and positive
+    s, v, g = orc.wiltonints(WTRI[0], WTRI[1], WTRI[2], WTRI.mean(axis=0))
+    assert s[0] == 0.0 and s[1] > 0
+
+
+# --------------------------------------------------------- Sauter-Schwab
+CASES = [("CF", T1, OP1), ("CF", T1, OP2), ("CV", T2_CV, OP1), ("CV", T2_CV, OP2), ("CV", T2_CV, OP3),
+         ("CE", T2_CE, OP1), ("CE", T2_CE, OP2), ("CE", T2_CE, OP3)]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_sauterschwab_vs_extraction(bj, orc, case):
+    """test_sauterschwabints.jl:27-143 cross-checks SS against Wilton extraction.
+
+    A composite-rule extraction (brute.se_composite) converges to the exact
+    integral; SS must sit closer to its finest level than two levels differ."""
+    name, sv, (kind, g, a, b) = CASES[case]
+    zss = orc.block_strategy(kind, g, a, b, "ss", T1, sv, n=12)
+    z2 = brute.se_composite(orc, bj.trgauss, kind, g, a, b, T1, sv, L=2)
+    z3 = brute.se_composite(orc, bj.trgauss, kind, g, a, b, T1, sv, L=3)
+    step = np.abs(z2 - z3).max()
+    assert np.abs(zss - z3).max() < 0.5 * step, (name, np.abs(zss - z3).max(), step)
+    # and the reference's own tolerances (CF 1e-4, CV 1e-7 relative to the block, CE 1e-5) hold
+    tol = {"CF": 1e-4, "CV": 1e-7, "CE": 1e-5}[name] * max(1.0, np.abs(zss).max())
+    assert np.abs(zss - z3).max() < 10 * tol
+
+
+def test_sauterschwab_converges(orc):
+    # test_sauterschwabints.jl:139-143: CE double layer, 12 vs 18 points agree to 1e-14
+    z12 = orc.block_strategy(*OP3, "ss", T1, T2_CE, n=12)
+    z16 = orc.block_strategy(*OP3, "ss", T1, T2_CE, n=16)
+    assert np.abs(z12 - z16).max() < 1e-14
+    for name, sv, op in CASES:
+        za = orc.block_strategy(*op, "ss", T1, sv, n=12)
+        zb = orc.block_strategy(*op, "ss", T1, sv, n=16)
+        assert np.abs(za - zb).max() < 1e-9 * np.abs(zb).max(), name
+
+
+def test_common_face_double_layer_vanishes(orc):
+    # on a flat cell grad G and g_j lie in the plane, so grad G x g_j is normal to it and f_i . (.) = 0
+    z = orc.block_strategy(1, 0.0 + 2.0j, 1.0, 0.0, "ss", T1, T1, n=5)
+    zs = orc.block_strategy(0, 2.0j, -2.0j, 0.5j, "ss", T1, T1, n=5)
+    assert np.abs(z).max() < 1e-15 * np.abs(zs).max() * 1e3
+
+
+def test_far_pair_dq_vs_extraction(bj, orc, golden):
+    # test_wiltonints.jl:325-361: a far pair of sphere2.in, double quadrature vs extraction within 1e-7
+    m = bj.readmesh(os.path.join(golden, "sphere2.in"))
+    tv, sv = m.vertices[m.cells[0]], m.vertices[m.cells[-1]]
+    zdq = orc.block_strategy(0, 1.0j, -1.0j, 1.0j, "dq", tv, sv, 8, 8)
+    zse = orc.block_strategy(0, 1.0j, -1.0j, 1.0j, "se", tv, sv, 8, 7)
+    assert np.linalg.norm(zdq - zse) < 1e-7
+
+
+# --------------------------------------------------------- classification
+def _classify_numpy(mt, ms, k2, wilton=True):
+    """Vectorised restatement of doublenumwiltonsauterqstrat.jl:35-68 (same operation order)."""
+    T = mt.vertices[mt.cells]
+    S = ms.vertices[ms.cells]
+    d = T[:, None, :, None, :] - S[None, :, None, :, :]
+    d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    hits = (np.sqrt(d2) < 1e3 * np.finfo(float).eps).sum(axis=(2, 3))
+    dmin2 = d2.min(axis=(2, 3))
+    area = ms.areas()
+    cls = np.zeros(hits.shape, dtype=np.int8)
+    if wilton:
+        near = np.maximum(dmin2 * k2, dmin2 / (16 * area[None, :])) < 0.2 * 0.2
+        cls[near] = 1
+    cls[hits == 1] = 2
+    cls[hits == 2] = 3
+    cls[hits == 3] = 4
+    return cls
+
+
+@pytest.mark.parametrize("k", [1.0, 3.0, 12.0])
+def test_classification_oracle_bitexact(bj, orc, golden, k):
+    m = bj.readmesh(os.path.join(golden, "sphere45.in"))
+    X = bj.raviartthomas(m)
+    ref = _classify_numpy(m, m, k * k)
+    out = np.zeros_like(ref)
+    for p in range(m.numcells):
+        for q in range(m.numcells):
+            out[p, q] = orc.classify(m.vertices[m.cells[p]], m.vertices[m.cells[q]], k * k)
+    assert np.array_equal(out, ref)
+    counts = np.bincount(ref.ravel(), minlength=5)
+    assert counts[4] == m.numcells and counts[3] == 3 * m.numcells   # CF on the diagonal, 3 CE per cell
+
+
+# --------------------------------------------------------- assembled system
+def test_efie_solve_residual(bj, orc, golden):
+    # test_itsolver.jl:9-24: EFIE k=1, plane wave z-direction / x-polarised, ||A u - b|| / ||b|| < 100 eps
+    m = bj.readmesh(os.path.join(golden, "sphere45.in"))
+    X = bj.raviartthomas(m)
+    k = 1.0
+    t = bj.Maxwell3D.singlelayer(wavenumber=k)
+    A = orc.assemble(0, t.gamma, t.alpha, t.beta, 0, (2, 3, 6, 7, 5, 5, 4, 3), X, X)
+    b = orc.planewave(X, [0, 0, 1.0], [1.0, 0, 0], 1j * k)
+    u = orc.lu_solve(A, b)
+    assert np.linalg.norm(A @ u - b) / np.linalg.norm(b) < 100 * np.finfo(float).eps
+    # complex symmetry of the Galerkin EFIE up to quadrature error
+    assert np.linalg.norm(A - A.T) / np.linalg.norm(A) < 1e-3
+
+
+def test_oracle_deterministic(bj, orc, golden):
+    m = bj.readmesh(os.path.join(golden, "sphere45.in"))
+    X = bj.raviartthomas(m)
+    Z1 = orc.assemble(1, 2j, 1.0, 0.0, 0, (2, 3, 6, 7, 5, 5, 4, 3), X, X)
+    Z2 = orc.assemble(1, 2j, 1.0, 0.0, 0, (2, 3, 6, 7, 5, 5, 4, 3), X, X)
+    assert np.array_equal(Z1, Z2)
+
+
+def test_hits_assertion(bj, orc):
+    # a sliver cell whose vertices sit within 1e3 eps of each other trips the reference's @assert hits <= 3
+    V = np.array([[0, 0, 0], [1e-14, 0, 0], [0, 1e-14, 0], [1.0, 1.0, 0]], float)
+    with pytest.raises(AssertionError):
+        orc.block(0, 1j, -1j, 1j, 0, (2, 3, 6, 7, 5, 5, 4, 3), V[[0, 1, 2]], V[[0, 1, 2]])
