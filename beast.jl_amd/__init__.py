@@ -11,7 +11,8 @@ from .operators import (Maxwell3D, MWSingleLayer3D, MWDoubleLayer3D, PlaneWaveMW
                         DoubleNumWiltonSauterQStrat, DoubleNumSauterQstrat, DoubleNumQStrat, SingleNumQStrat,
                         defaultquadstrat, gamma_wavenumber_handler, n, cross, x_hat, y_hat, z_hat,
                         TangTraceMW, CrossTraceMW)
-from .assembly import Context, default_context, assemble, assemble_rhs, classify, local_blocks, element_to_dof
+from .assembly import (Context, default_context, assemble, assemble_rhs, classify, local_blocks, element_to_dof,
+                       DeviceMatrix, lu)
 from ._native import BHError, UnsupportedOnGPU, SingularException, NoConvergence
 
 __all__ = [k for k in dir() if not k.startswith("_")]

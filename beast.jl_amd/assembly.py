@@ -30,6 +30,18 @@ class Context:
     def synchronize(self):
         nat.check(self._lib.bh_ctx_synchronize(self.handle))
 
+    def profile(self, enable=True):
+        """Bracket every library kernel launch with HIP events on this context's stream."""
+        nat.check(self._lib.bh_ctx_profile(self.handle, int(bool(enable))))
+
+    def kernel_stats(self, name):
+        """(launches, device seconds, algorithmic work) of one kernel since profiling was enabled."""
+        n = C.c_int64()
+        sec = C.c_double()
+        work = C.c_double()
+        nat.check(self._lib.bh_ctx_kernel_stats(self.handle, name.encode(), C.byref(n), C.byref(sec), C.byref(work)))
+        return n.value, sec.value, work.value
+
     def close(self):
         if self.handle:
             self._lib.bh_ctx_destroy(self.handle)
@@ -221,3 +233,80 @@ def element_to_dof(X, ctx=None):
     cf = np.zeros(n)
     nat.check(lib.bh_space_e2d(h, C.byref(k), nat.i32ptr(dof), nat.dptr(cf), n))
     return dof.reshape(X.geo.numcells, 3, k.value), cf.reshape(X.geo.numcells, 3, k.value)
+
+
+class DeviceMatrix:
+    """A square system matrix resident in HBM (FP64) with its LU factors.
+
+    Replaces the host Matrix{ComplexF64} + LAPACK LU of the reference
+    (src/solvers/lusolver.jl:26-34, BEAST.lu in src/solvers/lu.jl:29-47).
+    """
+
+    def __init__(self, handle, n, ctx):
+        self.h = handle
+        self.n = n
+        self.ctx = ctx
+        self.stats = None
+
+    @classmethod
+    def assemble(cls, op, X, quadstrat=defaultquadstrat, ctx=None):
+        ctx = ctx or default_context()
+        if isinstance(op, LinearCombinationOfOperators):
+            raise nat.UnsupportedOnGPU(nat.BH_ERR_UNSUPPORTED, "resident assembly takes a single operator")
+        qs = resolve_quadstrat(quadstrat, op, X, X)
+        h = C.c_void_p()
+        nat.check(nat.load().bh_matrix_assemble(ctx.handle, C.byref(op_struct(op)), C.byref(qs_struct(qs)),
+                                                space_handle(X, ctx), C.byref(h)))
+        return cls(h, X.numfunctions(), ctx)
+
+    @classmethod
+    def from_host(cls, Z, ctx=None):
+        ctx = ctx or default_context()
+        Z = np.asfortranarray(Z, dtype=np.complex128)
+        n = Z.shape[0]
+        if Z.shape != (n, n):
+            raise ValueError("square matrix expected")
+        h = C.c_void_p()
+        nat.check(nat.load().bh_matrix_from_host(ctx.handle, n, Z.ctypes.data_as(nat._D), n, C.byref(h)))
+        return cls(h, n, ctx)
+
+    def download(self):
+        Z = np.zeros((self.n, self.n), dtype=np.complex128, order="F")
+        nat.check(nat.load().bh_matrix_download(self.h, Z.ctypes.data_as(nat._D), self.n))
+        return Z
+
+    def factor(self, precision="fp64"):
+        nat.check(nat.load().bh_matrix_factor(self.h, {"fp64": nat.FP64, "fp32": nat.FP32}[precision]))
+        return self
+
+    def pivots(self):
+        ip = np.zeros(self.n, dtype=np.int64)
+        nat.check(nat.load().bh_matrix_pivots(self.h, nat.i64ptr(ip)))
+        return ip
+
+    def solve(self, B, tol=0.0, maxit=0):
+        """X = Z \\ B (B of shape [n] or [n, nrhs]); FP32 factors are refined to ``tol`` by GMRES-IR."""
+        B = np.asarray(B, dtype=np.complex128)
+        vec = B.ndim == 1
+        Bf = np.asfortranarray(B.reshape(self.n, -1))
+        Xf = np.zeros_like(Bf, order="F")
+        st = nat.bh_solve_stats()
+        nat.check(nat.load().bh_matrix_solve(self.h, Bf.shape[1], Bf.ctypes.data_as(nat._D),
+                                             Xf.ctypes.data_as(nat._D), float(tol), int(maxit), C.byref(st)))
+        self.stats = {"iterations": st.iterations, "inner": st.inner, "residual": st.residual,
+                      "seconds_factor": st.seconds_factor, "seconds_solve": st.seconds_solve}
+        return Xf[:, 0] if vec else Xf
+
+    def __del__(self):
+        try:
+            if self.h:
+                nat.load().bh_matrix_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def lu(A, precision="fp64", ctx=None):
+    """``BEAST.lu``-style factorisation object of a host or resident matrix."""
+    M = A if isinstance(A, DeviceMatrix) else DeviceMatrix.from_host(A, ctx)
+    return M.factor(precision)

@@ -57,6 +57,7 @@ extern "C" int bh_ctx_destroy(bh_ctx *ctx)
     BH_API_BEGIN
     if (!ctx) return BH_OK;
     (void)hipSetDevice(ctx->device);
+    resolve_stats(*ctx);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -68,6 +69,46 @@ extern "C" int bh_ctx_synchronize(bh_ctx *ctx)
     BH_API_BEGIN
     BH_REQUIRE(ctx != nullptr, "ctx is NULL");
     BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    BH_API_END
+}
+
+namespace bh {
+void resolve_stats(Ctx &ctx)
+{
+    (void)hipStreamSynchronize(ctx.stream);
+    for (auto &kv : ctx.kstats) {
+        for (auto &ev : kv.second.pending) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) kv.second.seconds += ms * 1e-3;
+            (void)hipEventDestroy(ev.first);
+            (void)hipEventDestroy(ev.second);
+        }
+        kv.second.pending.clear();
+    }
+}
+}  // namespace bh
+
+extern "C" int bh_ctx_profile(bh_ctx *ctx, int enable)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctx != nullptr, "ctx is NULL");
+    resolve_stats(*ctx);
+    if (enable) ctx->kstats.clear();
+    ctx->profiling = enable != 0;
+    BH_API_END
+}
+
+extern "C" int bh_ctx_kernel_stats(bh_ctx *ctx, const char *kernel, int64_t *launches, double *seconds, double *work)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctx && kernel, "NULL argument");
+    resolve_stats(*ctx);
+    auto it = ctx->kstats.find(kernel);
+    const KernelStat zero;
+    const KernelStat &k = it == ctx->kstats.end() ? zero : it->second;
+    if (launches) *launches = k.launches;
+    if (seconds) *seconds = k.seconds;
+    if (work) *work = k.work;
     BH_API_END
 }
 

@@ -956,6 +956,7 @@ void build_near(Ctx &ctx, Mesh &tm, Mesh &sm, const Strategy &st, const OpParams
     const double k2 = op.gr * op.gr + op.gi * op.gi;
     DevBuf<int> counts(nft), err(1);
     BH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    ProfScope prof_c(ctx, "k_classify_count", (double)nft * nfs);
     k_classify_count<<<nft, 256, 0, s>>>(tm.d_geo.p, nft, sm.d_geo.p, nfs, k2, st.mode, counts.p, err.p);
     check_launch("k_classify_count");
     std::vector<int> hc(nft);
@@ -989,6 +990,8 @@ void build_near(Ctx &ctx, Mesh &tm, Mesh &sm, const Strategy &st, const OpParams
         legendre01(gl.n, gl.x, gl.w);
         const int grid = (cnt + 3) / 4;
         const int *lp = lists.p + (size_t)cls * nd.nent;
+        const int nreg = cls == kCF ? 6 : (cls == kCE ? 5 : 2);
+        ProfScope prof_s(ctx, "k_sauter_schwab", (double)cnt * nreg * gl.n * gl.n * gl.n * gl.n);
         if (op.kind == BH_OP_MW_SINGLELAYER)
             k_sauter_schwab<BH_OP_MW_SINGLELAYER><<<grid, 256, 0, s>>>(tm.d_geo.p, sm.d_geo.p, nd.entries.p, lp, cnt,
                                                                        cls, gl, op, nd.blocks.p);
@@ -1005,6 +1008,7 @@ void build_near(Ctx &ctx, Mesh &tm, Mesh &sm, const Strategy &st, const OpParams
         BH_REQUIRE(Qo <= 64, "near outer rule too large");
         const int grid = (cnt + 3) / 4;
         const int *lp = lists.p + (size_t)kNear * nd.nent;
+        ProfScope prof_w(ctx, "k_wilton", (double)cnt * Qo * Qi);
         if (op.kind == BH_OP_MW_SINGLELAYER)
             k_wilton<BH_OP_MW_SINGLELAYER><<<grid, 256, 0, s>>>(tm.d_geo.p, sm.d_geo.p, nd.entries.p, lp, cnt, tq, Qo,
                                                                 sq, Qi, op, nd.blocks.p);
@@ -1064,12 +1068,15 @@ struct Plan {
     OpParams op;
     NearData nd;
     FarArgs far;
+    long long nft = 0, nfs = 0;
 };
 
 void make_plan(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, Mesh &tm, Mesh &sm, bool fp32, Plan &P)
 {
     P.st = parse_qs(qs);
     P.op = parse_op(op);
+    P.nft = tm.nf;
+    P.nfs = sm.nf;
     build_near(ctx, tm, sm, P.st, P.op, P.nd);
     FarArgs &F = P.far;
     F.tgeo = tm.d_geo.p;
@@ -1101,6 +1108,10 @@ void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long lon
     BH_REQUIRE(X.ntiles <= 65535 && Y.ntiles <= 65535, "too many dof tiles for one launch");
     dim3 grid((unsigned)Y.ntiles, (unsigned)X.ntiles);
     const bool sl = P.op.kind == BH_OP_MW_SINGLELAYER;
+    // algorithmic work: far point pairs x flops per pair of the reference formulation
+    // (EFIE 170, MFIE-K 210; mwops.jl:101-117,141-155 with doublenumints.jl:32-38)
+    const double far_pp = ((double)P.nft * P.nfs - P.nd.nent) * P.far.Qo * P.far.Qi;
+    ProfScope prof(ctx, "k_far_tiles", far_pp * (sl ? 170.0 : 210.0));
     if (rowmajor) {
         if (sl) k_far_tiles<R, BH_OP_MW_SINGLELAYER, true><<<grid, 256, 0, ctx.stream>>>(A);
         else k_far_tiles<R, BH_OP_MW_DOUBLELAYER, true><<<grid, 256, 0, ctx.stream>>>(A);

@@ -111,12 +111,44 @@ const void *quad_points(Mesh &m, int degree, int *Q, bool fp32, hipStream_t s);
 void assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ,
                      long long ld, bool rowmajor, bool accumulate, double sr, double si);
 
+// Per-kernel timing with HIP events on the context stream (measurement only).
+struct KernelStat {
+    long long launches = 0;
+    double work = 0.0;       // algorithmic flops or bytes, as the launch site defines
+    double seconds = 0.0;    // resolved from events
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cus = 0;
     std::string arch;
+    bool profiling = false;
+    std::map<std::string, KernelStat> kstats;
 };
+
+// Brackets one launch on ctx.stream when profiling is on.
+struct ProfScope {
+    Ctx &ctx;
+    const char *name;
+    double work;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ProfScope(Ctx &c, const char *n, double w) : ctx(c), name(n), work(w) {
+        if (!ctx.profiling) return;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return;
+        (void)hipEventRecord(e0, ctx.stream);
+    }
+    ~ProfScope() {
+        if (!ctx.profiling || !e0 || !e1) return;
+        (void)hipEventRecord(e1, ctx.stream);
+        KernelStat &k = ctx.kstats[name];
+        k.launches += 1;
+        k.work += work;
+        k.pending.emplace_back(e0, e1);
+    }
+};
+void resolve_stats(Ctx &ctx);
 
 struct Mesh {
     Ctx *ctx = nullptr;

@@ -9,7 +9,9 @@ struct bh_matrix {
     bh::DevBuf<double> a64;        // FP64 matrix, row-major interleaved complex (n*n*2)
     bh::DevBuf<double> lu64;       // FP64 LU factors (when factored in FP64)
     bh::DevBuf<float> lu32;        // FP32 LU factors (when factored in FP32)
-    bh::DevBuf<int64_t> ipiv;      // 0-based pivot rows
+    bh::DevBuf<long long> ipiv;    // 0-based pivot rows (sequential interchanges)
+    bh::DevBuf<int> perm;          // (P b)[i] = b[perm[i]]
+    long long lda = 0;             // leading dimension of the factors
     int factored = -1;             // -1 none, BH_FP64 or BH_FP32
     double seconds_factor = 0.0;
 };

@@ -1,0 +1,174 @@
+"""Benchmark: EFIE assemble + solve on one MI355X.
+
+Workload (default ``--config C3``): EFIE (Maxwell3D.singlelayer, k = 4.0) on a
+unit-sphere mesh with 13,334 Fibonacci vertices (26,664 triangles, 39,996 RT
+dofs), plane wave d = x, p = z.  One step = FP64 assembly of the resident
+system matrix (default quadstrat) + FP32 LU on MFMA + GMRES-IR with FP64
+residuals until ||b - A x|| / ||b|| <= 1e-11; the solution's relative error
+against the plain FP64 GPU path (FP64 assembly + FP64 LU) is measured once,
+outside the timed region, and reported.
+
+Prints one JSON line (see the keys below).  ``--gpus`` above 1 is refused:
+this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__  # noqa: E402
+
+METRIC = "EFIE assemble+solve sec at 1e-8 rel err (mixed vs FP64); p50/p99 serve latency"
+PEAK_FP32_MFMA = 157.3e12
+PEAK_FP64_MFMA = 78.6e12
+
+CONFIGS = {
+    # name: (mesh vertices, jitter, seed, wavenumber)
+    "C1": (236, 0.0, None, 2.0),
+    "C2": (5804, 0.02, 2, 2 * np.pi),
+    "C3": (13334, 0.0, None, 4.0),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_problem(bj, name):
+    nv, jit, seed, k = CONFIGS[name]
+    mesh = bj.meshsphere(1.0, nv=nv, jitter=jit, seed=seed)
+    X = bj.raviartthomas(mesh)
+    t = bj.Maxwell3D.singlelayer(wavenumber=k)
+    E = bj.Maxwell3D.planewave(direction=bj.x_hat, polarization=bj.z_hat, wavenumber=k)
+    e = bj.cross(bj.cross(bj.n, E), bj.n)
+    return mesh, X, t, e, k
+
+
+def cpu_baseline(bj, mesh, X, t, k, ncells=128, nlu=6000):
+    """Oracle (C/OpenMP port of the reference algorithm) + LAPACK, on a bounded sample."""
+    from oracle import oracle
+    import scipy.linalg as sla
+    F, N = mesh.numcells, X.numfunctions()
+    rng = np.random.default_rng(0)
+    cells = np.sort(rng.choice(F, size=min(ncells, F), replace=False))
+    sub = mesh.subset(cells)
+    S = len(cells)
+    # all three local shapes of every sampled cell as separate test functions
+    T = bj.RTBasis(sub, np.arange(0, 3 * S + 1, dtype=np.int32), np.repeat(np.arange(S), 3).astype(np.int32),
+                   np.tile(np.arange(3), S).astype(np.int32), np.ones(3 * S))
+    t0 = time.perf_counter()
+    oracle.assemble(0, t.gamma, t.alpha, t.beta, 0, (2, 3, 6, 7, 5, 5, 4, 3), T, X)
+    ta = time.perf_counter() - t0
+    A = rng.standard_normal((nlu, nlu)) + 1j * rng.standard_normal((nlu, nlu))
+    t0 = time.perf_counter()
+    lu = sla.lu_factor(A, check_finite=False, overwrite_a=True)
+    tl = time.perf_counter() - t0
+    del lu, A
+    est = ta * F / S + tl * (N / nlu) ** 3
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    return {"value": est, "unit": "s", "cores": threads, "kind": "port",
+            "sample": (f"oracle assembly of {S} test cells x {F} trial cells ({ta:.2f} s) scaled by {F}/{S}; "
+                       f"LAPACK getrf of a {nlu}^2 complex matrix ({tl:.2f} s) scaled by ({N}/{nlu})^3; "
+                       "CPU restatement of the reference algorithm, not BEAST.jl")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C3", choices=sorted(CONFIGS))
+    ap.add_argument("--tol", type=float, default=1e-11)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    bj = __graft_entry__.load_package()
+    ctx = bj.default_context()
+    mesh, X, t, e, k = build_problem(bj, args.config)
+    N, F = X.numfunctions(), mesh.numcells
+    log(f"bench: {args.config} F={F} N={N} k={k}")
+    b = bj.assemble(e, X)
+
+    # plain FP64 GPU path (also the reference solution for the error)
+    t0 = time.perf_counter()
+    M64 = bj.DeviceMatrix.assemble(t, X)
+    ctx.synchronize()
+    t_asm64 = time.perf_counter() - t0
+    M64.factor("fp64")
+    x64 = M64.solve(b)
+    t_fp64 = time.perf_counter() - t0
+    fp64_factor = M64.stats["seconds_factor"]
+    del M64
+    log(f"bench: FP64 path {t_fp64:.3f} s (assembly {t_asm64:.3f} s, LU {fp64_factor:.3f} s)")
+
+    def step(profile=False):
+        ts = time.perf_counter()
+        M = bj.DeviceMatrix.assemble(t, X)
+        ctx.synchronize()
+        ta = time.perf_counter()
+        M.factor("fp32")
+        tf = time.perf_counter()
+        x = M.solve(b, tol=args.tol, maxit=10)
+        te = time.perf_counter()
+        st = dict(M.stats)
+        del M
+        return x, {"assemble_s": ta - ts, "factor_s": tf - ta, "solve_s": te - tf, "total_s": te - ts,
+                   "ir_iterations": st["iterations"], "gmres_iterations": st["inner"], "residual": st["residual"]}
+
+    for w in range(args.warmup):
+        x, ph = step()
+        log(f"bench: warmup {w} {ph}")
+    ctx.synchronize()
+    ctx.profile(True)
+    times, phases = [], []
+    t_all = time.perf_counter()
+    for s in range(args.steps):
+        x, ph = step()
+        ctx.synchronize()
+        times.append(ph["total_s"])
+        phases.append(ph)
+        log(f"bench: step {s} {ph}")
+    t_all = time.perf_counter() - t_all
+    g_launch, g_sec, g_work = ctx.kernel_stats("k_cgemm")
+    f_launch, f_sec, f_work = ctx.kernel_stats("k_far_tiles")
+    p_launch, p_sec, _ = ctx.kernel_stats("k_panel")
+    ctx.profile(False)
+    err = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
+    value = t_all / args.steps
+    achieved = g_work / g_sec if g_sec > 0 else 0.0
+    out = {
+        "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * value, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "c64 LU, c128 assembly/residual", "data": "synthetic",
+        "config": {"workload": f"EFIE assemble+solve, unit sphere {F} triangles / {N} RT dofs, k={k}, "
+                               f"mixed precision (FP32 LU + FP64 GMRES-IR to {args.tol:g})",
+                   "config": args.config, "dofs": N, "triangles": F, "wavenumber": k},
+        "rel_err_vs_fp64": err,
+        "fp64_path_s": t_fp64,
+        "speedup_vs_fp64_path": t_fp64 / value,
+        "phases": {key: float(np.median([p[key] for p in phases])) for key in phases[0]},
+        "roofline": {"kernel": "k_cgemm<float> (LU trailing updates)", "bound": "mfma",
+                     "achieved": achieved / 1e12, "peak": PEAK_FP32_MFMA / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_MFMA, "traffic": None,
+                     "launches": g_launch // max(1, args.steps), "avg_launch_ms": 1e3 * g_sec / max(1, g_launch)},
+        "kernels": {"k_far_tiles": {"launches": f_launch, "seconds": f_sec,
+                                    "gflops_ref_formulation": f_work / max(f_sec, 1e-30) / 1e9},
+                    "k_panel": {"launches": p_launch, "seconds": p_sec}},
+    }
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline sample")
+        out["cpu_baseline"] = cpu_baseline(bj, mesh, X, t, k)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -85,6 +85,13 @@ const char *bh_last_error(void);
 int bh_ctx_create(int device, bh_ctx **out);
 int bh_ctx_destroy(bh_ctx *ctx);
 int bh_ctx_synchronize(bh_ctx *ctx);
+/* Measurement (no counterpart in the reference): while profiling is on, every
+ * launch of the library's kernels is bracketed by HIP events on the context
+ * stream; bh_ctx_kernel_stats returns, per kernel name (e.g. "k_cgemm"), the
+ * launch count, summed device seconds and the algorithmic work (flops or
+ * bytes, see DESIGN) the launches were given.  Enabling clears the counters. */
+int bh_ctx_profile(bh_ctx *ctx, int enable);
+int bh_ctx_kernel_stats(bh_ctx *ctx, const char *kernel, int64_t *launches, double *seconds, double *work);
 
 /* Triangle mesh: vertices[3*nv] (x,y,z per vertex), cells[3*nf] 0-based vertex
  * indices in chart order (CompScienceMeshes chart: r(u,v) = u v1 + v v2 + (1-u-v) v3). */

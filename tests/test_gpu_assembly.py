@@ -115,10 +115,16 @@ def test_local_blocks_every_class(bj, orc, golden):
         z = bj.local_blocks(op, m, m, pairs)
         kind = 0 if op.__class__.__name__ == "MWSingleLayer3D" else 1
         beta = op.beta if kind == 0 else 0.0
-        for (p, q), zg in zip(pairs, z):
+        ref = []
+        for p, q in pairs:
             zo, c = orc.block(kind, op.gamma, op.alpha, beta, 0, DEFAULT, m.vertices[m.cells[p]], m.vertices[m.cells[q]])
             assert c == cls[p, q]
-            assert np.abs(zg - zo).max() <= 1e-12 * np.abs(zo).max(), (c, p, q)
+            ref.append(zo)
+        # common-face MFIE-K blocks vanish on flat cells (both sides are rounding noise), so the
+        # tolerance is relative to the largest block of the operator
+        scale = max(np.abs(zo).max() for zo in ref)
+        for (p, q), zg, zo in zip(pairs, z, ref):
+            assert np.abs(zg - zo).max() <= 1e-12 * np.abs(zo).max() + 1e-13 * scale, (cls[p, q], p, q)
 
 
 def test_fp32_far_field_error_profile(bj, orc, golden):

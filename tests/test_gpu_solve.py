@@ -1,0 +1,73 @@
+"""Dense LU and solves on the GPU against LAPACK (needs an MI355X)."""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import relfro
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand(n, seed, cond_boost=True):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    if cond_boost:
+        A += 0.5 * np.sqrt(n) * np.eye(n)
+    return np.asfortranarray(A)
+
+
+@pytest.mark.parametrize("n", [1, 5, 31, 33, 64, 130, 257, 700])
+def test_lu_fp64_matches_lapack(bj, n):
+    A = _rand(n, n)
+    M = bj.DeviceMatrix.from_host(A)
+    assert np.array_equal(M.download(), A)
+    M.factor("fp64")
+    lu, piv = sla.lu_factor(A)
+    # LAPACK's pivot rule (largest |Re|+|Im|, first on ties) -> identical interchanges
+    assert np.array_equal(M.pivots(), piv + 1)
+    b = _rand(n, n + 1)[:, :3]
+    x = M.solve(b)
+    xr = sla.lu_solve((lu, piv), b)
+    assert relfro(x, xr) < 1e-12
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 100 * np.finfo(float).eps * max(1, np.sqrt(n))
+
+
+@pytest.mark.parametrize("n", [40, 300, 1025])
+def test_lu_fp32_gmres_ir(bj, n):
+    A = _rand(n, 7 * n)
+    b = _rand(n, 7 * n + 1)[:, 0]
+    M = bj.DeviceMatrix.from_host(A).factor("fp32")
+    x = M.solve(b, tol=1e-12, maxit=10)
+    xr = np.linalg.solve(A, b)
+    assert M.stats["residual"] <= 1e-12
+    assert relfro(x, xr) < 1e-10
+    assert M.stats["iterations"] >= 1
+
+
+def test_singular_pivot(bj):
+    A = _rand(40, 3)
+    A[:, 7] = 0.0
+    M = bj.DeviceMatrix.from_host(A)
+    with pytest.raises(bj.SingularException):
+        M.factor("fp64")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_efie_end_to_end(bj, orc, golden, precision):
+    # test_itsolver.jl:9-24 on sphere2.in: EFIE k=1, plane wave z / x-polarised
+    m = bj.readmesh(os.path.join(golden, "sphere2.in"))
+    X = bj.raviartthomas(m)
+    k = 1.0
+    t = bj.Maxwell3D.singlelayer(wavenumber=k)
+    E = bj.Maxwell3D.planewave(direction=bj.z_hat, polarization=bj.x_hat, wavenumber=k)
+    b = bj.assemble(bj.cross(bj.cross(bj.n, E), bj.n), X)
+    M = bj.DeviceMatrix.assemble(t, X)
+    Zo = orc.assemble(0, t.gamma, t.alpha, t.beta, 0, (2, 3, 6, 7, 5, 5, 4, 3), X, X)
+    assert relfro(M.download(), Zo) < 1e-12
+    M.factor(precision)
+    u = M.solve(b, tol=1e-12 if precision == "fp32" else 0.0)
+    uo = orc.lu_solve(Zo, orc.planewave(X, bj.z_hat, bj.x_hat, 1j * k))
+    assert np.linalg.norm(Zo @ u - b) / np.linalg.norm(b) < 100 * np.finfo(float).eps * (10 if precision == "fp32" else 1)
+    assert relfro(u, uo) < 1e-10
