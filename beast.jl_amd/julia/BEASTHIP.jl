@@ -1,0 +1,208 @@
+"""
+    BEASTHIP
+
+Julia side of the drop-in boundary: routes BEAST's `assemble(op, X, Y; quadstrat)`
+for `Maxwell3D.singlelayer` / `Maxwell3D.doublelayer` on `raviartthomas` spaces
+(BEAST src/operator.jl:100-116) to libbeast_hip through `ccall`, and offers the
+resident-matrix solve (FP64 LU, or FP32 LU + FP64 GMRES-IR).
+
+Selection mirrors the reference's threading traits (operator.jl:106-108):
+`assemble(op, X, Y; threading=:gpu)` takes the GPU path; any input the library
+reports as unsupported (other quadstrats, non-RT spaces, dofs with more than two
+shapes) falls back to BEAST's own `:dofsplitting` assembly.
+
+Not exercised in this repository (no Julia toolchain here); the C ABI it binds
+is tested from Python with the same argument conventions.
+"""
+module BEASTHIP
+
+using BEAST
+using CompScienceMeshes
+using LinearAlgebra
+
+const libbeast_hip = get(ENV, "BEAST_HIP_LIB", joinpath(@__DIR__, "..", "csrc", "libbeast_hip.so"))
+
+const BH_OK = Int32(0)
+const BH_ERR_ASSERT = Int32(-4)
+const BH_ERR_UNSUPPORTED = Int32(-5)
+
+struct BHOperator
+    kind::Int32
+    gamma::NTuple{2,Float64}
+    alpha::NTuple{2,Float64}
+    beta::NTuple{2,Float64}
+end
+
+struct BHQuadStrat
+    kind::Int32
+    params::NTuple{8,Int32}
+end
+
+struct BHSolveStats
+    iterations::Int32
+    inner::Int32
+    residual::Float64
+    seconds_factor::Float64
+    seconds_solve::Float64
+end
+
+struct Unsupported <: Exception
+    msg::String
+end
+
+lasterror() = unsafe_string(ccall((:bh_last_error, libbeast_hip), Cstring, ()))
+
+function check(rc)
+    rc == BH_OK && return nothing
+    rc == BH_ERR_ASSERT && throw(AssertionError(lasterror()))
+    rc == BH_ERR_UNSUPPORTED && throw(Unsupported(lasterror()))
+    error(lasterror())
+end
+
+mutable struct Context
+    ptr::Ptr{Cvoid}
+    function Context(device::Integer=0)
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:bh_ctx_create, libbeast_hip), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, r))
+        c = new(r[])
+        finalizer(c -> ccall((:bh_ctx_destroy, libbeast_hip), Cint, (Ptr{Cvoid},), c.ptr), c)
+        return c
+    end
+end
+
+const _ctx = Ref{Union{Nothing,Context}}(nothing)
+context() = (_ctx[] === nothing && (_ctx[] = Context(0)); _ctx[])
+
+c2(z) = (Float64(real(z)), Float64(imag(z)))
+
+_gamma(op) = BEAST.isstatic(op.gamma) ? 0.0 : op.gamma
+
+bhop(op::BEAST.MWSingleLayer3D) = BHOperator(0, c2(_gamma(op)), c2(op.α), c2(op.β))
+bhop(op::BEAST.MWDoubleLayer3D) = BHOperator(1, c2(_gamma(op)), c2(op.alpha), (0.0, 0.0))
+
+pad8(t) = ntuple(i -> i <= length(t) ? Int32(t[i]) : Int32(0), 8)
+bhqs(q::BEAST.DoubleNumWiltonSauterQStrat) = BHQuadStrat(0, pad8((q.outer_rule_far, q.inner_rule_far,
+    q.outer_rule_near, q.inner_rule_near, q.sauter_schwab_common_tetr, q.sauter_schwab_common_face,
+    q.sauter_schwab_common_edge, q.sauter_schwab_common_vert)))
+bhqs(q::BEAST.DoubleNumSauterQstrat) = BHQuadStrat(1, pad8((q.outer_rule, q.inner_rule,
+    q.sauter_schwab_common_tetr, q.sauter_schwab_common_face, q.sauter_schwab_common_edge,
+    q.sauter_schwab_common_vert)))
+bhqs(q::BEAST.DoubleNumQStrat) = BHQuadStrat(2, pad8((q.outer_rule, q.inner_rule)))
+bhqs(q) = throw(Unsupported("quadrature strategy $(typeof(q)) has no GPU path"))
+
+# mesh and space handles, cached per (object, context)
+const _handles = IdDict{Any,Ptr{Cvoid}}()
+
+function meshhandle(Γ, ctx)
+    get!(_handles, Γ) do
+        V = Float64[v[k] for k in 1:3, v in CompScienceMeshes.vertices(Γ)]
+        C = Int32[c[k] - 1 for k in 1:3, c in CompScienceMeshes.cells(Γ)]
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:bh_mesh_create, libbeast_hip), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Int32}, Int64, Ptr{Ptr{Cvoid}}),
+            ctx.ptr, V, size(V, 2), C, size(C, 2), r))
+        r[]
+    end
+end
+
+# RTBasis.fns (rtspace.jl:3-7) in CSR form: the dof map is BEAST's own, unchanged
+function spacehandle(X::BEAST.RTBasis, ctx)
+    get!(_handles, X) do
+        ptr = Int32[0]; cell = Int32[]; ref = Int32[]; coeff = Float64[]
+        for fn in X.fns
+            for sh in fn
+                push!(cell, sh.cellid - 1); push!(ref, sh.refid - 1); push!(coeff, sh.coeff)
+            end
+            push!(ptr, length(cell))
+        end
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:bh_space_create, libbeast_hip), Cint,
+            (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}, Ptr{Ptr{Cvoid}}),
+            meshhandle(X.geo, ctx), length(X.fns), ptr, cell, ref, coeff, r))
+        r[]
+    end
+end
+
+const GPUOp = Union{BEAST.MWSingleLayer3D,BEAST.MWDoubleLayer3D}
+
+"""
+    gpu_assemble!(Z, op, X, Y, qs; scale=1, row_offset=0, col_offset=0)
+
+`Z[row_offset+m, col_offset+n] += scale * A[m, n]` (the store of assemble!,
+operator.jl:168-184, 319-356) computed on the GPU.
+"""
+function gpu_assemble!(Z::Matrix{ComplexF64}, op::GPUOp, X::BEAST.RTBasis, Y::BEAST.RTBasis, qs;
+        scale=1.0, row_offset=0, col_offset=0, ctx=context())
+    o = Ref(bhop(op)); q = Ref(bhqs(qs)); s = Float64[real(scale), imag(scale)]
+    GC.@preserve Z begin
+        check(ccall((:bh_assemble, libbeast_hip), Cint,
+            (Ptr{Cvoid}, Ref{BHOperator}, Ref{BHQuadStrat}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Cint,
+             Ptr{ComplexF64}, Int64, Int64, Int64),
+            ctx.ptr, o, q, 0, spacehandle(X, ctx), spacehandle(Y, ctx), s, 1, Z, size(Z, 1), row_offset, col_offset))
+    end
+    return Z
+end
+
+function BEAST.assemble(op::GPUOp, X::BEAST.RTBasis, Y::BEAST.RTBasis;
+        threading=:dofsplitting, quadstrat=BEAST.defaultquadstrat, kwargs...)
+    if threading == :gpu
+        qs = quadstrat(op, X, Y)
+        Z = zeros(ComplexF64, numfunctions(X), numfunctions(Y))
+        (numfunctions(X) == 0 || numfunctions(Y) == 0) && return Z
+        try
+            return gpu_assemble!(Z, op, X, Y, qs)
+        catch e
+            e isa Unsupported || rethrow()
+            threading = :dofsplitting      # fall back to the reference's CPU path
+        end
+    end
+    return invoke(BEAST.assemble, Tuple{BEAST.AbstractOperator,Any,Any}, op, X, Y;
+        threading=threading, quadstrat=quadstrat, kwargs...)
+end
+
+"""Resident system matrix (HBM) with its LU factors."""
+mutable struct DeviceMatrix
+    ptr::Ptr{Cvoid}
+    n::Int
+    function DeviceMatrix(ptr, n)
+        M = new(ptr, n)
+        finalizer(M -> ccall((:bh_matrix_destroy, libbeast_hip), Cint, (Ptr{Cvoid},), M.ptr), M)
+        return M
+    end
+end
+
+function DeviceMatrix(op::GPUOp, X::BEAST.RTBasis; quadstrat=BEAST.defaultquadstrat, ctx=context())
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:bh_matrix_assemble, libbeast_hip), Cint,
+        (Ptr{Cvoid}, Ref{BHOperator}, Ref{BHQuadStrat}, Ptr{Cvoid}, Ptr{Ptr{Cvoid}}),
+        ctx.ptr, Ref(bhop(op)), Ref(bhqs(quadstrat(op, X, X))), spacehandle(X, ctx), r))
+    return DeviceMatrix(r[], numfunctions(X))
+end
+
+function DeviceMatrix(Z::Matrix{ComplexF64}; ctx=context())
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:bh_matrix_from_host, libbeast_hip), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{ComplexF64}, Int64, Ptr{Ptr{Cvoid}}), ctx.ptr, size(Z, 1), Z, size(Z, 1), r))
+    return DeviceMatrix(r[], size(Z, 1))
+end
+
+"""`factor!(M, :fp64)` (getrf) or `factor!(M, :fp32)` (FP32 LU for GMRES-IR)."""
+function factor!(M::DeviceMatrix, precision::Symbol=:fp64)
+    check(ccall((:bh_matrix_factor, libbeast_hip), Cint, (Ptr{Cvoid}, Cint), M.ptr, precision == :fp64 ? 0 : 1))
+    return M
+end
+
+function LinearAlgebra.ldiv!(x::AbstractVecOrMat{ComplexF64}, M::DeviceMatrix, b::AbstractVecOrMat{ComplexF64};
+        tol=0.0, maxit=0)
+    B = Matrix{ComplexF64}(reshape(b, M.n, :)); Xm = similar(B)
+    st = Ref{BHSolveStats}()
+    check(ccall((:bh_matrix_solve, libbeast_hip), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{ComplexF64}, Ptr{ComplexF64}, Float64, Cint, Ref{BHSolveStats}),
+        M.ptr, size(B, 2), B, Xm, tol, maxit, st))
+    copyto!(x, Xm)
+    return x
+end
+
+Base.:\(M::DeviceMatrix, b::AbstractVecOrMat) = ldiv!(similar(b, ComplexF64), M, ComplexF64.(b))
+
+end # module
