@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <atomic>
 #include <cstring>
 #include <new>
 
@@ -49,6 +50,7 @@ extern "C" int bh_ctx_create(int device, bh_ctx **out)
     c->arch = prop.gcnArchName;
     BH_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *out = c.release();
+    bh::context_created();
     BH_API_END
 }
 
@@ -61,6 +63,7 @@ extern "C" int bh_ctx_destroy(bh_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    bh::context_destroyed();
     BH_API_END
 }
 
@@ -73,6 +76,18 @@ extern "C" int bh_ctx_synchronize(bh_ctx *ctx)
 }
 
 namespace bh {
+namespace {
+std::atomic<int> g_live{0};
+}
+int live_contexts() { return g_live.load(); }
+void context_created() { g_live.fetch_add(1); }
+void context_destroyed() { g_live.fetch_sub(1); }
+std::mutex &coop_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
 void resolve_stats(Ctx &ctx)
 {
     (void)hipStreamSynchronize(ctx.stream);

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <vector>
@@ -149,6 +150,12 @@ struct ProfScope {
     }
 };
 void resolve_stats(Ctx &ctx);
+// number of contexts alive in the process, and the lock that serialises
+// cooperative (grid-barrier) launches between them
+int live_contexts();
+void context_created();
+void context_destroyed();
+std::mutex &coop_mutex();
 
 struct Mesh {
     Ctx *ctx = nullptr;

@@ -20,7 +20,9 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <mutex>
 #include <climits>
 #include <cmath>
 #include <complex>
@@ -291,7 +293,16 @@ __device__ __forceinline__ MfmaVec<double>::v4 mfma16<double>(double a, double b
 
 constexpr int kBK = 16;
 
-template <typename T, int TM, int TN>
+// C[M x N] -= A[M x K] B[K x N], row-major complex, K a multiple of kBK.
+// 256 threads = 2 x 2 waves, wave tile (16 TM) x (16 TN) complex, workgroup
+// tile (32 TM) x (32 TN).  K advances in chunks of 16 through two LDS buffers:
+// the next chunk is loaded into registers while the current one feeds the
+// MFMAs, then stored to the other buffer (one barrier per chunk).
+// M3 = false: four real products per complex product (Cr += ArBr - AiBi,
+// Ci += ArBi + AiBr); M3 = true: three (T1 = ArBr, T2 = AiBi,
+// T3 = (Ar+Ai)(Br+Bi), Cr = T1 - T2, Ci = T3 - T1 - T2), normwise stable and used
+// for the FP32 factorisation that GMRES-IR refines.
+template <typename T, int TM, int TN, bool M3>
 __global__ __launch_bounds__(256) void k_cgemm(const typename CT<T>::t *__restrict__ Ap, long long lda,
                                                const typename CT<T>::t *__restrict__ Bp, long long ldb,
                                                typename CT<T>::t *Cp, long long ldc, int M, int N, int K)
@@ -299,50 +310,90 @@ __global__ __launch_bounds__(256) void k_cgemm(const typename CT<T>::t *__restri
     using V = typename CT<T>::t;
     using v4 = typename MfmaVec<T>::v4;
     constexpr int BM = 32 * TM, BN = 32 * TN;
-    __shared__ V As[BM][kBK + 1];
-    __shared__ V Bs[kBK][BN + 1];
+    constexpr int APT = BM * kBK / 256;          // A elements per thread
+    constexpr int BPT = kBK * BN / 256;          // B elements per thread
+    constexpr int AROW = kBK / APT;              // threads per A row
+    constexpr int BROW = BN / BPT;               // threads per B row
+    constexpr int NACC = M3 ? 3 : 2;
+    __shared__ V As[2][BM][kBK + 1];
+    __shared__ V Bs[2][kBK][BN + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
     const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
-    v4 accr[TM][TN], acci[TM][TN];
+    const int ar = tid / AROW, ac = (tid % AROW) * APT;
+    const int br = tid / BROW, bc = (tid % BROW) * BPT;
+    const bool arow_ok = m0 + ar < M;
+    const V *ag = Ap + (m0 + ar) * lda + ac;
+    const V *bg = Bp + (long long)br * ldb + n0 + bc;
+    V ra[APT], rb[BPT];
+    auto load = [&](int k0) {
 #pragma unroll
-    for (int a = 0; a < TM; ++a)
+        for (int j = 0; j < APT; ++j) ra[j] = arow_ok ? ag[k0 + j] : V{0, 0};
 #pragma unroll
-        for (int b = 0; b < TN; ++b) {
-            accr[a][b] = v4{0, 0, 0, 0};
-            acci[a][b] = v4{0, 0, 0, 0};
-        }
+        for (int j = 0; j < BPT; ++j) rb[j] = (n0 + bc + j < N) ? bg[(long long)k0 * ldb + j] : V{0, 0};
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < APT; ++j) As[buf][ar][ac + j] = ra[j];
+#pragma unroll
+        for (int j = 0; j < BPT; ++j) Bs[buf][br][bc + j] = rb[j];
+    };
+    v4 acc[NACC][TM][TN];
+#pragma unroll
+    for (int q = 0; q < NACC; ++q)
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b) acc[q][a][b] = v4{0, 0, 0, 0};
     const int li = lane & 15, lk = lane >> 4;
+    load(0);
+    store(0);
+    __syncthreads();
+    int cur = 0;
     for (int k0 = 0; k0 < K; k0 += kBK) {
-        for (int idx = tid; idx < BM * kBK; idx += 256) {
-            const int r = idx / kBK, c = idx % kBK;
-            const long long gr = m0 + r;
-            As[r][c] = gr < M ? Ap[gr * lda + k0 + c] : V{0, 0};
-        }
-        for (int idx = tid; idx < kBK * BN; idx += 256) {
-            const int r = idx / BN, c = idx % BN;
-            const long long gc = n0 + c;
-            Bs[r][c] = gc < N ? Bp[(long long)(k0 + r) * ldb + gc] : V{0, 0};
-        }
-        __syncthreads();
+        const bool more = k0 + kBK < K;
+        if (more) load(k0 + kBK);
 #pragma unroll
         for (int kk = 0; kk < kBK; kk += 4) {
             V af[TM], bf[TN];
 #pragma unroll
-            for (int a = 0; a < TM; ++a) af[a] = As[wm * 16 * TM + a * 16 + li][kk + lk];
+            for (int a = 0; a < TM; ++a) af[a] = As[cur][wm * 16 * TM + a * 16 + li][kk + lk];
 #pragma unroll
-            for (int b = 0; b < TN; ++b) bf[b] = Bs[kk + lk][wn * 16 * TN + b * 16 + li];
+            for (int b = 0; b < TN; ++b) bf[b] = Bs[cur][kk + lk][wn * 16 * TN + b * 16 + li];
+            if constexpr (M3) {
+                T as[TM], bs[TN];
 #pragma unroll
-            for (int a = 0; a < TM; ++a)
+                for (int a = 0; a < TM; ++a) as[a] = af[a].x + af[a].y;
 #pragma unroll
-                for (int b = 0; b < TN; ++b) {
-                    accr[a][b] = mfma16<T>(af[a].x, bf[b].x, accr[a][b]);
-                    accr[a][b] = mfma16<T>(-af[a].y, bf[b].y, accr[a][b]);
-                    acci[a][b] = mfma16<T>(af[a].x, bf[b].y, acci[a][b]);
-                    acci[a][b] = mfma16<T>(af[a].y, bf[b].x, acci[a][b]);
-                }
+                for (int b = 0; b < TN; ++b) bs[b] = bf[b].x + bf[b].y;
+#pragma unroll
+                for (int a = 0; a < TM; ++a)
+#pragma unroll
+                    for (int b = 0; b < TN; ++b) {
+                        acc[0][a][b] = mfma16<T>(af[a].x, bf[b].x, acc[0][a][b]);
+                        acc[1][a][b] = mfma16<T>(af[a].y, bf[b].y, acc[1][a][b]);
+                        acc[2][a][b] = mfma16<T>(as[a], bs[b], acc[2][a][b]);
+                    }
+            } else {
+#pragma unroll
+                for (int a = 0; a < TM; ++a)
+#pragma unroll
+                    for (int b = 0; b < TN; ++b) {
+                        acc[0][a][b] = mfma16<T>(af[a].x, bf[b].x, acc[0][a][b]);
+                        acc[1][a][b] = mfma16<T>(af[a].x, bf[b].y, acc[1][a][b]);
+                    }
+#pragma unroll
+                for (int a = 0; a < TM; ++a)
+#pragma unroll
+                    for (int b = 0; b < TN; ++b) {
+                        acc[0][a][b] = mfma16<T>(-af[a].y, bf[b].y, acc[0][a][b]);
+                        acc[1][a][b] = mfma16<T>(af[a].y, bf[b].x, acc[1][a][b]);
+                    }
+            }
         }
+        if (more) store(cur ^ 1);
         __syncthreads();
+        cur ^= 1;
     }
     // output row of register r in lane l (column l % 16): the f32 16x16x4 MFMA
     // keeps 4 consecutive rows per lane group (4 (l/16) + r), the f64 one
@@ -357,9 +408,17 @@ __global__ __launch_bounds__(256) void k_cgemm(const typename CT<T>::t *__restri
                 const long long gi = m0 + wm * 16 * TM + a * 16 + ri;
                 const long long gj = n0 + wn * 16 * TN + b * 16 + (lane & 15);
                 if (gi < M && gj < N) {
+                    T re, im;
+                    if constexpr (M3) {
+                        re = acc[0][a][b][r] - acc[1][a][b][r];
+                        im = acc[2][a][b][r] - acc[0][a][b][r] - acc[1][a][b][r];
+                    } else {
+                        re = acc[0][a][b][r];
+                        im = acc[1][a][b][r];
+                    }
                     V c = Cp[gi * ldc + gj];
-                    c.x -= accr[a][b][r];
-                    c.y -= acci[a][b][r];
+                    c.x -= re;
+                    c.y -= im;
                     Cp[gi * ldc + gj] = c;
                 }
             }
@@ -637,8 +696,9 @@ void gemm(Ctx &ctx, typename CT<T>::t *A, long long lda, int ar, int ac, int br,
     dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
     BH_REQUIRE(grid.y <= 65535, "gemm grid too large");
     ProfScope prof(ctx, "k_cgemm", 8.0 * M * N * K);   // real flops of the complex update
-    k_cgemm<T, TM, TN><<<grid, 256, 0, s>>>(A + (long long)ar * lda + ac, lda, A + (long long)br * lda + bc, lda,
-                                             A + (long long)cr * lda + cc, lda, M, N, K);
+    constexpr bool M3 = sizeof(T) == 4;   // 3M products for the FP32 factorisation only
+    k_cgemm<T, TM, TN, M3><<<grid, 256, 0, s>>>(A + (long long)ar * lda + ac, lda, A + (long long)br * lda + bc, lda,
+                                                 A + (long long)cr * lda + cc, lda, M, N, K);
     check("k_cgemm");
 }
 
@@ -672,7 +732,16 @@ void panel(Ctx &ctx, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, i
     const size_t shm = (size_t)rpb * kPW * sizeof(typename CT<T>::t);
     void *args[] = {&A, (void *)&lda, (void *)&n, (void *)&k0, (void *)&pw, (void *)&rpb, &ipiv, &pw_};
     ProfScope prof(ctx, "k_panel", (double)rows * pw * sizeof(typename CT<T>::t) * 2);
-    BH_CHECK_HIP(hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3(G), dim3(256), args, shm, ctx.stream));
+    // A grid barrier needs all its workgroups resident at once.  Two cooperative
+    // panels on different streams could each hold part of the device and wait
+    // for the rest, so with several live contexts the panels run one at a time.
+    if (live_contexts() > 1) {
+        std::lock_guard<std::mutex> lock(coop_mutex());
+        BH_CHECK_HIP(hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3(G), dim3(256), args, shm, ctx.stream));
+        BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
+    } else {
+        BH_CHECK_HIP(hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3(G), dim3(256), args, shm, ctx.stream));
+    }
 }
 
 template <typename T>

@@ -91,3 +91,30 @@ def wilton_brute(trgauss, p1, p2, p3, x, L=4):
     v = np.array([(W / R ** 3) @ yr, (W / R) @ yr, W @ yr, (W * R) @ yr, (W * R ** 2) @ yr])
     g = np.array([-(v[0] - s[0] * n), v[1] - d * s[1] * n, 2 * (v[2] - d * s[2] * n), 3 * (v[3] - d * s[3] * n)])
     return s, v, g
+
+
+def farfield_magnitudes(bj, u, X, thetas, gamma=1.0j, omega=1.0, c=1.0, degree=8):
+    """|E_far| of the current sum_b u_b f_b at directions (sin t, 0, cos t), as test_farfield.jl:8-40:
+    MWFarField3D(gamma): integrand (y x (exp(gamma y.x) J)) x y, scaled by i omega / (4 pi c)."""
+    mesh = X.geo
+    uv, w = bj.trgauss(degree)
+    V = mesh.vertices[mesh.cells]
+    A = mesh.areas()
+    P = (uv[0][None, :, None] * V[:, 0:1, :] + uv[1][None, :, None] * V[:, 1:2, :]
+         + (1 - uv[0] - uv[1])[None, :, None] * V[:, 2:3, :])
+    W = w[None, :] * 2 * A[:, None]
+    J = np.zeros(P.shape, complex)
+    for b in range(X.numfunctions()):
+        for (cell, r, cf) in X.basisfunction(b):
+            J[cell] += cf * u[b] * (P[cell] - V[cell, r][None, :]) / (2 * A[cell])
+    out = []
+    for th in thetas:
+        y = np.array([np.sin(th), 0.0, np.cos(th)])
+        S = np.einsum("fq,fq,fqk->k", W, np.exp(gamma * (P @ y)), J)
+        out.append(np.linalg.norm(np.cross(np.cross(y, S), y) * (1j * omega / (4 * np.pi * c))))
+    return np.array(out)
+
+
+# test/test_farfield.jl:34-40 (sphere824.in = sphere2.in, EFIE solution at kappa = 1)
+FARFIELD_GOLDEN = [0.6426645162880508, 0.5222644345134998, 0.28287430186444407, 0.3873439919314161,
+                   0.6788129041674158, 0.8800967217249486, 0.9487895695366841]

@@ -71,3 +71,18 @@ def test_efie_end_to_end(bj, orc, golden, precision):
     uo = orc.lu_solve(Zo, orc.planewave(X, bj.z_hat, bj.x_hat, 1j * k))
     assert np.linalg.norm(Zo @ u - b) / np.linalg.norm(b) < 100 * np.finfo(float).eps * (10 if precision == "fp32" else 1)
     assert relfro(u, uo) < 1e-10
+
+
+def test_farfield_golden_gpu(bj, golden):
+    """GPU assembly + RHS + mixed solve reproduce the reference's far-field golden values
+    (test_farfield.jl:34-40) at the oracle's discretisation-level agreement."""
+    import brute
+    m = bj.readmesh(os.path.join(golden, "sphere2.in"))
+    X = bj.raviartthomas(m)
+    t = bj.Maxwell3D.singlelayer(wavenumber=1.0)
+    E = bj.Maxwell3D.planewave(direction=bj.z_hat, polarization=bj.x_hat, wavenumber=1.0)
+    b = bj.assemble(bj.cross(bj.cross(bj.n, E), bj.n), X)
+    M = bj.DeviceMatrix.assemble(t, X).factor("fp32")
+    u = M.solve(b, tol=1e-12)
+    a = brute.farfield_magnitudes(bj, u, X, np.linspace(0, np.pi, 7))
+    assert np.abs(a - np.array(brute.FARFIELD_GOLDEN)).max() < 1e-3
