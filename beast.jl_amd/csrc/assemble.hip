@@ -601,15 +601,18 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
             return;
         }
     }
-    R a[9], b[9], dc[3];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        a[k] = (R)(tg[k] - tg[kGeoC + k % 3]);
-        b[k] = (R)(sg[k] - sg[kGeoC + k % 3]);
-    }
+    R dc[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) dc[k] = (R)(tg[kGeoC + k] - sg[kGeoC + k]);
-    const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
+    // vertices relative to the centroids are only needed once the moments are
+    // summed; loading them late keeps them out of the point loops' registers
+    auto load_verts = [&](R *a, R *b) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            a[k] = (R)(tg[k] - tg[kGeoC + k % 3]);
+            b[k] = (R)(sg[k] - sg[kGeoC + k % 3]);
+        }
+    };
     const R gr = (R)A.op.gr, gi = (R)A.op.gi;
     const R *xq = (const R *)A.tq + (size_t)p * A.Qo * 4;
     const R *yq = (const R *)A.sq + (size_t)q * A.Qi * 4;
@@ -643,6 +646,9 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
             T1.r += wp * (x[0] * S1[0].r + x[1] * S1[1].r + x[2] * S1[2].r);
             T1.i += wp * (x[0] * S1[0].i + x[1] * S1[1].i + x[2] * S1[2].i);
         }
+        R a[9], b[9];
+        load_verts(a, b);
+        const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
         efie_block<R>(A.op, s1, a, b, T0, Tx, Ty, T1, z, stride);
     } else {
         // M1 = sum w x.U, MU = sum w U, MX = sum w x x Rv, MR = sum w Rv with
@@ -688,6 +694,9 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
                 MR[k].r += wp * Rv[k].r; MR[k].i += wp * Rv[k].i;
             }
         }
+        R a[9], b[9];
+        load_verts(a, b);
+        const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -733,46 +742,26 @@ template <typename R, int OPK, bool ROWMAJOR>
 __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
 {
     __shared__ int s_rc[kTileCells], s_cc[kTileCells];
+    __shared__ int s_rslot[kTile][2], s_cslot[kTile][2];
+    __shared__ R s_rcoef[kTile][2], s_ccoef[kTile][2];
     __shared__ cx<R> s_blk[kPairChunk * 9];
     const int tr = blockIdx.y, tc = blockIdx.x;
     const int tid = threadIdx.x;
     const int r0 = A.row.ptr[tr], nr = A.row.ptr[tr + 1] - r0;
     const int c0 = A.col.ptr[tc], nc = A.col.ptr[tc + 1] - c0;
+    const int mbase = tr * kTile, nbase = tc * kTile;
     for (int k = tid; k < nr; k += 256) s_rc[k] = A.row.cells[r0 + k];
     for (int k = tid; k < nc; k += 256) s_cc[k] = A.col.cells[c0 + k];
-
+    if (tid < 2 * kTile) {
+        const int l = tid >> 1, s = tid & 1;
+        const int m = mbase + l, n = nbase + l;
+        s_rslot[l][s] = m < A.row.ndofs ? A.row.slot[2 * m + s] : -1;
+        s_rcoef[l][s] = m < A.row.ndofs ? (R)A.row.coef[2 * m + s] : (R)0;
+        s_cslot[l][s] = n < A.col.ndofs ? A.col.slot[2 * n + s] : -1;
+        s_ccoef[l][s] = n < A.col.ndofs ? (R)A.col.coef[2 * n + s] : (R)0;
+    }
     // ownership: ROWMAJOR -> one column, 16 rows; else one row, 16 columns
     const int own = tid & 63, grp = tid >> 6;
-    int rs[ROWMAJOR ? 32 : 2], cs_[ROWMAJOR ? 2 : 32];
-    R rcf[ROWMAJOR ? 32 : 2], ccf[ROWMAJOR ? 2 : 32];
-    const int mbase = tr * kTile, nbase = tc * kTile;
-    if constexpr (ROWMAJOR) {
-        const int n = nbase + own;
-        for (int s = 0; s < 2; ++s) {
-            cs_[s] = n < A.col.ndofs ? A.col.slot[2 * n + s] : -1;
-            ccf[s] = n < A.col.ndofs ? (R)A.col.coef[2 * n + s] : (R)0;
-        }
-        for (int e = 0; e < 16; ++e) {
-            const int m = mbase + grp * 16 + e;
-            for (int s = 0; s < 2; ++s) {
-                rs[2 * e + s] = m < A.row.ndofs ? A.row.slot[2 * m + s] : -1;
-                rcf[2 * e + s] = m < A.row.ndofs ? (R)A.row.coef[2 * m + s] : (R)0;
-            }
-        }
-    } else {
-        const int m = mbase + own;
-        for (int s = 0; s < 2; ++s) {
-            rs[s] = m < A.row.ndofs ? A.row.slot[2 * m + s] : -1;
-            rcf[s] = m < A.row.ndofs ? (R)A.row.coef[2 * m + s] : (R)0;
-        }
-        for (int e = 0; e < 16; ++e) {
-            const int n = nbase + grp * 16 + e;
-            for (int s = 0; s < 2; ++s) {
-                cs_[2 * e + s] = n < A.col.ndofs ? A.col.slot[2 * n + s] : -1;
-                ccf[2 * e + s] = n < A.col.ndofs ? (R)A.col.coef[2 * n + s] : (R)0;
-            }
-        }
-    }
     cx<R> acc[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = {(R)0, (R)0};
@@ -790,25 +779,23 @@ __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
         // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-            const int *cslot = ROWMAJOR ? cs_ : cs_ + 2 * e;
-            const R *cco = ROWMAJOR ? ccf : ccf + 2 * e;
-            const int *rslot = ROWMAJOR ? rs + 2 * e : rs;
-            const R *rco = ROWMAJOR ? rcf + 2 * e : rcf;
+            const int ml = ROWMAJOR ? grp * 16 + e : own;
+            const int nl = ROWMAJOR ? own : grp * 16 + e;
 #pragma unroll
             for (int sb = 0; sb < 2; ++sb) {
-                const int cslt = cslot[sb];
+                const int cslt = s_cslot[nl][sb];
                 if (cslt < 0) continue;
                 const int lc = (cslt >> 2) - cb;
                 if (lc < 0 || lc >= cn) continue;
                 const int j = cslt & 3;
-                const R bc = cco[sb];
+                const R bc = s_ccoef[nl][sb];
 #pragma unroll
                 for (int sa = 0; sa < 2; ++sa) {
-                    const int rslt = rslot[sa];
+                    const int rslt = s_rslot[ml][sa];
                     if (rslt < 0) continue;
                     const int i = rslt & 3;
                     const cx<R> zz = s_blk[(lc * nr + (rslt >> 2)) + (i + 3 * j) * kPairChunk];
-                    const R ac = rco[sa];
+                    const R ac = s_rcoef[ml][sa];
                     acc[e].r += ac * (zz.r * bc);
                     acc[e].i += ac * (zz.i * bc);
                 }
