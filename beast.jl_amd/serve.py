@@ -218,6 +218,8 @@ class Server:
                 queue.append(None)
                 qlock.notify_all()
 
+        errors = []
+
         def worker(ctx):
             while True:
                 with qlock:
@@ -227,7 +229,11 @@ class Server:
                     if r is None:
                         return
                     queue.pop(0)
-                self.process(r, ctx)
+                try:
+                    self.process(r, ctx)
+                except BaseException as exc:   # reported to the caller after the join
+                    errors.append(exc)
+                    continue
                 finished.append(r)
 
         th = [threading.Thread(target=dispatcher)] + [threading.Thread(target=worker, args=(c,))
@@ -237,6 +243,8 @@ class Server:
         for t in th:
             t.join()
         wall = time.perf_counter() - t0
+        if errors:
+            raise errors[0]
         lat = np.array([r.latency for r in finished])
         return {"requests": len(finished), "wall_s": wall, "throughput_rps": len(finished) / wall,
                 "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
