@@ -54,16 +54,32 @@ extern "C" int bh_ctx_create(int device, bh_ctx **out)
     BH_API_END
 }
 
+namespace bh {
+static void ctx_free(Ctx *c)
+{
+    (void)hipSetDevice(c->device);
+    resolve_stats(*c);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete static_cast<bh_ctx *>(c);
+    context_destroyed();
+}
+
+void ctx_retain(Ctx *c) { c->refs.fetch_add(1); }
+
+void ctx_release(Ctx *c)
+{
+    if (c->refs.fetch_sub(1) == 1 && c->closing.load()) ctx_free(c);
+}
+}  // namespace bh
+
 extern "C" int bh_ctx_destroy(bh_ctx *ctx)
 {
     BH_API_BEGIN
     if (!ctx) return BH_OK;
-    (void)hipSetDevice(ctx->device);
-    resolve_stats(*ctx);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
-    bh::context_destroyed();
+    ctx->closing.store(true);
+    if (ctx->refs.load() > 0) return BH_OK;   // freed with the last mesh or matrix created on it
+    bh::ctx_free(ctx);
     BH_API_END
 }
 
@@ -165,6 +181,7 @@ extern "C" int bh_mesh_create(bh_ctx *ctx, const double *vertices, int64_t nv, c
     }
     m->d_geo.upload(m->geo.data(), m->geo.size(), ctx->stream);
     BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    ctx_retain(ctx);
     *out = m.release();
     BH_API_END
 }
@@ -173,8 +190,10 @@ extern "C" int bh_mesh_destroy(bh_mesh *mesh)
 {
     BH_API_BEGIN
     if (mesh) {
-        (void)hipSetDevice(mesh->ctx->device);
+        Ctx *c = mesh->ctx;
+        (void)hipSetDevice(c->device);
         delete mesh;
+        ctx_release(c);
     }
     BH_API_END
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -39,7 +40,9 @@ struct Error {
         throw ::bh::Error{code};                                                             \
     } while (0)
 
-#define BH_API_BEGIN try {
+#define BH_API_BEGIN                                                                         \
+    (void)hipGetLastError(); /* drop errors left by earlier, unrelated calls */               \
+    try {
 #define BH_API_END                                                                           \
     }                                                                                        \
     catch (const ::bh::Error &e_) { return e_.code; }                                        \
@@ -127,7 +130,13 @@ struct Ctx {
     std::string arch;
     bool profiling = false;
     std::map<std::string, KernelStat> kstats;
+    // objects (meshes, matrices) created on this context; destruction of the
+    // context is deferred until the last of them is gone
+    std::atomic<int> refs{0};
+    std::atomic<bool> closing{false};
 };
+void ctx_retain(Ctx *c);
+void ctx_release(Ctx *c);
 
 // Brackets one launch on ctx.stream when profiling is on.
 struct ProfScope {

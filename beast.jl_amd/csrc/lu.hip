@@ -57,26 +57,52 @@ __device__ __forceinline__ V csubv(V a, V b)
 }
 
 // ---------------------------------------------------------------- grid barrier
-// Sense by generation: the last arriving workgroup resets the count and bumps
-// the generation; the others wait for the bump.  Agent-scope atomics and fences
-// make the data written before the barrier visible to every XCD after it.
+// Two-level barrier by generation.  Workgroup g arrives on the counter of group
+// g % 8 (the XCD it was dispatched to, each counter on its own 128-byte line);
+// the last arrival of a group arrives on the top counter; the last group bumps
+// the generation that everyone waits on.  Counters reset themselves, so the
+// state carries over between launches.  Agent-scope release/acquire makes the
+// data written before the barrier visible to every XCD after it.
+constexpr int kBarStride = 32;   // unsigned words per cache line
+constexpr int kBarWords = kBarStride * 10;
+
 __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned nblocks, unsigned &gen)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
+        const unsigned ngroups = nblocks < 8 ? nblocks : 8;
+        const unsigned grp = blockIdx.x & 7;
+        const unsigned gsize = (nblocks - grp + 7) / 8;
+        unsigned *gcnt = bar + kBarStride * (1 + grp);
+        unsigned *top = bar + kBarStride * 9;
         __threadfence();
-        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen)
-                __builtin_amdgcn_s_sleep(1);
+        const unsigned a = __hip_atomic_fetch_add(gcnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (a == gsize - 1) {
+            __hip_atomic_store(gcnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned t = __hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (t == ngroups - 1) {
+                __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
+        while (__hip_atomic_load(&bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen)
+            __builtin_amdgcn_s_sleep(1);
         __threadfence();
     }
     gen += 1;
     __syncthreads();
+}
+
+// (value, row) argmax across a wave: larger value wins, smaller row on ties
+template <typename T>
+__device__ __forceinline__ void wave_argmax(T &v, int &r)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const T v2 = __shfl_xor(v, off, 64);
+        const int r2 = __shfl_xor(r, off, 64);
+        if (v2 > v || (v2 == v && r2 < r)) { v = v2; r = r2; }
+    }
 }
 
 // ---------------------------------------------------------------- panel
@@ -88,7 +114,7 @@ struct PanelWork {
     int *slot_row;                  // [2][G]
     typename CT<T>::t *cand;        // [2][G][kPW]
     typename CT<T>::t *diag;        // [2][kPW]
-    unsigned *bar;                  // [2]
+    unsigned *bar;                  // [kBarWords]
     int *info;                      // first zero pivot (1-based), 0 if none
 };
 
@@ -100,11 +126,10 @@ __global__ __launch_bounds__(256) void k_panel(typename CT<T>::t *A, long long l
     extern __shared__ char smem_raw[];
     V *L = (V *)smem_raw;   // [rpb][kPW]
     __shared__ V prow[kPW], drow[kPW];
-    __shared__ T rv[256];
-    __shared__ int rr[256];
-    __shared__ int s_brow, s_piv;
+    __shared__ T wv_val[4];
+    __shared__ int wv_row[4];
     const int G = gridDim.x;
-    const int g = blockIdx.x, tid = threadIdx.x;
+    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int r0 = k0 + g * rpb;
     const int r1 = min(n, r0 + rpb);
     const int nrows = max(0, r1 - r0);
@@ -112,7 +137,7 @@ __global__ __launch_bounds__(256) void k_panel(typename CT<T>::t *A, long long l
         const int r = idx / kPW, c = idx % kPW;
         L[idx] = c < pw ? A[(long long)(r0 + r) * lda + k0 + c] : V{0, 0};
     }
-    unsigned gen = __hip_atomic_load(&w.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned gen = __hip_atomic_load(&w.bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     for (int j = 0; j < pw; ++j) {
         const int c = k0 + j;
@@ -127,59 +152,42 @@ __global__ __launch_bounds__(256) void k_panel(typename CT<T>::t *A, long long l
             const T v = fabs(a.x) + fabs(a.y);
             if (v > best || (v == best && grw < brow)) { best = v; brow = grw; }
         }
-        rv[tid] = best;
-        rr[tid] = brow;
+        wave_argmax(best, brow);
+        if (lane == 0) { wv_val[wid] = best; wv_row[wid] = brow; }
         __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (tid < s) {
-                const T v2 = rv[tid + s];
-                const int r2 = rr[tid + s];
-                if (v2 > rv[tid] || (v2 == rv[tid] && r2 < rr[tid])) { rv[tid] = v2; rr[tid] = r2; }
-            }
-            __syncthreads();
-        }
+        best = wv_val[0];
+        brow = wv_row[0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q)
+            if (wv_val[q] > best || (wv_val[q] == best && wv_row[q] < brow)) { best = wv_val[q]; brow = wv_row[q]; }
         if (tid == 0) {
-            w.slot_val[par * G + g] = rv[0];
-            w.slot_row[par * G + g] = rr[0];
-            s_brow = rr[0];
+            w.slot_val[par * G + g] = best;
+            w.slot_row[par * G + g] = brow;
         }
-        __syncthreads();
-        const int bb = s_brow;
-        if (bb != INT_MAX)
-            for (int cc = tid; cc < kPW; cc += 256) w.cand[((size_t)par * G + g) * kPW + cc] = L[(bb - r0) * kPW + cc];
-        if (c >= r0 && c < r1)
-            for (int cc = tid; cc < kPW; cc += 256) w.diag[par * kPW + cc] = L[(c - r0) * kPW + cc];
+        if (brow != INT_MAX && tid < kPW) w.cand[((size_t)par * G + g) * kPW + tid] = L[(brow - r0) * kPW + tid];
+        if (c >= r0 && c < r1 && tid >= 64 && tid < 64 + kPW) w.diag[par * kPW + tid - 64] = L[(c - r0) * kPW + tid - 64];
         grid_barrier(w.bar, G, gen);
         // global winner over the G candidates
         best = (T)-1;
         brow = INT_MAX;
-        int bg = -1;
         for (int t = tid; t < G; t += 256) {
             const T v = w.slot_val[par * G + t];
             const int rw = w.slot_row[par * G + t];
             if (rw == INT_MAX) continue;
-            if (v > best || (v == best && rw < brow)) { best = v; brow = rw; bg = t; }
+            if (v > best || (v == best && rw < brow)) { best = v; brow = rw; }
         }
-        rv[tid] = best;
-        rr[tid] = brow;
+        wave_argmax(best, brow);
+        if (lane == 0) { wv_val[wid] = best; wv_row[wid] = brow; }
         __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (tid < s) {
-                const T v2 = rv[tid + s];
-                const int r2 = rr[tid + s];
-                if (v2 > rv[tid] || (v2 == rv[tid] && r2 < rr[tid])) { rv[tid] = v2; rr[tid] = r2; }
-            }
-            __syncthreads();
-        }
-        if (tid == 0) s_piv = rr[0];
-        __syncthreads();
-        const int p = s_piv;
-        // which workgroup published p: the one owning it
+        int p = wv_row[0];
+        best = wv_val[0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q)
+            if (wv_val[q] > best || (wv_val[q] == best && wv_row[q] < p)) { best = wv_val[q]; p = wv_row[q]; }
+        // the workgroup owning row p published it
         const int gp = (p - k0) / rpb;
-        for (int cc = tid; cc < kPW; cc += 256) {
-            prow[cc] = w.cand[((size_t)par * G + gp) * kPW + cc];
-            drow[cc] = w.diag[par * kPW + cc];
-        }
+        if (tid < kPW) prow[tid] = w.cand[((size_t)par * G + gp) * kPW + tid];
+        else if (tid >= 64 && tid < 64 + kPW) drow[tid - 64] = w.diag[par * kPW + tid - 64];
         __syncthreads();
         const V piv = prow[j];
         if (g == 0 && tid == 0) {
@@ -187,10 +195,8 @@ __global__ __launch_bounds__(256) void k_panel(typename CT<T>::t *A, long long l
             if (piv.x == (T)0 && piv.y == (T)0 && *w.info == 0) *w.info = c + 1;
         }
         if (p != c) {
-            if (p >= r0 && p < r1)
-                for (int cc = tid; cc < kPW; cc += 256) L[(p - r0) * kPW + cc] = drow[cc];
-            if (c >= r0 && c < r1)
-                for (int cc = tid; cc < kPW; cc += 256) L[(c - r0) * kPW + cc] = prow[cc];
+            if (p >= r0 && p < r1 && tid < kPW) L[(p - r0) * kPW + tid] = drow[tid];
+            if (c >= r0 && c < r1 && tid >= 64 && tid < 64 + kPW) L[(c - r0) * kPW + tid - 64] = prow[tid - 64];
         }
         __syncthreads();
         if (piv.x != (T)0 || piv.y != (T)0) {
@@ -204,7 +210,6 @@ __global__ __launch_bounds__(256) void k_panel(typename CT<T>::t *A, long long l
             }
         }
         __syncthreads();
-        (void)bg;
     }
     for (int idx = tid; idx < nrows * kPW; idx += 256) {
         const int r = idx / kPW, c = idx % kPW;
@@ -302,8 +307,8 @@ constexpr int kBK = 16;
 // Ci += ArBi + AiBr); M3 = true: three (T1 = ArBr, T2 = AiBi,
 // T3 = (Ar+Ai)(Br+Bi), Cr = T1 - T2, Ci = T3 - T1 - T2), normwise stable and used
 // for the FP32 factorisation that GMRES-IR refines.
-template <typename T, int TM, int TN, bool M3>
-__global__ __launch_bounds__(256) void k_cgemm(const typename CT<T>::t *__restrict__ Ap, long long lda,
+template <typename T, int TM, int TN, bool M3, int WPE>
+__global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__restrict__ Ap, long long lda,
                                                const typename CT<T>::t *__restrict__ Bp, long long ldb,
                                                typename CT<T>::t *Cp, long long ldc, int M, int N, int K)
 {
@@ -453,20 +458,27 @@ constexpr int kMaxRhs = 8;
 
 // Solve the diagonal block `blk` on xs (LDS, kTB x nrhs, holding the updated
 // right-hand side of the block): unit lower (forward) or upper (backward).
-// All 256 threads call it; wave 0 does the work.
+// The block is first staged in LDS (coalesced), so the serial row sweep runs at
+// LDS latency.  All 256 threads call it; wave 0 does the sweep.
 template <typename T, bool LOWER>
-__device__ void trsv_diag(const typename CT<T>::t *A, long long lda, int n, typename CT<T>::t *xs, int nrhs, int blk)
+__device__ void trsv_diag(const typename CT<T>::t *A, long long lda, int n, typename CT<T>::t *xs, int nrhs, int blk,
+                          typename CT<T>::t (*D)[kTB + 1])
 {
     using V = typename CT<T>::t;
     const int lane = threadIdx.x & 63;
     const int b0 = blk * kTB, bs = min(kTB, n - b0);
+    for (int idx = threadIdx.x; idx < bs * kTB; idx += blockDim.x) {
+        const int i = idx / kTB, j = idx % kTB;
+        D[i][j] = j < bs ? A[(long long)(b0 + i) * lda + b0 + j] : V{0, 0};
+    }
+    __syncthreads();
     if (threadIdx.x < 64) {
         for (int r = 0; r < nrhs; ++r) {
             for (int s = 0; s < bs; ++s) {
                 const int i = LOWER ? s : bs - 1 - s;
                 V prod{0, 0};
                 const bool use = LOWER ? (lane < i) : (lane > i && lane < bs);
-                if (use) prod = cmulv(A[(long long)(b0 + i) * lda + b0 + lane], xs[lane * nrhs + r]);
+                if (use) prod = cmulv(D[i][lane], xs[lane * nrhs + r]);
                 for (int off = 32; off >= 1; off >>= 1) {
                     prod.x += __shfl_xor(prod.x, off, 64);
                     prod.y += __shfl_xor(prod.y, off, 64);
@@ -474,7 +486,7 @@ __device__ void trsv_diag(const typename CT<T>::t *A, long long lda, int n, type
                 if (lane == 0) {
                     V v = csubv(xs[i * nrhs + r], prod);
                     if (!LOWER) {
-                        const V d = A[(long long)(b0 + i) * lda + b0 + i];
+                        const V d = D[i][i];
                         const T den = d.x * d.x + d.y * d.y;
                         v = {(v.x * d.x + v.y * d.y) / den, (v.y * d.x - v.x * d.y) / den};
                     }
@@ -497,10 +509,11 @@ __global__ __launch_bounds__(256) void k_trsv_first(const typename CT<T>::t *A, 
 {
     using V = typename CT<T>::t;
     __shared__ V xs[kTB * kMaxRhs];
+    __shared__ V D[kTB][kTB + 1];
     const int b0 = blk * kTB, bs = min(kTB, n - b0);
     for (int t = threadIdx.x; t < bs * nrhs; t += blockDim.x) xs[t] = X[(long long)b0 * nrhs + t];
     __syncthreads();
-    trsv_diag<T, LOWER>(A, lda, n, xs, nrhs, blk);
+    trsv_diag<T, LOWER>(A, lda, n, xs, nrhs, blk, D);
     for (int t = threadIdx.x; t < bs * nrhs; t += blockDim.x) X[(long long)b0 * nrhs + t] = xs[t];
 }
 
@@ -514,6 +527,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(const typename CT<T>::t *A, l
     using V = typename CT<T>::t;
     __shared__ V xs[kTB * kMaxRhs];
     __shared__ V xb[kTB * kMaxRhs];
+    __shared__ V D[kTB][kTB + 1];
     const int next = LOWER ? blk + 1 : blk - 1;
     const int ch = LOWER ? next + (int)blockIdx.x : next - (int)blockIdx.x;
     if (ch < 0 || ch >= nblk) return;   // uniform per workgroup
@@ -535,7 +549,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(const typename CT<T>::t *A, l
         }
     }
     __syncthreads();
-    if (blockIdx.x == 0) trsv_diag<T, LOWER>(A, lda, n, xs, nrhs, next);
+    if (blockIdx.x == 0) trsv_diag<T, LOWER>(A, lda, n, xs, nrhs, next, D);
     for (int t = threadIdx.x; t < rs * nrhs; t += blockDim.x) X[(long long)r0 * nrhs + t] = xs[t];
 }
 
@@ -684,22 +698,30 @@ struct LUWork {
     int rpb_max = 0;
 };
 
+template <typename T, int TM, int TN, bool M3, int WPE>
+void gemm_launch(Ctx &ctx, typename CT<T>::t *A, long long lda, int ar, int ac, int br, int bc, int cr, int cc, int M,
+                 int N, int K)
+{
+    constexpr int BM = 32 * TM, BN = 32 * TN;
+    dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
+    BH_REQUIRE(grid.y <= 65535, "gemm grid too large");
+    ProfScope prof(ctx, "k_cgemm", 8.0 * M * N * K);   // real flops of the complex update
+    k_cgemm<T, TM, TN, M3, WPE><<<grid, 256, 0, ctx.stream>>>(A + (long long)ar * lda + ac, lda,
+                                                               A + (long long)br * lda + bc, lda,
+                                                               A + (long long)cr * lda + cc, lda, M, N, K);
+    check("k_cgemm");
+}
+
+// Tile shapes chosen by measurement on MI355X (N = 12288 LU): FP32 3M products,
+// 128 x 64 complex tiles, two waves per SIMD; FP64 4M, 64 x 64, two waves per SIMD.
 template <typename T>
 void gemm(Ctx &ctx, typename CT<T>::t *A, long long lda, int ar, int ac, int br, int bc, int cr, int cc, int M, int N,
           int K)
 {
-    hipStream_t s = ctx.stream;
     if (M <= 0 || N <= 0 || K <= 0) return;
-    constexpr int TM = sizeof(T) == 4 ? 4 : 2, TN = 4;
-    constexpr int BM = 32 * TM, BN = 32 * TN;
     BH_REQUIRE(K % kBK == 0, "gemm K=%d not a multiple of %d", K, kBK);
-    dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
-    BH_REQUIRE(grid.y <= 65535, "gemm grid too large");
-    ProfScope prof(ctx, "k_cgemm", 8.0 * M * N * K);   // real flops of the complex update
-    constexpr bool M3 = sizeof(T) == 4;   // 3M products for the FP32 factorisation only
-    k_cgemm<T, TM, TN, M3><<<grid, 256, 0, s>>>(A + (long long)ar * lda + ac, lda, A + (long long)br * lda + bc, lda,
-                                                 A + (long long)cr * lda + cc, lda, M, N, K);
-    check("k_cgemm");
+    if constexpr (sizeof(T) == 4) gemm_launch<T, 4, 2, true, 2>(ctx, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
+    else gemm_launch<T, 2, 2, false, 2>(ctx, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
 }
 
 template <typename T>
@@ -760,11 +782,11 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
     W.slot_row.alloc(2 * (size_t)W.maxG);
     W.cand.alloc(2 * (size_t)W.maxG * kPW);
     W.diag.alloc(2 * kPW);
-    W.bar.alloc(2);
+    W.bar.alloc(kBarWords);
     W.info.alloc(1);
-    BH_CHECK_HIP(hipMemsetAsync(W.bar.p, 0, 2 * sizeof(unsigned), s));
+    BH_CHECK_HIP(hipMemsetAsync(W.bar.p, 0, kBarWords * sizeof(unsigned), s));
     BH_CHECK_HIP(hipMemsetAsync(W.info.p, 0, sizeof(int), s));
-    const int NB = 128;
+    const int NB = 256;   // block column of the right-looking LU (trailing-update depth)
     for (int k = 0; k < n; k += NB) {
         const int kb = std::min(NB, n - k);
         for (int sp = 0; sp < kb; sp += kPW) {

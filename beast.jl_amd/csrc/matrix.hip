@@ -57,6 +57,7 @@ extern "C" int bh_matrix_assemble(bh_ctx *ctx, const bh_operator *op, const bh_q
     M->a64.alloc((size_t)M->n * M->n * 2);
     bh_space *sp = const_cast<bh_space *>(X);
     assemble_device(*ctx, op, qs, false, *sp, *sp, M->a64.p, M->n, true, false, 1.0, 0.0);
+    ctx_retain(ctx);
     *out = M.release();
     BH_API_END
 }
@@ -77,6 +78,7 @@ extern "C" int bh_matrix_from_host(bh_ctx *ctx, int64_t n, const double *A, int6
     k_colmajor_to_rowmajor<<<grid, block, 0, ctx->stream>>>((const double2 *)tmp.p, ld, (double2 *)M->a64.p, n);
     BH_CHECK_HIP(hipGetLastError());
     BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    ctx_retain(ctx);
     *out = M.release();
     BH_API_END
 }
@@ -85,8 +87,10 @@ extern "C" int bh_matrix_destroy(bh_matrix *M)
 {
     BH_API_BEGIN
     if (M) {
-        (void)hipStreamSynchronize(M->ctx->stream);
+        Ctx *c = M->ctx;
+        (void)hipStreamSynchronize(c->stream);
         delete M;
+        ctx_release(c);
     }
     BH_API_END
 }
