@@ -22,7 +22,7 @@ BH_ERR_NOCONV = -7
 
 OP_SINGLELAYER, OP_DOUBLELAYER = 0, 1
 QS_WILTON_SAUTER, QS_SAUTER, QS_DOUBLENUM = 0, 1, 2
-FP64, FP32 = 0, 1
+FP64, FP32, FP32_SPLIT = 0, 1, 2
 
 
 class BHError(RuntimeError):

@@ -276,7 +276,8 @@ class DeviceMatrix:
         return Z
 
     def factor(self, precision="fp64"):
-        nat.check(nat.load().bh_matrix_factor(self.h, {"fp64": nat.FP64, "fp32": nat.FP32}[precision]))
+        nat.check(nat.load().bh_matrix_factor(self.h, {"fp64": nat.FP64, "fp32": nat.FP32,
+                                                       "fp32split": nat.FP32_SPLIT}[precision]))
         return self
 
     def pivots(self):

@@ -57,6 +57,8 @@ extern "C" int bh_ctx_create(int device, bh_ctx **out)
 namespace bh {
 static void ctx_free(Ctx *c)
 {
+    bool expected = false;
+    if (!c->freed.compare_exchange_strong(expected, true)) return;   // exactly once
     (void)hipSetDevice(c->device);
     resolve_stats(*c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);

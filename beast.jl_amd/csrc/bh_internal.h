@@ -134,6 +134,7 @@ struct Ctx {
     // context is deferred until the last of them is gone
     std::atomic<int> refs{0};
     std::atomic<bool> closing{false};
+    std::atomic<bool> freed{false};
 };
 void ctx_retain(Ctx *c);
 void ctx_release(Ctx *c);

@@ -429,6 +429,177 @@ __global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__r
             }
 }
 
+// ---------------------------------------------------------------- split-bf16 GEMM
+// Mixed-path trailing update C -= A B with FP32 complex C.  Each FP32 value x is
+// split into bf16 hi = bf16(x) and lo = bf16(x - hi); a real product is taken as
+// hi*hi + hi*lo + lo*hi (the lo*lo term is below 2^-16 relative), i.e. three
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation, about 16 significant bits per
+// product.  The FP32 LU that GMRES-IR refines tolerates this.  A and B are split
+// once per panel step into planes (Re hi, Re lo, Im hi, Im lo) with k contiguous,
+// so the GEMM streams bf16 through LDS and does no conversions.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kSplitBK = 32;
+constexpr int kSplitPad = 8;   // bf16 padding per LDS row
+
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo)
+{
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+// A (rows x K, row-major complex, lda) -> P[4][rows][K] bf16
+__global__ void k_split_rows(const float2 *__restrict__ A, long long lda, int rows, int K, __bf16 *__restrict__ P)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)rows * K) return;
+    const long long r = idx / K, k = idx % K;
+    const float2 v = A[r * lda + k];
+    const long long plane = (long long)rows * K;
+    split_bf16(v.x, P[idx], P[plane + idx]);
+    split_bf16(v.y, P[2 * plane + idx], P[3 * plane + idx]);
+}
+
+// B (K x cols, row-major complex, ldb) -> P[4][cols][K] bf16 (transposed: k contiguous)
+__global__ __launch_bounds__(256) void k_split_cols(const float2 *__restrict__ B, long long ldb, int K, int cols,
+                                                    __bf16 *__restrict__ P)
+{
+    __shared__ float2 t[32][33];
+    const int c0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int k = k0 + r, c = c0 + tx;
+        t[r][tx] = (k < K && c < cols) ? B[(long long)k * ldb + c] : float2{0, 0};
+    }
+    __syncthreads();
+    const long long plane = (long long)cols * K;
+    for (int r = ty; r < 32; r += 8) {
+        const int c = c0 + r, k = k0 + tx;
+        if (c < cols && k < K) {
+            const float2 v = t[tx][r];
+            const long long o = (long long)c * K + k;
+            split_bf16(v.x, P[o], P[plane + o]);
+            split_bf16(v.y, P[2 * plane + o], P[3 * plane + o]);
+        }
+    }
+}
+
+// C[M x N] -= A B from split planes PA[4][M][K], PB[4][N][K]; K a multiple of 32.
+// 4 waves in 2 x 2, wave tile (16 TM) x (16 TN) complex.
+template <int TM, int TN>
+__global__ __launch_bounds__(256, 2) void k_cgemm_bf16x3(const __bf16 *__restrict__ PA, const __bf16 *__restrict__ PB,
+                                                         float2 *C, long long ldc, int M, int N, int K)
+{
+    constexpr int BM = 32 * TM, BN = 32 * TN, BK = kSplitBK, RS = BK + kSplitPad;
+    __shared__ __align__(16) __bf16 As[4][BM][RS];
+    __shared__ __align__(16) __bf16 Bs[4][BN][RS];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
+    const long long planeA = (long long)M * K, planeB = (long long)N * K;
+    // 16-byte chunks (8 bf16) per chunk of K: A has 4 * BM * 4, B has 4 * BN * 4
+    constexpr int ACH = 4 * BM * (BK / 8), BCH = 4 * BN * (BK / 8);
+    constexpr int AP = (ACH + 255) / 256, BP = (BCH + 255) / 256;
+    uint4 ra[AP], rb[BP];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < AP; ++q) {
+            const int c = tid + 256 * q;
+            if (c < ACH) {
+                const int p = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, g = rem % 4;
+                const long long row = m0 + r;
+                ra[q] = row < M ? *(const uint4 *)(PA + p * planeA + row * K + k0 + 8 * g) : uint4{0, 0, 0, 0};
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < BP; ++q) {
+            const int c = tid + 256 * q;
+            if (c < BCH) {
+                const int p = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, g = rem % 4;
+                const long long col = n0 + r;
+                rb[q] = col < N ? *(const uint4 *)(PB + p * planeB + col * K + k0 + 8 * g) : uint4{0, 0, 0, 0};
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int q = 0; q < AP; ++q) {
+            const int c = tid + 256 * q;
+            if (c < ACH) {
+                const int p = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, g = rem % 4;
+                *(uint4 *)&As[p][r][8 * g] = ra[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < BP; ++q) {
+            const int c = tid + 256 * q;
+            if (c < BCH) {
+                const int p = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, g = rem % 4;
+                *(uint4 *)&Bs[p][r][8 * g] = rb[q];
+            }
+        }
+    };
+    f32x4 arr[TM][TN], aii[TM][TN], ari[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+            arr[a][b] = f32x4{0, 0, 0, 0};
+            aii[a][b] = f32x4{0, 0, 0, 0};
+            ari[a][b] = f32x4{0, 0, 0, 0};
+        }
+    const int li = lane & 15, lg = lane >> 4;   // row/col within a 16-block, k group of 8
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        store();
+        __syncthreads();
+        if (k0 + BK < K) load(k0 + BK);
+        bf16x8 afr[TM][4];
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int p = 0; p < 4; ++p) afr[a][p] = *(const bf16x8 *)&As[p][wm * 16 * TM + a * 16 + li][8 * lg];
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+            bf16x8 bfr[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) bfr[p] = *(const bf16x8 *)&Bs[p][wn * 16 * TN + b * 16 + li][8 * lg];
+#pragma unroll
+            for (int a = 0; a < TM; ++a) {
+                // planes: 0 Re hi, 1 Re lo, 2 Im hi, 3 Im lo
+                arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[0], arr[a][b], 0, 0, 0);
+                aii[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], bfr[2], aii[a][b], 0, 0, 0);
+                ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[2], ari[a][b], 0, 0, 0);
+                ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], bfr[0], ari[a][b], 0, 0, 0);
+                arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[1], arr[a][b], 0, 0, 0);
+                aii[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], bfr[3], aii[a][b], 0, 0, 0);
+                ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[3], ari[a][b], 0, 0, 0);
+                ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], bfr[1], ari[a][b], 0, 0, 0);
+                arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][1], bfr[0], arr[a][b], 0, 0, 0);
+                aii[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][3], bfr[2], aii[a][b], 0, 0, 0);
+                ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][1], bfr[2], ari[a][b], 0, 0, 0);
+                ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][3], bfr[0], ari[a][b], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long long gi = m0 + wm * 16 * TM + a * 16 + 4 * (lane >> 4) + r;
+                const long long gj = n0 + wn * 16 * TN + b * 16 + (lane & 15);
+                if (gi < M && gj < N) {
+                    float2 c = C[gi * ldc + gj];
+                    c.x -= arr[a][b][r] - aii[a][b][r];
+                    c.y -= ari[a][b][r];
+                    C[gi * ldc + gj] = c;
+                }
+            }
+}
+
 // ---------------------------------------------------------------- conversions
 template <typename T>
 __global__ void k_to_work(const double2 *__restrict__ src, int n, typename CT<T>::t *dst, long long lda)
@@ -766,11 +937,40 @@ void panel(Ctx &ctx, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, i
     }
 }
 
+// Trailing update C -= A B through split-bf16 planes (FP32 path only).
+void gemm_split(Ctx &ctx, float2 *A, long long lda, int ar, int ac, int br, int bc, int cr, int cc, int M, int N, int K,
+                DevBuf<__bf16> &PA, DevBuf<__bf16> &PB)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    BH_REQUIRE(K % kSplitBK == 0, "split gemm K=%d not a multiple of %d", K, kSplitBK);
+    hipStream_t s = ctx.stream;
+    if (PA.n < (size_t)4 * M * K) PA.alloc((size_t)4 * M * K);
+    if (PB.n < (size_t)4 * N * K) PB.alloc((size_t)4 * N * K);
+    {
+        ProfScope prof(ctx, "k_split", 8.0 * ((double)M + N) * K * 2.0);
+        const long long na = (long long)M * K;
+        k_split_rows<<<(unsigned)((na + 255) / 256), 256, 0, s>>>(A + (long long)ar * lda + ac, lda, M, K, PA.p);
+        check("k_split_rows");
+        dim3 g2((unsigned)((N + 31) / 32), (unsigned)((K + 31) / 32));
+        k_split_cols<<<g2, 256, 0, s>>>(A + (long long)br * lda + bc, lda, K, N, PB.p);
+        check("k_split_cols");
+    }
+    constexpr int TM = 2, TN = 4, BM = 32 * TM, BN = 32 * TN;
+    dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
+    BH_REQUIRE(grid.y <= 65535, "gemm grid too large");
+    // work: bf16 MFMA flops executed (3 products per real product, 4 real per complex)
+    ProfScope prof(ctx, "k_cgemm_bf16x3", 24.0 * M * N * K);
+    k_cgemm_bf16x3<TM, TN><<<grid, 256, 0, s>>>(PA.p, PB.p, A + (long long)cr * lda + cc, lda, M, N, K);
+    check("k_cgemm_bf16x3");
+}
+
 template <typename T>
-void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *ipiv, int *info_host)
+void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *ipiv, int *info_host,
+               bool split = false)
 {
     hipStream_t s = ctx.stream;
     LUWork<T> W;
+    DevBuf<__bf16> PA, PB;
     // rows per workgroup kept in LDS: 64 KiB of panel slice
     W.rpb_max = (int)(65536 / (kPW * sizeof(typename CT<T>::t)));
     int per_cu = 0;
@@ -806,6 +1006,12 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
                 if (sp + pw < kb)
                     gemm<T>(ctx, A, lda, k + sp + pw, k + sp, k + sp, k + kb, k + sp + pw, k + kb, kb - (sp + pw),
                             n - (k + kb), pw);
+            }
+            if constexpr (sizeof(T) == 4) {
+                if (split) {
+                    gemm_split(ctx, A, lda, k + kb, k, k, k + kb, k + kb, k + kb, n - (k + kb), n - (k + kb), kb, PA, PB);
+                    continue;
+                }
             }
             gemm<T>(ctx, A, lda, k + kb, k, k, k + kb, k + kb, k + kb, n - (k + kb), n - (k + kb), kb);
         }
@@ -843,7 +1049,8 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
 {
     BH_API_BEGIN
     BH_REQUIRE(M != nullptr, "NULL matrix");
-    BH_REQUIRE(precision == BH_FP64 || precision == BH_FP32, "precision must be BH_FP64 or BH_FP32");
+    BH_REQUIRE(precision == BH_FP64 || precision == BH_FP32 || precision == BH_FP32_SPLIT,
+               "precision must be BH_FP64, BH_FP32 or BH_FP32_SPLIT");
     Ctx &ctx = *M->ctx;
     const int64_t n = M->n;
     BH_REQUIRE(n > 0 && n < (int64_t(1) << 31), "bad matrix size");
@@ -866,7 +1073,7 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
         M->lu32.alloc((size_t)lda * n * 2);
         k_to_work<float><<<cgrid, 256, 0, ctx.stream>>>((const double2 *)M->a64.p, (int)n, (float2 *)M->lu32.p, lda);
         check("k_to_work");
-        lu_factor<float>(ctx, (float2 *)M->lu32.p, lda, (int)n, M->ipiv.p, &info);
+        lu_factor<float>(ctx, (float2 *)M->lu32.p, lda, (int)n, M->ipiv.p, &info, precision == BH_FP32_SPLIT);
     }
     BH_CHECK_HIP(hipEventRecord(e1, ctx.stream));
     BH_CHECK_HIP(hipEventSynchronize(e1));
@@ -884,7 +1091,7 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
     M->perm.upload(perm.data(), n, ctx.stream);
     BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
     M->lda = lda;
-    M->factored = info ? -1 : precision;
+    M->factored = info ? -1 : (precision == BH_FP64 ? BH_FP64 : BH_FP32);
     if (info) BH_FAIL(BH_ERR_SINGULAR, "SingularException(%d): exactly zero pivot", info);
     BH_API_END
 }

@@ -52,10 +52,13 @@ enum bh_op_kind { BH_OP_MW_SINGLELAYER = 0, BH_OP_MW_DOUBLELAYER = 1 };
  * parameters are Gauss-Legendre orders 1..16. */
 enum bh_qs_kind { BH_QS_WILTON_SAUTER = 0, BH_QS_SAUTER = 1, BH_QS_DOUBLENUM = 2 };
 
-/* Arithmetic of the far-field quadrature: BH_FP64 everywhere, or BH_FP32 for
- * the far-field double-Gauss only (classification, Sauter-Schwab and Wilton
- * blocks stay FP64 and are rounded when stored). */
-enum bh_precision { BH_FP64 = 0, BH_FP32 = 1 };
+/* Arithmetic.  Assembly: BH_FP64 everywhere, or BH_FP32 for the far-field
+ * double-Gauss only (classification, Sauter-Schwab and Wilton blocks stay FP64
+ * and are rounded when stored).  Factorisation: BH_FP64 (getrf), BH_FP32 (FP32
+ * storage, FP32 MFMA products), or BH_FP32_SPLIT (FP32 storage and accumulation;
+ * the trailing-update products use split bf16 pairs on the bf16 MFMA, about 16
+ * significant bits per product); the FP32 modes are meant for GMRES-IR. */
+enum bh_precision { BH_FP64 = 0, BH_FP32 = 1, BH_FP32_SPLIT = 2 };
 
 typedef struct bh_ctx bh_ctx;
 typedef struct bh_mesh bh_mesh;
@@ -160,7 +163,7 @@ int bh_matrix_destroy(bh_matrix *M);
 int bh_matrix_size(const bh_matrix *M, int64_t *n);
 int bh_matrix_download(bh_matrix *M, double *A, int64_t ld);
 /* LU with partial pivoting (getrf semantics) of the matrix in the given precision:
- * BH_FP64 factors the FP64 matrix, BH_FP32 a rounded FP32 copy. */
+ * BH_FP64 factors the FP64 matrix, BH_FP32 / BH_FP32_SPLIT a rounded FP32 copy. */
 int bh_matrix_factor(bh_matrix *M, int precision);
 /* Solve A X = B for nrhs column-major right-hand sides (ld = n).  With FP64
  * factors: getrs.  With FP32 factors: FP64-residual iterative refinement with
