@@ -3,10 +3,15 @@
 Workload (default ``--config C3``): EFIE (Maxwell3D.singlelayer, k = 4.0) on a
 unit-sphere mesh with 13,334 Fibonacci vertices (26,664 triangles, 39,996 RT
 dofs), plane wave d = x, p = z.  One step = FP64 assembly of the resident
-system matrix (default quadstrat) + FP32 LU on MFMA + GMRES-IR with FP64
-residuals until ||b - A x|| / ||b|| <= 1e-11; the solution's relative error
+system matrix (default quadstrat) + LU of its FP32 copy on MFMA (trailing updates
+from split-bf16 products by default, ``--lu fp32`` for FP32 products) +
+GMRES-IR with FP64 residuals until ||b - A x|| / ||b|| <= 1e-11; the solution's relative error
 against the plain FP64 GPU path (FP64 assembly + FP64 LU) is measured once,
 outside the timed region, and reported.
+
+After the timed steps, a batched sample (64 requests at t = 0, N in [500, 8000])
+and a Poisson serving sample (20 req/s, N up to 30,000) run through the
+continuous-batching service and report throughput and p50/p99 latency.
 
 Prints one JSON line (see the keys below).  ``--gpus`` above 1 is refused:
 this build runs on one GPU.
@@ -27,6 +32,7 @@ import __graft_entry__  # noqa: E402
 METRIC = "EFIE assemble+solve sec at 1e-8 rel err (mixed vs FP64); p50/p99 serve latency"
 PEAK_FP32_MFMA = 157.3e12
 PEAK_FP64_MFMA = 78.6e12
+PEAK_BF16_MFMA = 2.5e15
 
 CONFIGS = {
     # name: (mesh vertices, jitter, seed, wavenumber)
@@ -86,6 +92,11 @@ def main():
     ap.add_argument("--config", default="C3", choices=sorted(CONFIGS))
     ap.add_argument("--tol", type=float, default=1e-11)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lu", default="fp32split", choices=["fp32", "fp32split"],
+                    help="factorisation of the mixed path")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests of the Poisson serving sample (0 skips serving)")
+    ap.add_argument("--batch-requests", type=int, default=64, help="requests of the batched sample (0 skips)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -115,7 +126,7 @@ def main():
         M = bj.DeviceMatrix.assemble(t, X)
         ctx.synchronize()
         ta = time.perf_counter()
-        M.factor("fp32")
+        M.factor(args.lu)
         tf = time.perf_counter()
         x = M.solve(b, tol=args.tol, maxit=10)
         te = time.perf_counter()
@@ -138,32 +149,60 @@ def main():
         phases.append(ph)
         log(f"bench: step {s} {ph}")
     t_all = time.perf_counter() - t_all
-    g_launch, g_sec, g_work = ctx.kernel_stats("k_cgemm")
+    dom = "k_cgemm_bf16x3" if args.lu == "fp32split" else "k_cgemm"
+    g_launch, g_sec, g_work = ctx.kernel_stats(dom)
     f_launch, f_sec, f_work = ctx.kernel_stats("k_far_tiles")
     p_launch, p_sec, _ = ctx.kernel_stats("k_panel")
     ctx.profile(False)
     err = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
     value = t_all / args.steps
     achieved = g_work / g_sec if g_sec > 0 else 0.0
+    if dom == "k_cgemm_bf16x3":
+        peak = PEAK_BF16_MFMA
+        work_note = ("bf16 MFMA flops executed per launch = 24 M N K (complex product as 4 real products, each as "
+                     "3 split-bf16 products); the complex FP32-equivalent work is 8 M N K")
+    else:
+        peak = PEAK_FP32_MFMA
+        work_note = "8 M N K real flops per launch (complex update, counted as 4 real products)"
     out = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * value, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "c64 LU, c128 assembly/residual", "data": "synthetic",
         "config": {"workload": f"EFIE assemble+solve, unit sphere {F} triangles / {N} RT dofs, k={k}, "
-                               f"mixed precision (FP32 LU + FP64 GMRES-IR to {args.tol:g})",
+                               f"mixed precision ({args.lu} LU + FP64 GMRES-IR to {args.tol:g})",
                    "config": args.config, "dofs": N, "triangles": F, "wavenumber": k},
         "rel_err_vs_fp64": err,
         "fp64_path_s": t_fp64,
         "speedup_vs_fp64_path": t_fp64 / value,
         "phases": {key: float(np.median([p[key] for p in phases])) for key in phases[0]},
-        "roofline": {"kernel": "k_cgemm<float> (LU trailing updates)", "bound": "mfma",
-                     "achieved": achieved / 1e12, "peak": PEAK_FP32_MFMA / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_MFMA, "traffic": None,
+        "roofline": {"kernel": dom + " (LU trailing updates)", "bound": "mfma",
+                     "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None, "work": work_note,
                      "launches": g_launch // max(1, args.steps), "avg_launch_ms": 1e3 * g_sec / max(1, g_launch)},
         "kernels": {"k_far_tiles": {"launches": f_launch, "seconds": f_sec,
                                     "gflops_ref_formulation": f_work / max(f_sec, 1e-30) / 1e9},
                     "k_panel": {"launches": p_launch, "seconds": p_sec}},
     }
+    if args.serve_requests > 0 or args.batch_requests > 0:
+        from importlib import import_module
+        sv = import_module("beast_jl_amd.serve")
+        if args.batch_requests > 0:
+            log(f"bench: batched sample ({args.batch_requests} requests at t=0)")
+            srv = sv.Server(workers=3, tol=1e-10)
+            st = srv.run(sv.batched_requests(seed=4, count=args.batch_requests), realtime=False)
+            srv.close()
+            out["batched"] = dict(st, workload="seeded mix of spheres / 2x1x1 ellipsoids / cubes, N log-uniform in "
+                                             "[500, 8000], k ~ U[0.5, 8], 1-4 plane waves each, EFIE, mixed precision")
+            log(f"bench: batched {st}")
+        if args.serve_requests > 0:
+            log(f"bench: serving sample ({args.serve_requests} Poisson arrivals at 20/s)")
+            srv = sv.Server(workers=3, tol=1e-10)
+            st = srv.run(sv.poisson_requests(seed=5, rate=20.0, count=args.serve_requests, nmax=30000))
+            srv.close()
+            out["serving"] = dict(st, workload=f"{args.serve_requests} requests, Poisson 20/s (seed 5), geometry mix "
+                                              "with N log-uniform in [500, 30000], 30% repeats on resident factors, "
+                                              "EFIE, mixed precision")
+            log(f"bench: serving {st}")
     if not args.no_cpu_baseline:
         log("bench: CPU baseline sample")
         out["cpu_baseline"] = cpu_baseline(bj, mesh, X, t, k)
