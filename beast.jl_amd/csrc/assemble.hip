@@ -137,7 +137,7 @@ template <typename R>
 __device__ __forceinline__ cx<R> green(R gr, R gi, R dist, R idist)
 {
     R s, c;
-    if constexpr (sizeof(R) == 8) sincos(gi * dist, &s, &c);
+    if constexpr (sizeof(R) == 8) sincos_f64(gi * dist, &s, &c);
     else sincosf(gi * dist, &s, &c);
     R amp = idist * (R)kInv4Pi;
     if (gr != (R)0) {
@@ -578,9 +578,11 @@ struct FarArgs {
 };
 
 // Far-field 3x3 block of (test p, trial q) by double Gauss, written to z[k*stride].
-template <typename R, int OPK>
+// QO, QI: the point counts when fixed at compile time (0: read from A).
+template <typename R, int OPK, int QO = 0, int QI = 0>
 __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> *z, int stride)
 {
+    const int Qo = QO ? QO : A.Qo, Qi = QI ? QI : A.Qi;
     const double *tg = A.tgeo + (size_t)p * kGeo, *sg = A.sgeo + (size_t)q * kGeo;
     if (A.mode != kModeDoubleNum) {
         int pk;
@@ -614,22 +616,24 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         }
     };
     const R gr = (R)A.op.gr, gi = (R)A.op.gi;
-    const R *xq = (const R *)A.tq + (size_t)p * A.Qo * 4;
-    const R *yq = (const R *)A.sq + (size_t)q * A.Qi * 4;
+    const R *xq = (const R *)A.tq + (size_t)p * Qo * 4;
+    const R *yq = (const R *)A.sq + (size_t)q * Qi * 4;
     if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
         cx<R> T0{0, 0}, T1{0, 0}, Tx[3] = {{0, 0}, {0, 0}, {0, 0}}, Ty[3] = {{0, 0}, {0, 0}, {0, 0}};
-        for (int ip = 0; ip < A.Qo; ++ip) {
+#pragma unroll 1
+        for (int ip = 0; ip < Qo; ++ip) {
             const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
             const R wp = xq[4 * ip + 3];
             const R xd[3] = {x[0] + dc[0], x[1] + dc[1], x[2] + dc[2]};
             cx<R> S0{0, 0}, S1[3] = {{0, 0}, {0, 0}, {0, 0}};
-            for (int iq = 0; iq < A.Qi; ++iq) {
+#pragma unroll
+            for (int iq = 0; iq < Qi; ++iq) {
                 const R y[3] = {yq[4 * iq], yq[4 * iq + 1], yq[4 * iq + 2]};
                 const R wq = yq[4 * iq + 3];
                 const R r[3] = {xd[0] - y[0], xd[1] - y[1], xd[2] - y[2]};
                 const R R2 = dot3(r, r);
                 R dist, idist;
-                if constexpr (sizeof(R) == 8) { dist = sqrt(R2); idist = 1.0 / dist; }
+                if constexpr (sizeof(R) == 8) { idist = rsqrt_f64(R2); dist = R2 * idist; }
                 else { idist = rsqrtf(R2); dist = R2 * idist; }
                 const cx<R> G = green<R>(gr, gi, dist, idist);
                 const cx<R> wg{wq * G.r, wq * G.i};
@@ -655,18 +659,20 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         // U = xd x H1, Rv = xd H0 - H1, H0 = sum w h, H1 = sum w h y, h = -(gamma + 1/R) G / R
         cx<R> M1{0, 0}, MU[3] = {{0, 0}, {0, 0}, {0, 0}}, MX[3] = {{0, 0}, {0, 0}, {0, 0}},
                         MR[3] = {{0, 0}, {0, 0}, {0, 0}};
-        for (int ip = 0; ip < A.Qo; ++ip) {
+#pragma unroll 1
+        for (int ip = 0; ip < Qo; ++ip) {
             const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
             const R wp = xq[4 * ip + 3];
             const R xd[3] = {x[0] + dc[0], x[1] + dc[1], x[2] + dc[2]};
             cx<R> H0{0, 0}, H1[3] = {{0, 0}, {0, 0}, {0, 0}};
-            for (int iq = 0; iq < A.Qi; ++iq) {
+#pragma unroll
+            for (int iq = 0; iq < Qi; ++iq) {
                 const R y[3] = {yq[4 * iq], yq[4 * iq + 1], yq[4 * iq + 2]};
                 const R wq = yq[4 * iq + 3];
                 const R r[3] = {xd[0] - y[0], xd[1] - y[1], xd[2] - y[2]};
                 const R R2 = dot3(r, r);
                 R dist, idist;
-                if constexpr (sizeof(R) == 8) { dist = sqrt(R2); idist = 1.0 / dist; }
+                if constexpr (sizeof(R) == 8) { idist = rsqrt_f64(R2); dist = R2 * idist; }
                 else { idist = rsqrtf(R2); dist = R2 * idist; }
                 const cx<R> G = green<R>(gr, gi, dist, idist);
                 cx<R> h = cmul(cx<R>{gr + idist, gi}, G);
@@ -738,7 +744,7 @@ struct TileArgs {
     double scale_r, scale_i;
 };
 
-template <typename R, int OPK, bool ROWMAJOR>
+template <typename R, int OPK, bool ROWMAJOR, int QO, int QI>
 __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
 {
     __shared__ int s_rc[kTileCells], s_cc[kTileCells];
@@ -773,7 +779,7 @@ __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
         // phase 1: pair blocks of (all row cells) x (column cells cb .. cb+cn)
         if (tid < nr * cn) {
             const int rc = tid % nr, cc = tid / nr;
-            far_block<R, OPK>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + tid, kPairChunk);
+            far_block<R, OPK, QO, QI>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + tid, kPairChunk);
         }
         __syncthreads();
         // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184
@@ -1099,13 +1105,22 @@ void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long lon
     // (EFIE 170, MFIE-K 210; mwops.jl:101-117,141-155 with doublenumints.jl:32-38)
     const double far_pp = ((double)P.nft * P.nfs - P.nd.nent) * P.far.Qo * P.far.Qi;
     ProfScope prof(ctx, "k_far_tiles", far_pp * (sl ? 170.0 : 210.0));
+    // the default far rules (Dunavant degrees 2 and 3: 3 x 4 points) get fully
+    // unrolled point loops
+    const bool dflt = P.far.Qo == 3 && P.far.Qi == 4;
+#define BH_TILES(OPK, RM)                                                               \
+    do {                                                                                \
+        if (dflt) k_far_tiles<R, OPK, RM, 3, 4><<<grid, 256, 0, ctx.stream>>>(A);      \
+        else k_far_tiles<R, OPK, RM, 0, 0><<<grid, 256, 0, ctx.stream>>>(A);           \
+    } while (0)
     if (rowmajor) {
-        if (sl) k_far_tiles<R, BH_OP_MW_SINGLELAYER, true><<<grid, 256, 0, ctx.stream>>>(A);
-        else k_far_tiles<R, BH_OP_MW_DOUBLELAYER, true><<<grid, 256, 0, ctx.stream>>>(A);
+        if (sl) BH_TILES(BH_OP_MW_SINGLELAYER, true);
+        else BH_TILES(BH_OP_MW_DOUBLELAYER, true);
     } else {
-        if (sl) k_far_tiles<R, BH_OP_MW_SINGLELAYER, false><<<grid, 256, 0, ctx.stream>>>(A);
-        else k_far_tiles<R, BH_OP_MW_DOUBLELAYER, false><<<grid, 256, 0, ctx.stream>>>(A);
+        if (sl) BH_TILES(BH_OP_MW_SINGLELAYER, false);
+        else BH_TILES(BH_OP_MW_DOUBLELAYER, false);
     }
+#undef BH_TILES
     check_launch("k_far_tiles");
 }
 

@@ -87,6 +87,60 @@ __device__ __forceinline__ float wave_sum(float v)
     return v;
 }
 
+// 1/sqrt(x) in FP64 for x > 0: hardware estimate refined by two Newton steps,
+// y <- y + y (1 - x y^2) / 2 (each step squares the relative error).
+__device__ __forceinline__ double rsqrt_f64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double r = __builtin_fma(-(x * y), y, 1.0);
+        y = __builtin_fma(0.5 * y, r, y);
+    }
+    return y;
+}
+
+// sin and cos of t in FP64: Cody & Waite reduction by pi/2 (q = nearest integer
+// to 2t/pi; r = t - q pi/2 with pi/2 split into three doubles: the first FMA is
+// exact, since its result is a multiple of 2^-52 below 1 in magnitude, and the
+// split leaves q * 1e-49 unaccounted), then Taylor series of sin (to r^15) and
+// cos (to r^16) on |r| <= pi/4, where the first omitted terms are below 5e-17.
+__device__ __forceinline__ void sincos_f64(double t, double *sp, double *cp)
+{
+    constexpr double kTwoOverPi = 0.6366197723675814;          // 2/pi
+    constexpr double kP1 = 1.5707963267948966;                 // pi/2 = kP1 + kP2 + kP3
+    constexpr double kP2 = 6.123233995736766e-17;
+    constexpr double kP3 = -1.4973849048591698e-33;
+    const double q = __builtin_rint(t * kTwoOverPi);
+    double r = __builtin_fma(-q, kP1, t);
+    r = __builtin_fma(-q, kP2, r);
+    r = __builtin_fma(-q, kP3, r);
+    const double r2 = r * r;
+    // (-1)^k / (2k+1)!, k = 1..7 and (-1)^k / (2k)!, k = 1..8
+    double ps = -1.0 / 1307674368000.0;
+    ps = __builtin_fma(ps, r2, 1.0 / 6227020800.0);
+    ps = __builtin_fma(ps, r2, -1.0 / 39916800.0);
+    ps = __builtin_fma(ps, r2, 1.0 / 362880.0);
+    ps = __builtin_fma(ps, r2, -1.0 / 5040.0);
+    ps = __builtin_fma(ps, r2, 1.0 / 120.0);
+    ps = __builtin_fma(ps, r2, -1.0 / 6.0);
+    const double sn = __builtin_fma(ps * r2, r, r);
+    double pc = 1.0 / 20922789888000.0;
+    pc = __builtin_fma(pc, r2, -1.0 / 87178291200.0);
+    pc = __builtin_fma(pc, r2, 1.0 / 479001600.0);
+    pc = __builtin_fma(pc, r2, -1.0 / 3628800.0);
+    pc = __builtin_fma(pc, r2, 1.0 / 40320.0);
+    pc = __builtin_fma(pc, r2, -1.0 / 720.0);
+    pc = __builtin_fma(pc, r2, 1.0 / 24.0);
+    pc = __builtin_fma(pc, r2, -0.5);
+    const double cs = __builtin_fma(pc, r2, 1.0);
+    // t = q pi/2 + r: rotate by the quadrant
+    const int quad = (int)(q - 4.0 * __builtin_floor(q * 0.25));   // q mod 4, exact for |q| < 2^53
+    const double s1 = (quad & 1) ? cs : sn, c1 = (quad & 1) ? sn : cs;
+    *sp = (quad & 2) ? -s1 : s1;
+    *cp = ((quad + 1) & 2) ? -c1 : c1;
+}
+
 template <typename R>
 __device__ __forceinline__ void cross3(R *o, const R *a, const R *b)
 {

@@ -1,6 +1,8 @@
 // Resident matrix object (row-major complex in HBM) and its factors.
 #pragma once
 
+#include <mutex>
+
 #include "bh_internal.h"
 
 struct bh_matrix {
@@ -9,9 +11,12 @@ struct bh_matrix {
     bh::DevBuf<double> a64;        // FP64 matrix, row-major interleaved complex (n*n*2)
     bh::DevBuf<double> lu64;       // FP64 LU factors (when factored in FP64)
     bh::DevBuf<float> lu32;        // FP32 LU factors (when factored in FP32)
+    bh::DevBuf<float> dinv32;      // inverted 128 x 128 diagonal blocks of the FP32 factors (L, then U)
+    bh::DevBuf<float> x32;         // FP32 solve vector (n x 8 complex)
     bh::DevBuf<long long> ipiv;    // 0-based pivot rows (sequential interchanges)
     bh::DevBuf<int> perm;          // (P b)[i] = b[perm[i]]
     long long lda = 0;             // leading dimension of the factors
     int factored = -1;             // -1 none, BH_FP64 or BH_FP32
     double seconds_factor = 0.0;
+    std::mutex mu;                 // factor and solve calls on one matrix run one at a time
 };

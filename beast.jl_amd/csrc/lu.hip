@@ -217,25 +217,290 @@ __global__ __launch_bounds__(256) void k_panel(typename CT<T>::t *A, long long l
     }
 }
 
-// ---------------------------------------------------------------- row interchanges
-// Apply ipiv[k1..k2) in order to columns [c0, c1) and [c2, c3).
+// ---------------------------------------------------------------- tournament pivoting
+// Panel factorisation with tournament pivoting (TSLU, the panel step of CALU:
+// Grigori, Demmel & Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)), used by the
+// FP32 factorisations that GMRES-IR refines.  Partial pivoting needs one
+// grid-wide exchange per column; here the 32 pivot rows of a panel are chosen
+//   1. per workgroup: partial pivoting over its own rows selects <= 32 candidates,
+//   2. by a reduction tree: each workgroup stacks up to 8 candidate sets (256
+//      rows, original values) and keeps the 32 rows partial pivoting selects,
+//   3. by one workgroup: partial pivoting on the 32 winners gives their order,
+//      L11 and U11 and the sequential interchanges ipiv,
+// after which L21 = A21 U11^-1 row by row.  Every step is an ordinary launch.
+constexpr int kTsluRows = 256;     // rows a workgroup holds in LDS
+constexpr int kTsluFan = 8;        // candidate sets merged per workgroup
+
+// Partial pivoting over up to 256 rows held in registers, one row per thread
+// (v: the row's first kPW entries; rows with used set on entry never pivot).
+// piv[j] (LDS) receives the thread whose row pivots column j (-1 if none is
+// left); the pivot row is broadcast through LDS and the other rows are
+// eliminated in registers.  Largest |Re|+|Im|, smaller thread on ties.
 template <typename T>
-__global__ void k_laswp(typename CT<T>::t *A, long long lda, int k1, int k2, const long long *ipiv, int c0, int c1,
-                        int c2, int c3)
+__device__ __forceinline__ void block_gepp(typename CT<T>::t (&v)[kPW], bool used, int pw, int *piv, T *wv, int *wr,
+                                           typename CT<T>::t *prow)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int w1 = c1 - c0;
-    int col;
+    using V = typename CT<T>::t;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+    for (int j = 0; j < kPW; ++j) {
+        if (j < pw) {   // uniform
+            T best = used ? (T)-1 : fabs(v[j].x) + fabs(v[j].y);
+            int brow = used ? INT_MAX : tid;
+            wave_argmax(best, brow);
+            if (lane == 0) { wv[wid] = best; wr[wid] = brow; }
+            __syncthreads();
+            int p = wr[0];
+            best = wv[0];
+#pragma unroll
+            for (int q = 1; q < 4; ++q)
+                if (wv[q] > best || (wv[q] == best && wr[q] < p)) { best = wv[q]; p = wr[q]; }
+            if (tid == 0 && p == INT_MAX) piv[j] = -1;   // no row left
+            if (tid == p) {
+                piv[j] = p;
+#pragma unroll
+                for (int c = j; c < kPW; ++c) prow[c] = v[c];
+                used = true;
+            }
+            __syncthreads();
+            if (!used && p != INT_MAX) {
+                const V d = prow[j];
+                if (d.x != (T)0 || d.y != (T)0) {
+                    const T den = d.x * d.x + d.y * d.y;
+                    const V l = cmulv(v[j], V{d.x / den, -d.y / den});
+                    v[j] = l;
+#pragma unroll
+                    for (int c = j + 1; c < kPW; ++c) v[c] = csubv(v[c], cmulv(l, prow[c]));
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Stage 1: workgroup g selects up to pw candidates among rows
+// [k0 + g*kTsluRows, k0 + (g+1)*kTsluRows).  Set g of the output: candidate rows
+// (cidx, -1 for none) and their original panel rows (cval, kPW x kPW).
+template <typename T>
+__global__ __launch_bounds__(256) void k_tslu_local(const typename CT<T>::t *A, long long lda, int n, int k0, int pw,
+                                                    int *cidx, typename CT<T>::t *cval)
+{
+    using V = typename CT<T>::t;
+    __shared__ int piv[kPW], wr[4];
+    __shared__ T wv[4];
+    __shared__ V prow[kPW];
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const int r0 = k0 + g * kTsluRows, nrows = min(kTsluRows, n - r0);
+    V v[kPW];
+    const bool mine = tid < nrows;
+#pragma unroll
+    for (int c = 0; c < kPW; ++c) v[c] = (mine && c < pw) ? A[(long long)(r0 + tid) * lda + k0 + c] : V{0, 0};
+    block_gepp<T>(v, !mine, pw, piv, wv, wr, prow);
+    for (int idx = tid; idx < kPW * kPW; idx += 256) {
+        const int j = idx / kPW, c = idx % kPW;
+        const int p = j < pw ? piv[j] : -1;
+        if (c == 0) cidx[g * kPW + j] = p < 0 ? -1 : r0 + p;
+        cval[((size_t)g * kPW + j) * kPW + c] = (p >= 0 && c < pw) ? A[(long long)(r0 + p) * lda + k0 + c] : V{0, 0};
+    }
+}
+
+// Stage 2: workgroup g merges sets [g*kTsluFan, min((g+1)*kTsluFan, nsets)).
+template <typename T>
+__global__ __launch_bounds__(256) void k_tslu_merge(const int *in_idx, const typename CT<T>::t *in_val, int nsets,
+                                                    int pw, int *out_idx, typename CT<T>::t *out_val)
+{
+    using V = typename CT<T>::t;
+    __shared__ int piv[kPW], wr[4];
+    __shared__ T wv[4];
+    __shared__ V prow[kPW];
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const int s0 = g * kTsluFan, nrows = min(kTsluFan, nsets - s0) * kPW;
+    const int *idx_in = in_idx + (size_t)s0 * kPW;
+    const V *val_in = in_val + (size_t)s0 * kPW * kPW;
+    const bool mine = tid < nrows && idx_in[tid] >= 0;
+    V v[kPW];
+#pragma unroll
+    for (int c = 0; c < kPW; ++c) v[c] = mine ? val_in[(size_t)tid * kPW + c] : V{0, 0};
+    block_gepp<T>(v, !mine, pw, piv, wv, wr, prow);
+    for (int idx = tid; idx < kPW * kPW; idx += 256) {
+        const int j = idx / kPW, c = idx % kPW;
+        const int p = j < pw ? piv[j] : -1;
+        if (c == 0) out_idx[g * kPW + j] = p < 0 ? -1 : idx_in[p];
+        out_val[((size_t)g * kPW + j) * kPW + c] = p >= 0 ? val_in[(size_t)p * kPW + c] : V{0, 0};
+    }
+}
+
+// Stage 3 (one workgroup): partial pivoting on the winners orders them and
+// gives L11 \ U11 (top, kPW x kPW, row j = the row that goes to k0 + j) and the
+// sequential interchanges ipiv[k0 .. k0+pw) that bring winner j to row k0 + j.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tslu_finish(const int *widx, const typename CT<T>::t *wval, int k0, int pw,
+                                                     long long *ipiv, typename CT<T>::t *top, int *info)
+{
+    using V = typename CT<T>::t;
+    __shared__ int piv[kPW], wr[4], zero_at;
+    __shared__ T wv[4];
+    __shared__ V prow[kPW];
+    __shared__ long long rows[2 * kPW], at[2 * kPW];   // rows moved so far and their positions
+    const int tid = threadIdx.x;
+    const bool mine = tid < kPW && widx[tid] >= 0;
+    V v[kPW];
+#pragma unroll
+    for (int c = 0; c < kPW; ++c) v[c] = mine ? wval[(size_t)tid * kPW + c] : V{0, 0};
+    if (tid == 0) zero_at = INT_MAX;
+    block_gepp<T>(v, !mine, pw, piv, wv, wr, prow);
+    // the thread whose row pivoted column j writes row j of the factored block
+    int j_of = -1;
+    for (int j = 0; j < pw; ++j)
+        if (piv[j] == tid) j_of = j;
+    if (j_of >= 0) {
+#pragma unroll
+        for (int c = 0; c < kPW; ++c) top[j_of * kPW + c] = c < pw ? v[c] : V{0, 0};
+#pragma unroll
+        for (int c = 0; c < kPW; ++c)
+            if (c == j_of && v[c].x == (T)0 && v[c].y == (T)0) atomicMin(&zero_at, j_of);
+    }
+    for (int idx = tid; idx < kPW * kPW; idx += 256)
+        if (idx / kPW >= pw) top[idx] = V{0, 0};
+    __syncthreads();
+    if (tid == 0) {
+        int nt = 0, first_bad = zero_at;
+        for (int j = 0; j < pw; ++j) {
+            const long long here = k0 + j;
+            const int w = piv[j];
+            if (w < 0) {   // fewer candidate rows than columns: cannot happen for pw <= n - k0
+                ipiv[here] = here;
+                first_bad = min(first_bad, j);
+                continue;
+            }
+            const long long target = widx[w];
+            long long p = target, occupant = here;
+            int it = -1, io = -1;
+            for (int q = 0; q < nt; ++q) {
+                if (rows[q] == target) { p = at[q]; it = q; }
+                if (at[q] == here) { occupant = rows[q]; io = q; }
+            }
+            ipiv[here] = p;
+            if (p != here) {
+                if (it < 0) { it = nt; rows[nt] = target; ++nt; }
+                if (io < 0) { io = nt; rows[nt] = occupant; ++nt; }
+                at[it] = here;
+                at[io] = p;
+            }
+        }
+        if (first_bad != INT_MAX && *info == 0) *info = k0 + first_bad + 1;
+    }
+}
+
+// Stage 4, after the interchanges: rows k0 .. k0+pw of the panel become
+// L11 \ U11, the rows below L21 = A21 U11^-1 (one thread per row).
+template <typename T>
+__global__ __launch_bounds__(256) void k_tslu_apply(typename CT<T>::t *A, long long lda, int n, int k0, int pw,
+                                                    const typename CT<T>::t *top)
+{
+    using V = typename CT<T>::t;
+    __shared__ V U[kPW][kPW + 1];
+    for (int idx = threadIdx.x; idx < kPW * kPW; idx += blockDim.x) U[idx / kPW][idx % kPW] = top[idx];
+    __syncthreads();
+    const long long r = (long long)k0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    V *row = A + r * lda + k0;
+    if (r < k0 + pw) {
+        const int j = (int)(r - k0);
+        for (int c = 0; c < pw; ++c) row[c] = U[j][c];
+        return;
+    }
+    V x[kPW];
+#pragma unroll
+    for (int c = 0; c < kPW; ++c) x[c] = c < pw ? row[c] : V{0, 0};
+#pragma unroll
+    for (int c = 0; c < kPW; ++c) {
+        V s = x[c];
+#pragma unroll
+        for (int i = 0; i < c; ++i) s = csubv(s, cmulv(x[i], U[i][c]));
+        const V d = U[c][c];
+        const T den = d.x * d.x + d.y * d.y;
+        x[c] = (c < pw && den != (T)0) ? V{(s.x * d.x + s.y * d.y) / den, (s.y * d.x - s.x * d.y) / den} : V{0, 0};
+    }
+#pragma unroll
+    for (int c = 0; c < kPW; ++c)
+        if (c < pw) row[c] = x[c];
+}
+
+// ---------------------------------------------------------------- row interchanges
+// Sequential interchanges ipiv[k1..k2) (LAPACK laswp) on a set of columns.
+// The same interchanges as one permutation of the touched rows.  Replaying
+// them sequentially per column costs one dependent global round trip per
+// interchange; instead k_swap_plan replays them once on row indices (LDS) and
+// k_laswp_apply moves every touched row of every column in a single pass.
+// Plan slots: [0, len) are rows k1 + i; [len, 2 len) the rows ipiv[k1 + i] >= k2
+// (slot len + i for the first i naming that row).  plan[s] = the row whose
+// content ends at slot s, plan[2 len + s] = the slot's row (-1: slot unused).
+constexpr int kMaxSwaps = 256;
+
+__global__ __launch_bounds__(256) void k_swap_plan(const long long *ipiv, int k1, int k2, int *plan)
+{
+    __shared__ int s_piv[kMaxSwaps], s_partner[kMaxSwaps], s_src[2 * kMaxSwaps];
+    const int len = k2 - k1, tid = threadIdx.x;
+    for (int i = tid; i < len; i += 256) s_piv[i] = (int)ipiv[k1 + i];
+    __syncthreads();
+    for (int i = tid; i < len; i += 256) {
+        const int p = s_piv[i];
+        int partner = p - k1;
+        if (p >= k2) {
+            int first = i;
+            for (int j = 0; j < i; ++j)
+                if (s_piv[j] == p) { first = j; break; }
+            partner = len + first;
+        }
+        s_partner[i] = partner;
+        s_src[i] = k1 + i;
+        const bool head = p >= k2 && partner == len + i;
+        s_src[len + i] = head ? p : -1;
+        plan[2 * len + i] = k1 + i;
+        plan[3 * len + i] = head ? p : -1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int i = 0; i < len; ++i) {
+            const int q = s_partner[i];
+            const int t = s_src[i];
+            s_src[i] = s_src[q];
+            s_src[q] = t;
+        }
+    }
+    __syncthreads();
+    for (int s = tid; s < 2 * len; s += 256) plan[s] = s_src[s];
+}
+
+// Apply a swap plan of 2 len slots to columns [c0, c1) and [c2, c3): CW
+// columns per workgroup (64 KiB of LDS at most), every moved row read into LDS
+// before any is written.
+template <typename T>
+constexpr int laswp_cols() { return sizeof(T) == 4 ? 16 : 8; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp_apply(typename CT<T>::t *A, long long lda, const int *plan, int len,
+                                                     int c0, int c1, int c2, int c3)
+{
+    using V = typename CT<T>::t;
+    constexpr int CW = laswp_cols<T>(), RL = 256 / CW;
+    extern __shared__ char smem_raw[];
+    V *buf = (V *)smem_raw;   // [2 len][CW]
+    const int tid = threadIdx.x, cl = tid % CW, rl = tid / CW;
+    const int t = blockIdx.x * CW + cl, w1 = c1 - c0;
+    int col = -1;
     if (t < w1) col = c0 + t;
     else if (t < w1 + (c3 - c2)) col = c2 + (t - w1);
-    else return;
-    for (int i = k1; i < k2; ++i) {
-        const long long p = ipiv[i];
-        if (p != i) {
-            auto a = A[(long long)i * lda + col];
-            A[(long long)i * lda + col] = A[p * lda + col];
-            A[p * lda + col] = a;
-        }
+    const int *src = plan, *pos = plan + 2 * len;
+    for (int s = rl; s < 2 * len; s += RL) {
+        const int p = pos[s], q = src[s];
+        if (col >= 0 && p >= 0 && p != q) buf[s * CW + cl] = A[(long long)q * lda + col];
+    }
+    __syncthreads();
+    for (int s = rl; s < 2 * len; s += RL) {
+        const int p = pos[s], q = src[s];
+        if (col >= 0 && p >= 0 && p != q) A[(long long)p * lda + col] = buf[s * CW + cl];
     }
 }
 
@@ -724,6 +989,155 @@ __global__ __launch_bounds__(256) void k_trsv_step(const typename CT<T>::t *A, l
     for (int t = threadIdx.x; t < rs * nrhs; t += blockDim.x) X[(long long)r0 * nrhs + t] = xs[t];
 }
 
+// ---------------------------------------------------------------- FP32 solves with inverted diagonal blocks
+// The FP32 factors only precondition GMRES-IR, so their triangular solves use
+// explicit inverses of the 128 x 128 diagonal blocks (computed once per
+// factorisation): each block step is then two small matrix-vector products with
+// no serial sweep, one launch per block.
+constexpr int kIB = 128;
+
+// In-place inverse of diagonal block blockIdx.x of the unit-lower (LOWER) or
+// upper factor, row by row in LDS: X[i][t] = ([i == t] - sum_k T[i][k] X[k][t]) / T[i][i]
+// over the rows k already inverted (entries outside the triangle are zero).
+// One thread per column; rows past the end of the matrix become identity rows.
+template <bool LOWER>
+__global__ __launch_bounds__(kIB) void k_trinv(const float2 *A, long long lda, int n, float2 *Dinv)
+{
+    __shared__ float2 S[kIB * kIB];
+    const int t = threadIdx.x;
+    const int b0 = blockIdx.x * kIB, bs = min(kIB, n - b0);
+    for (int idx = t; idx < kIB * kIB; idx += kIB) {
+        const int i = idx / kIB, j = idx % kIB;
+        float2 v{0, 0};
+        if (i < bs && j < bs && (LOWER ? j < i : j >= i)) v = A[(long long)(b0 + i) * lda + b0 + j];
+        if (i == j && (LOWER || i >= bs)) v = {1, 0};
+        S[idx] = v;
+    }
+    __syncthreads();
+    for (int s = LOWER ? 1 : 0; s < kIB; ++s) {
+        const int i = LOWER ? s : kIB - 1 - s;
+        float2 acc{0, 0};
+        if (LOWER) {
+            for (int k = 0; k < i; ++k) {
+                const float2 a = S[i * kIB + k], x = S[k * kIB + t];
+                acc.x += a.x * x.x - a.y * x.y;
+                acc.y += a.x * x.y + a.y * x.x;
+            }
+        } else {
+            for (int k = i + 1; k < kIB; ++k) {
+                const float2 a = S[i * kIB + k], x = S[k * kIB + t];
+                acc.x += a.x * x.x - a.y * x.y;
+                acc.y += a.x * x.y + a.y * x.x;
+            }
+        }
+        const float2 d = S[i * kIB + i];
+        __syncthreads();
+        if (LOWER) {
+            if (t < i) S[i * kIB + t] = {-acc.x, -acc.y};
+        } else {
+            const float den = d.x * d.x + d.y * d.y;
+            const float2 inv{d.x / den, -d.y / den};
+            if (t > i) S[i * kIB + t] = {-(acc.x * inv.x - acc.y * inv.y), -(acc.x * inv.y + acc.y * inv.x)};
+            else if (t == i) S[i * kIB + i] = inv;
+        }
+        __syncthreads();
+    }
+    float2 *out = Dinv + (size_t)blockIdx.x * kIB * kIB;
+    for (int idx = t; idx < kIB * kIB; idx += kIB) out[idx] = S[idx];
+}
+
+// out[i] = (SUB ? out[i] : 0) - / + sum_c Mx[i][c] xin[c] for rows i < rs, columns
+// c < cs (xin, out: LDS, row-major x nrhs; they may alias).  256 threads: thread t
+// takes row t / 2 and half t % 2 of the columns, whose 64 entries it loads into
+// registers at once (one memory round trip per block instead of one per entry).
+template <bool SUB>
+__device__ void blk_matvec(const float2 *Mx, long long ld, int rs, int cs, const float2 *xin, float2 *out, int nrhs)
+{
+    constexpr int H = kIB / 2;
+    const int t = threadIdx.x, i = t >> 1, h = t & 1;
+    float2 a[H];
+    const float2 *row = Mx + (long long)min(i, max(rs - 1, 0)) * ld + h * H;
+    if (rs == kIB && cs == kIB) {
+        const float4 *row4 = (const float4 *)row;   // 16-byte aligned: ld and block origins are multiples of 64
+#pragma unroll
+        for (int c = 0; c < H / 2; ++c) {
+            const float4 v = row4[c];
+            a[2 * c] = {v.x, v.y};
+            a[2 * c + 1] = {v.z, v.w};
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < H; ++c) a[c] = (i < rs && h * H + c < cs) ? row[c] : float2{0, 0};
+    }
+    float2 acc[kMaxRhs];
+#pragma unroll
+    for (int r = 0; r < kMaxRhs; ++r) acc[r] = {0, 0};
+#pragma unroll
+    for (int c = 0; c < H; ++c) {
+#pragma unroll
+        for (int r = 0; r < kMaxRhs; ++r) {
+            if (r < nrhs) {
+                const float2 x = xin[(h * H + c) * nrhs + r];
+                acc[r].x += a[c].x * x.x - a[c].y * x.y;
+                acc[r].y += a[c].x * x.y + a[c].y * x.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kMaxRhs; ++r) {
+        acc[r].x += __shfl_xor(acc[r].x, 1, 64);
+        acc[r].y += __shfl_xor(acc[r].y, 1, 64);
+    }
+    __syncthreads();
+    if (i < rs && h == 0) {
+#pragma unroll
+        for (int r = 0; r < kMaxRhs; ++r) {
+            if (r < nrhs) {
+                float2 v = acc[r];
+                if (SUB) v = {out[i * nrhs + r].x - v.x, out[i * nrhs + r].y - v.y};
+                out[i * nrhs + r] = v;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_inv_first(const float2 *Dinv, int n, float2 *X, int nrhs, int blk)
+{
+    __shared__ float2 xs[kIB * kMaxRhs];
+    const int b0 = blk * kIB, bs = min(kIB, n - b0);
+    // rows past the block end read as zero (their matrix entries are zero too, but 0 * garbage is not)
+    for (int t = threadIdx.x; t < kIB * nrhs; t += blockDim.x) xs[t] = t < bs * nrhs ? X[(long long)b0 * nrhs + t] : float2{0, 0};
+    __syncthreads();
+    blk_matvec<false>(Dinv + (size_t)blk * kIB * kIB, kIB, bs, bs, xs, xs, nrhs);
+    for (int t = threadIdx.x; t < bs * nrhs; t += blockDim.x) X[(long long)b0 * nrhs + t] = xs[t];
+}
+
+// Block blk of X is final: every later (LOWER) / earlier block subtracts its
+// product with blk; workgroup 0 owns the adjacent block and finishes it with
+// the inverted diagonal block.
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_inv_step(const float2 *A, long long lda, int n, const float2 *Dinv,
+                                                       float2 *X, int nrhs, int blk, int nblk)
+{
+    __shared__ float2 xs[kIB * kMaxRhs];
+    __shared__ float2 xb[kIB * kMaxRhs];
+    const int next = LOWER ? blk + 1 : blk - 1;
+    const int ch = LOWER ? next + (int)blockIdx.x : next - (int)blockIdx.x;
+    if (ch < 0 || ch >= nblk) return;   // uniform per workgroup
+    const int c0 = blk * kIB, cs = min(kIB, n - c0);
+    const int r0 = ch * kIB, rs = min(kIB, n - r0);
+    for (int t = threadIdx.x; t < kIB * nrhs; t += blockDim.x) {
+        xb[t] = t < cs * nrhs ? X[(long long)c0 * nrhs + t] : float2{0, 0};
+        xs[t] = t < rs * nrhs ? X[(long long)r0 * nrhs + t] : float2{0, 0};
+    }
+    __syncthreads();
+    blk_matvec<true>(A + (long long)r0 * lda + c0, lda, rs, cs, xb, xs, nrhs);
+    if (blockIdx.x == 0) blk_matvec<false>(Dinv + (size_t)next * kIB * kIB, kIB, rs, rs, xs, xs, nrhs);
+    for (int t = threadIdx.x; t < rs * nrhs; t += blockDim.x) X[(long long)r0 * nrhs + t] = xs[t];
+}
+
 // r = b - A x in FP64 (A row-major n x n), one wave per row; x, b, r are n x nrhs row-major.
 __global__ __launch_bounds__(256) void k_residual(const double2 *__restrict__ A, int n, const double2 *__restrict__ x,
                                                   const double2 *__restrict__ b, double2 *__restrict__ r, int nrhs)
@@ -867,6 +1281,9 @@ struct LUWork {
     DevBuf<int> info;
     int maxG = 0;
     int rpb_max = 0;
+    DevBuf<int> plan;                      // row-interchange plan (k_swap_plan)
+    DevBuf<int> tidx[2];                   // tournament candidate sets (row ids)
+    DevBuf<typename CT<T>::t> tval[2], top;   // their panel rows; the factored winners
 };
 
 template <typename T, int TM, int TN, bool M3, int WPE>
@@ -896,13 +1313,21 @@ void gemm(Ctx &ctx, typename CT<T>::t *A, long long lda, int ar, int ac, int br,
 }
 
 template <typename T>
-void laswp(hipStream_t s, typename CT<T>::t *A, long long lda, int k1, int k2, const long long *ipiv, int c0, int c1,
-           int c2, int c3)
+void laswp(Ctx &ctx, typename CT<T>::t *A, long long lda, int k1, int k2, const long long *ipiv, int c0, int c1,
+           int c2, int c3, int *plan)
 {
+    hipStream_t s = ctx.stream;
     const int w = std::max(0, c1 - c0) + std::max(0, c3 - c2);
-    if (w <= 0 || k2 <= k1) return;
-    k_laswp<T><<<(w + 255) / 256, 256, 0, s>>>(A, lda, k1, k2, ipiv, c0, std::max(c0, c1), c2, std::max(c2, c3));
-    check("k_laswp");
+    const int len = k2 - k1;
+    if (w <= 0 || len <= 0) return;
+    ProfScope prof(ctx, "k_laswp", 2.0 * 2 * len * w * sizeof(typename CT<T>::t));   // upper bound: every slot moves
+    BH_REQUIRE(len <= kMaxSwaps, "laswp: %d interchanges exceed %d", len, kMaxSwaps);
+    k_swap_plan<<<1, 256, 0, s>>>(ipiv, k1, k2, plan);
+    check("k_swap_plan");
+    constexpr int CW = laswp_cols<T>();
+    const size_t shm = (size_t)2 * len * CW * sizeof(typename CT<T>::t);
+    k_laswp_apply<T><<<(w + CW - 1) / CW, 256, shm, s>>>(A, lda, plan, len, c0, std::max(c0, c1), c2, std::max(c2, c3));
+    check("k_laswp_apply");
 }
 
 template <typename T>
@@ -935,6 +1360,30 @@ void panel(Ctx &ctx, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, i
     } else {
         BH_CHECK_HIP(hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3(G), dim3(256), args, shm, ctx.stream));
     }
+}
+
+// Panel of columns [k0, k0 + pw) by tournament pivoting: selects the pivot rows,
+// writes ipiv[k0 .. k0+pw) and the factored pw x pw block to W.top.  The caller
+// interchanges the rows, then k_tslu_apply finishes the panel.
+template <typename T>
+void tslu_select(Ctx &ctx, LUWork<T> &W, const typename CT<T>::t *A, long long lda, int n, int k0, int pw,
+                 long long *ipiv)
+{
+    hipStream_t s = ctx.stream;
+    int nsets = (n - k0 + kTsluRows - 1) / kTsluRows;
+    k_tslu_local<T><<<nsets, 256, 0, s>>>(A, lda, n, k0, pw, W.tidx[0].p, W.tval[0].p);
+    check("k_tslu_local");
+    int cur = 0;
+    while (nsets > 1) {
+        const int nout = (nsets + kTsluFan - 1) / kTsluFan;
+        k_tslu_merge<T><<<nout, 256, 0, s>>>(W.tidx[cur].p, W.tval[cur].p, nsets, pw, W.tidx[cur ^ 1].p,
+                                             W.tval[cur ^ 1].p);
+        check("k_tslu_merge");
+        cur ^= 1;
+        nsets = nout;
+    }
+    k_tslu_finish<T><<<1, 256, 0, s>>>(W.tidx[cur].p, W.tval[cur].p, k0, pw, ipiv, W.top.p, W.info.p);
+    check("k_tslu_finish");
 }
 
 // Trailing update C -= A B through split-bf16 planes (FP32 path only).
@@ -984,21 +1433,44 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
     W.diag.alloc(2 * kPW);
     W.bar.alloc(kBarWords);
     W.info.alloc(1);
+    W.plan.alloc(4 * kMaxSwaps);
     BH_CHECK_HIP(hipMemsetAsync(W.bar.p, 0, kBarWords * sizeof(unsigned), s));
     BH_CHECK_HIP(hipMemsetAsync(W.info.p, 0, sizeof(int), s));
+    // FP32 factors only precondition GMRES-IR: their panels use tournament
+    // pivoting; FP64 keeps LAPACK's partial pivoting (the same ipiv as zgetrf).
+    const bool tournament = sizeof(T) == 4;
+    if (tournament) {
+        const size_t sets = (size_t)(n + kTsluRows - 1) / kTsluRows;
+        for (int q = 0; q < 2; ++q) {
+            W.tidx[q].alloc(sets * kPW);
+            W.tval[q].alloc(sets * kPW * kPW);
+        }
+        W.top.alloc((size_t)kPW * kPW);
+    }
     const int NB = 256;   // block column of the right-looking LU (trailing-update depth)
     for (int k = 0; k < n; k += NB) {
         const int kb = std::min(NB, n - k);
         for (int sp = 0; sp < kb; sp += kPW) {
             const int c = k + sp, pw = std::min(kPW, kb - sp);
-            panel<T>(ctx, W, A, lda, n, c, pw, ipiv);
-            laswp<T>(s, A, lda, c, c + pw, ipiv, k, c, c + pw, k + kb);
+            if constexpr (sizeof(T) == 4) {
+                {
+                    ProfScope prof(ctx, "k_tslu", (double)(n - c) * pw * sizeof(typename CT<T>::t));
+                    tslu_select<T>(ctx, W, A, lda, n, c, pw, ipiv);
+                }
+                laswp<T>(ctx, A, lda, c, c + pw, ipiv, k, k + kb, 0, 0, W.plan.p);
+                ProfScope prof(ctx, "k_tslu_apply", (double)(n - c) * pw * sizeof(typename CT<T>::t) * 2);
+                k_tslu_apply<T><<<(n - c + 255) / 256, 256, 0, s>>>(A, lda, n, c, pw, W.top.p);
+                check("k_tslu_apply");
+            } else {
+                panel<T>(ctx, W, A, lda, n, c, pw, ipiv);
+                laswp<T>(ctx, A, lda, c, c + pw, ipiv, k, c, c + pw, k + kb, W.plan.p);
+            }
             if (c + pw < k + kb) {
                 trsm_unit<T>(s, A, lda, c, pw, c, c + pw, k + kb - (c + pw));
                 gemm<T>(ctx, A, lda, c + pw, c, c, c + pw, c + pw, c + pw, n - (c + pw), k + kb - (c + pw), pw);
             }
         }
-        laswp<T>(s, A, lda, k, k + kb, ipiv, 0, k, k + kb, n);
+        laswp<T>(ctx, A, lda, k, k + kb, ipiv, 0, k, k + kb, n, W.plan.p);
         if (k + kb < n) {
             for (int sp = 0; sp < kb; sp += kPW) {
                 const int pw = std::min(kPW, kb - sp);
@@ -1039,6 +1511,34 @@ void lu_solve_inplace(hipStream_t s, const typename CT<T>::t *A, long long lda, 
     }
 }
 
+// Inverses of the diagonal blocks of both FP32 factors: Dinv[0 .. nb) lower, [nb .. 2nb) upper.
+void invert_diagonal_blocks(hipStream_t s, const float2 *LU, long long lda, int n, float2 *Dinv)
+{
+    const int nb = (n + kIB - 1) / kIB;
+    k_trinv<true><<<nb, kIB, 0, s>>>(LU, lda, n, Dinv);
+    check("k_trinv");
+    k_trinv<false><<<nb, kIB, 0, s>>>(LU, lda, n, Dinv + (size_t)nb * kIB * kIB);
+    check("k_trinv");
+}
+
+void lu_solve_inv(hipStream_t s, const float2 *A, long long lda, int n, const float2 *Dinv, float2 *X, int nrhs)
+{
+    const int nb = (n + kIB - 1) / kIB;
+    const float2 *DL = Dinv, *DU = Dinv + (size_t)nb * kIB * kIB;
+    k_trsv_inv_first<true><<<1, 256, 0, s>>>(DL, n, X, nrhs, 0);
+    check("k_trsv_inv_first");
+    for (int b = 0; b + 1 < nb; ++b) {
+        k_trsv_inv_step<true><<<nb - 1 - b, 256, 0, s>>>(A, lda, n, DL, X, nrhs, b, nb);
+        check("k_trsv_inv_step");
+    }
+    k_trsv_inv_first<false><<<1, 256, 0, s>>>(DU, n, X, nrhs, nb - 1);
+    check("k_trsv_inv_first");
+    for (int b = nb - 1; b > 0; --b) {
+        k_trsv_inv_step<false><<<b, 256, 0, s>>>(A, lda, n, DU, X, nrhs, b, nb);
+        check("k_trsv_inv_step");
+    }
+}
+
 long long padded_ld(int64_t n) { return (n + 63) / 64 * 64; }
 
 }  // namespace bh
@@ -1051,6 +1551,7 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
     BH_REQUIRE(M != nullptr, "NULL matrix");
     BH_REQUIRE(precision == BH_FP64 || precision == BH_FP32 || precision == BH_FP32_SPLIT,
                "precision must be BH_FP64, BH_FP32 or BH_FP32_SPLIT");
+    std::lock_guard<std::mutex> lock(M->mu);
     Ctx &ctx = *M->ctx;
     const int64_t n = M->n;
     BH_REQUIRE(n > 0 && n < (int64_t(1) << 31), "bad matrix size");
@@ -1064,6 +1565,7 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
     BH_CHECK_HIP(hipEventRecord(e0, ctx.stream));
     if (precision == BH_FP64) {
         M->lu32.release();
+        M->dinv32.release();
         M->lu64.alloc((size_t)lda * n * 2);
         k_to_work<double><<<cgrid, 256, 0, ctx.stream>>>((const double2 *)M->a64.p, (int)n, (double2 *)M->lu64.p, lda);
         check("k_to_work");
@@ -1074,6 +1576,11 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
         k_to_work<float><<<cgrid, 256, 0, ctx.stream>>>((const double2 *)M->a64.p, (int)n, (float2 *)M->lu32.p, lda);
         check("k_to_work");
         lu_factor<float>(ctx, (float2 *)M->lu32.p, lda, (int)n, M->ipiv.p, &info, precision == BH_FP32_SPLIT);
+        if (info == 0) {
+            const size_t nb = (size_t)(n + kIB - 1) / kIB;
+            M->dinv32.alloc(2 * nb * kIB * kIB * 2);
+            invert_diagonal_blocks(ctx.stream, (const float2 *)M->lu32.p, lda, (int)n, (float2 *)M->dinv32.p);
+        }
     }
     BH_CHECK_HIP(hipEventRecord(e1, ctx.stream));
     BH_CHECK_HIP(hipEventSynchronize(e1));
@@ -1110,13 +1617,16 @@ void apply_factors(bh_matrix *M, const double2 *B, double2 *Xd, int nrhs)
         check("k_permute_in");
         lu_solve_inplace<double>(s, (const double2 *)M->lu64.p, M->lda, n, Xd, nrhs);
     } else {
-        DevBuf<float2> x32(cnt);
-        k_permute_in<float><<<g, 256, 0, s>>>(B, n, nrhs, M->perm.p, x32.p);
+        if (M->x32.n < (size_t)cnt * 2) M->x32.alloc((size_t)n * kMaxRhs * 2);
+        float2 *x32 = (float2 *)M->x32.p;
+        k_permute_in<float><<<g, 256, 0, s>>>(B, n, nrhs, M->perm.p, x32);
         check("k_permute_in");
-        lu_solve_inplace<float>(s, (const float2 *)M->lu32.p, M->lda, n, x32.p, nrhs);
-        k_widen<float><<<g, 256, 0, s>>>(x32.p, cnt, Xd);
+        {
+            ProfScope prof(*M->ctx, "k_trsv_inv", 8.0 * n * (double)n);   // one read of both FP32 factors
+            lu_solve_inv(s, (const float2 *)M->lu32.p, M->lda, n, (const float2 *)M->dinv32.p, x32, nrhs);
+        }
+        k_widen<float><<<g, 256, 0, s>>>(x32, cnt, Xd);
         check("k_widen");
-        BH_CHECK_HIP(hipStreamSynchronize(s));
     }
 }
 
@@ -1128,93 +1638,135 @@ double host_norm(const std::vector<double2> &v, int n, int nrhs, int q)
     return std::sqrt(s);
 }
 
-// GMRES on A d = r0 (one right-hand side) left-preconditioned by the FP32 LU, FP64 vectors.
-int gmres_correction(bh_matrix *M, const double2 *r0, double2 *d, int restart, double rtol, int maxinner)
+// Sum of 256 per-block partial dot products (k_dots output, column j of m).
+std::complex<double> sum_partials(const std::vector<double2> &hp, int m, int j)
 {
+    double ax = 0, ay = 0;
+    for (int b = 0; b < 256; ++b) { ax += hp[(size_t)b * m + j].x; ay += hp[(size_t)b * m + j].y; }
+    return {ax, ay};
+}
+
+// GMRES on A d = r0 (one right-hand side) right-preconditioned by the factors,
+// A M^-1 u = r0, d = M^-1 u, with the preconditioned basis Z_k = M^-1 V_k kept
+// (flexible form, so d = Z y needs no further solve).  FP64 vectors and FP64
+// matrix products; classical Gram-Schmidt applied twice; Givens rotations.
+// The Arnoldi residual estimate is the true residual ||r0 - A d|| in exact
+// arithmetic, so the iteration stops at ||r0 - A d|| <= atol.  Restarts use
+// the recomputed true residual.  Returns the number of iterations.
+int gmres_right(bh_matrix *M, const double2 *r0, double r0norm, double2 *d, double atol, int restart, int maxinner)
+{
+    using C = std::complex<double>;
     hipStream_t s = M->ctx->stream;
     const int n = (int)M->n;
     const unsigned g = (unsigned)((n + 255) / 256);
-    DevBuf<double2> Vb((size_t)(restart + 1) * n), w(n), z(n), part((size_t)256 * (restart + 1)), hd(restart + 1);
-    // v0 = M^-1 r0
-    apply_factors(M, r0, z.p, 1);
-    std::vector<double2> hz(n);
-    BH_CHECK_HIP(hipMemcpyAsync(hz.data(), z.p, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
-    BH_CHECK_HIP(hipStreamSynchronize(s));
-    const double beta = host_norm(hz, n, 1, 0);
-    BH_CHECK_HIP(hipMemsetAsync(d, 0, sizeof(double2) * n, s));
-    if (beta == 0) return 0;
-    k_scale_copy<<<g, 256, 0, s>>>(z.p, n, 1.0 / beta, Vb.p);
-    check("k_scale_copy");
-    using C = std::complex<double>;
-    std::vector<C> H((size_t)(restart + 1) * restart, 0.0), cs(restart), sn(restart), gvec(restart + 1, 0.0);
-    gvec[0] = beta;
-    int k = 0, total = 0;
+    DevBuf<double2> Vb((size_t)(restart + 1) * n), Zb((size_t)restart * n), w(n), r(n),
+        part((size_t)256 * (restart + 1)), hd(restart + 1);
     std::vector<double2> hp((size_t)256 * (restart + 1));
-    for (; k < restart && total < maxinner; ++k, ++total) {
-        // w = M^-1 A v_k
-        {
-            ProfScope prof(*M->ctx, "k_matvec", 16.0 * n * (double)n);
-            k_matvec<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, Vb.p + (size_t)k * n, w.p);
-        }
-        check("k_matvec");
-        apply_factors(M, w.p, z.p, 1);
-        // classical Gram-Schmidt, twice
-        std::vector<C> h(k + 1, 0.0);
-        for (int pass = 0; pass < 2; ++pass) {
-            k_dots<<<256, 256, 0, s>>>(Vb.p, n, k + 1, z.p, part.p);
-            check("k_dots");
-            BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * 256 * (k + 1), hipMemcpyDeviceToHost, s));
-            BH_CHECK_HIP(hipStreamSynchronize(s));
-            std::vector<double2> hh(k + 1);
-            for (int j = 0; j <= k; ++j) {
-                double ax = 0, ay = 0;
-                for (int b = 0; b < 256; ++b) { ax += hp[(size_t)b * (k + 1) + j].x; ay += hp[(size_t)b * (k + 1) + j].y; }
-                hh[j] = {ax, ay};
-                h[j] += C(ax, ay);
+    BH_CHECK_HIP(hipMemsetAsync(d, 0, sizeof(double2) * n, s));
+    double beta = r0norm;
+    const double2 *rcur = r0;
+    int total = 0;
+    while (beta > atol && total < maxinner) {
+        k_scale_copy<<<g, 256, 0, s>>>(rcur, n, 1.0 / beta, Vb.p);
+        check("k_scale_copy");
+        std::vector<C> H((size_t)(restart + 1) * restart, 0.0), cs(restart), sn(restart), gvec(restart + 1, 0.0);
+        gvec[0] = beta;
+        int k = 0;
+        bool done = false;
+        for (; k < restart && total < maxinner; ++k, ++total) {
+            double2 *zk = Zb.p + (size_t)k * n;
+            apply_factors(M, Vb.p + (size_t)k * n, zk, 1);
+            {
+                ProfScope prof(*M->ctx, "k_matvec", 16.0 * n * (double)n);
+                k_matvec<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, zk, w.p);
             }
-            hd.upload(hh.data(), k + 1, s);
-            k_axpys<<<g, 256, 0, s>>>(Vb.p, n, k + 1, hd.p, z.p);
-            check("k_axpys");
+            check("k_matvec");
+            std::vector<C> h(k + 1, 0.0);
+            for (int pass = 0; pass < 2; ++pass) {
+                k_dots<<<256, 256, 0, s>>>(Vb.p, n, k + 1, w.p, part.p);
+                check("k_dots");
+                BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * 256 * (k + 1), hipMemcpyDeviceToHost, s));
+                BH_CHECK_HIP(hipStreamSynchronize(s));
+                std::vector<double2> hh(k + 1);
+                for (int j = 0; j <= k; ++j) {
+                    const C v = sum_partials(hp, k + 1, j);
+                    hh[j] = {v.real(), v.imag()};
+                    h[j] += v;
+                }
+                hd.upload(hh.data(), k + 1, s);
+                k_axpys<<<g, 256, 0, s>>>(Vb.p, n, k + 1, hd.p, w.p);
+                check("k_axpys");
+            }
+            k_dots<<<256, 256, 0, s>>>(w.p, n, 1, w.p, part.p);
+            check("k_dots");
+            BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * 256, hipMemcpyDeviceToHost, s));
+            BH_CHECK_HIP(hipStreamSynchronize(s));
+            const double hn = std::sqrt(std::max(0.0, sum_partials(hp, 1, 0).real()));
+            for (int j = 0; j <= k; ++j) H[(size_t)j * restart + k] = h[j];
+            H[(size_t)(k + 1) * restart + k] = hn;
+            for (int j = 0; j < k; ++j) {
+                const C x = H[(size_t)j * restart + k], y = H[(size_t)(j + 1) * restart + k];
+                H[(size_t)j * restart + k] = std::conj(cs[j]) * x + std::conj(sn[j]) * y;
+                H[(size_t)(j + 1) * restart + k] = -sn[j] * x + cs[j] * y;
+            }
+            const C x = H[(size_t)k * restart + k], y = H[(size_t)(k + 1) * restart + k];
+            const double den = std::sqrt(std::norm(x) + std::norm(y));
+            cs[k] = den == 0 ? C(1.0) : x / den;
+            sn[k] = den == 0 ? C(0.0) : y / den;
+            H[(size_t)k * restart + k] = den;
+            H[(size_t)(k + 1) * restart + k] = 0.0;
+            gvec[k + 1] = -sn[k] * gvec[k];
+            gvec[k] = std::conj(cs[k]) * gvec[k];
+            if (hn != 0 && k + 1 < restart) {
+                k_scale_copy<<<g, 256, 0, s>>>(w.p, n, 1.0 / hn, Vb.p + (size_t)(k + 1) * n);
+                check("k_scale_copy");
+            }
+            if (std::abs(gvec[k + 1]) <= atol || hn == 0) { ++k; ++total; done = true; break; }
         }
-        BH_CHECK_HIP(hipMemcpyAsync(hz.data(), z.p, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+        // d += Z y with H y = g (upper triangular k x k)
+        std::vector<C> y(k, 0.0);
+        for (int i = k - 1; i >= 0; --i) {
+            C t = gvec[i];
+            for (int j = i + 1; j < k; ++j) t -= H[(size_t)i * restart + j] * y[j];
+            y[i] = H[(size_t)i * restart + i] == 0.0 ? C(0.0) : t / H[(size_t)i * restart + i];
+        }
+        std::vector<double2> yn(k);
+        for (int j = 0; j < k; ++j) yn[j] = {-y[j].real(), -y[j].imag()};
+        hd.upload(yn.data(), k, s);
+        k_axpys<<<g, 256, 0, s>>>(Zb.p, n, k, hd.p, d);
+        check("k_axpys");
+        if (done || total >= maxinner) break;
+        // restart from the true residual r0 - A d
+        {
+            ProfScope prof(*M->ctx, "k_residual", 16.0 * n * (double)n);
+            k_residual<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, d, r0, r.p, 1);
+        }
+        check("k_residual");
+        k_dots<<<256, 256, 0, s>>>(r.p, n, 1, r.p, part.p);
+        check("k_dots");
+        BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * 256, hipMemcpyDeviceToHost, s));
         BH_CHECK_HIP(hipStreamSynchronize(s));
-        const double hn = host_norm(hz, n, 1, 0);
-        for (int j = 0; j <= k; ++j) H[(size_t)j * restart + k] = h[j];
-        H[(size_t)(k + 1) * restart + k] = hn;
-        // Givens rotations
-        for (int j = 0; j < k; ++j) {
-            const C a = H[(size_t)j * restart + k], b = H[(size_t)(j + 1) * restart + k];
-            H[(size_t)j * restart + k] = std::conj(cs[j]) * a + std::conj(sn[j]) * b;
-            H[(size_t)(j + 1) * restart + k] = -sn[j] * a + cs[j] * b;
-        }
-        const C a = H[(size_t)k * restart + k], b = H[(size_t)(k + 1) * restart + k];
-        const double den = std::sqrt(std::norm(a) + std::norm(b));
-        cs[k] = den == 0 ? C(1.0) : a / den;
-        sn[k] = den == 0 ? C(0.0) : b / den;
-        H[(size_t)k * restart + k] = den;
-        H[(size_t)(k + 1) * restart + k] = 0.0;
-        gvec[k + 1] = -sn[k] * gvec[k];
-        gvec[k] = std::conj(cs[k]) * gvec[k];
-        if (hn != 0) {
-            k_scale_copy<<<g, 256, 0, s>>>(z.p, n, 1.0 / hn, Vb.p + (size_t)(k + 1) * n);
-            check("k_scale_copy");
-        }
-        if (std::abs(gvec[k + 1]) <= rtol * beta || hn == 0) { ++k; ++total; break; }
+        beta = std::sqrt(std::max(0.0, sum_partials(hp, 1, 0).real()));
+        rcur = r.p;
     }
-    // solve the triangular system and form d = V y
-    std::vector<C> y(k, 0.0);
-    for (int i = k - 1; i >= 0; --i) {
-        C t = gvec[i];
-        for (int j = i + 1; j < k; ++j) t -= H[(size_t)i * restart + j] * y[j];
-        y[i] = t / H[(size_t)i * restart + i];
-    }
-    std::vector<double2> yn(k);
-    for (int j = 0; j < k; ++j) yn[j] = {-y[j].real(), -y[j].imag()};
-    hd.upload(yn.data(), k, s);
-    k_axpys<<<g, 256, 0, s>>>(Vb.p, n, k, hd.p, d);
-    check("k_axpys");
     BH_CHECK_HIP(hipStreamSynchronize(s));
     return total;
+}
+
+// Column r of a row-major n x q block: out[i] = X[i q + r] (get) or X[i q + r] += in[i] (add).
+__global__ void k_get_col(const double2 *__restrict__ X, int n, int q, int r, double2 *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = X[(size_t)i * q + r];
+}
+
+__global__ void k_add_col(double2 *__restrict__ X, int n, int q, int r, const double2 *__restrict__ in)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        X[(size_t)i * q + r].x += in[i].x;
+        X[(size_t)i * q + r].y += in[i].y;
+    }
 }
 
 }  // namespace
@@ -1226,6 +1778,7 @@ extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, doub
     BH_REQUIRE(M && B && X, "NULL argument");
     BH_REQUIRE(M->factored == BH_FP64 || M->factored == BH_FP32, "matrix is not factored");
     BH_REQUIRE(nrhs >= 1 && nrhs <= 8, "nrhs must be 1..8");
+    std::lock_guard<std::mutex> lock(M->mu);
     hipStream_t s = M->ctx->stream;
     const int n = (int)M->n, q = (int)nrhs;
     const long long cnt = (long long)n * q;
@@ -1262,28 +1815,22 @@ extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, doub
             std::vector<double> rel(q);
             for (int r = 0; r < q; ++r) {
                 rel[r] = bn[r] > 0 ? host_norm(hr, n, q, r) / bn[r] : 0.0;
+                if (!std::isfinite(rel[r])) BH_FAIL(BH_ERR_NOCONV, "refinement produced a non-finite residual");
                 res = std::max(res, rel[r]);
             }
             const double target = tol > 0 ? tol : 1e-12;
             if (res <= target || iters >= maxit_) break;
             ++iters;
+            const unsigned gn = (unsigned)((n + 255) / 256);
             for (int r = 0; r < q; ++r) {
                 if (rel[r] <= target) continue;
-                std::vector<double2> col(n);
-                for (int i = 0; i < n; ++i) col[i] = hr[(size_t)i * q + r];
-                rcol.upload(col.data(), n, s);
-                inner += gmres_correction(M, rcol.p, dcol.p, 30, 1e-4, 30);
-                std::vector<double2> dcl(n);
-                BH_CHECK_HIP(hipMemcpyAsync(dcl.data(), dcol.p, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
-                BH_CHECK_HIP(hipStreamSynchronize(s));
-                std::vector<double2> xs(cnt);
-                BH_CHECK_HIP(hipMemcpyAsync(xs.data(), dx.p, sizeof(double2) * cnt, hipMemcpyDeviceToHost, s));
-                BH_CHECK_HIP(hipStreamSynchronize(s));
-                for (int i = 0; i < n; ++i) {
-                    xs[(size_t)i * q + r].x += dcl[i].x;
-                    xs[(size_t)i * q + r].y += dcl[i].y;
-                }
-                dx.upload(xs.data(), cnt, s);
+                // solve A d = r_r to 1/2 of the remaining target, then x_r += d
+                k_get_col<<<gn, 256, 0, s>>>(dr.p, n, q, r, rcol.p);
+                check("k_get_col");
+                const double rn = rel[r] * bn[r];
+                inner += gmres_right(M, rcol.p, rn, dcol.p, 0.5 * target * bn[r], 30, 200);
+                k_add_col<<<gn, 256, 0, s>>>(dx.p, n, q, r, dcol.p);
+                check("k_add_col");
             }
         }
     }

@@ -186,9 +186,13 @@ function DeviceMatrix(Z::Matrix{ComplexF64}; ctx=context())
     return DeviceMatrix(r[], size(Z, 1))
 end
 
-"""`factor!(M, :fp64)` (getrf) or `factor!(M, :fp32)` (FP32 LU for GMRES-IR)."""
+const PRECISIONS = Dict(:fp64 => Cint(0), :fp32 => Cint(1), :fp32split => Cint(2))
+
+"""`factor!(M, :fp64)` (getrf), `factor!(M, :fp32)` (FP32 LU for GMRES-IR) or
+`factor!(M, :fp32split)` (FP32 LU whose trailing updates use split-bf16 products)."""
 function factor!(M::DeviceMatrix, precision::Symbol=:fp64)
-    check(ccall((:bh_matrix_factor, libbeast_hip), Cint, (Ptr{Cvoid}, Cint), M.ptr, precision == :fp64 ? 0 : 1))
+    haskey(PRECISIONS, precision) || throw(ArgumentError("precision must be one of $(keys(PRECISIONS))"))
+    check(ccall((:bh_matrix_factor, libbeast_hip), Cint, (Ptr{Cvoid}, Cint), M.ptr, PRECISIONS[precision]))
     return M
 end
 

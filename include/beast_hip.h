@@ -162,8 +162,11 @@ int bh_matrix_from_host(bh_ctx *ctx, int64_t n, const double *A, int64_t ld, bh_
 int bh_matrix_destroy(bh_matrix *M);
 int bh_matrix_size(const bh_matrix *M, int64_t *n);
 int bh_matrix_download(bh_matrix *M, double *A, int64_t ld);
-/* LU with partial pivoting (getrf semantics) of the matrix in the given precision:
- * BH_FP64 factors the FP64 matrix, BH_FP32 / BH_FP32_SPLIT a rounded FP32 copy. */
+/* LU of the resident matrix (getrf semantics: sequential row interchanges, unit
+ * lower L).  BH_FP64 factors the FP64 matrix with LAPACK's partial pivoting (the
+ * same ipiv as zgetrf).  BH_FP32 / BH_FP32_SPLIT factor a rounded FP32 copy; their
+ * panels choose pivot rows by tournament pivoting, so ipiv may differ from
+ * zgetrf's.  Calls on one matrix are serialised with its solves. */
 int bh_matrix_factor(bh_matrix *M, int precision);
 /* Solve A X = B for nrhs column-major right-hand sides (ld = n).  With FP64
  * factors: getrs.  With FP32 factors: FP64-residual iterative refinement with

@@ -35,7 +35,7 @@ def test_lu_fp64_matches_lapack(bj, n):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp32split"])
-@pytest.mark.parametrize("n", [40, 300, 1025])
+@pytest.mark.parametrize("n", [40, 300, 1025, 2100])
 def test_lu_fp32_gmres_ir(bj, n, precision):
     A = _rand(n, 7 * n)
     b = _rand(n, 7 * n + 1)[:, 0]
@@ -47,12 +47,43 @@ def test_lu_fp32_gmres_ir(bj, n, precision):
     assert M.stats["iterations"] >= 1
 
 
-def test_singular_pivot(bj):
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32split"])
+def test_singular_pivot(bj, precision):
     A = _rand(40, 3)
     A[:, 7] = 0.0
     M = bj.DeviceMatrix.from_host(A)
     with pytest.raises(bj.SingularException):
-        M.factor("fp64")
+        M.factor(precision)
+
+
+def _perm_from_ipiv(ipiv1):
+    """(P A)[i] = A[perm[i]] from 1-based sequential interchanges (LAPACK getrf convention)."""
+    perm = np.arange(len(ipiv1))
+    for i, p in enumerate(ipiv1 - 1):
+        perm[i], perm[p] = perm[p], perm[i]
+    return perm
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32split"])
+@pytest.mark.parametrize("n", [300, 2100, 9000])
+def test_fp32_tournament_finds_permutation(bj, n, precision):
+    """A row-permuted, strongly diagonally dominant matrix: every pivot search (per
+    workgroup, at every merge level, and in the final ordering) must pick the
+    dominant entry, so P A is the unpermuted matrix and IR converges at once."""
+    rng = np.random.default_rng(n)
+    B = 0.01 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    B += np.diag(1.0 + rng.uniform(0, 1, n)) * np.exp(2j * np.pi * rng.uniform(0, 1, n))
+    sigma = rng.permutation(n)
+    A = np.asfortranarray(B[sigma])
+    M = bj.DeviceMatrix.from_host(A).factor(precision)
+    ipiv = M.pivots()
+    assert ipiv.shape == (n,) and np.all(ipiv >= np.arange(1, n + 1)) and np.all(ipiv <= n)
+    perm = _perm_from_ipiv(ipiv)
+    assert np.array_equal(sigma[perm], np.arange(n))
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    M.solve(b, tol=1e-12, maxit=10)
+    assert M.stats["residual"] <= 1e-12
+    assert M.stats["iterations"] <= 3
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32split"])
