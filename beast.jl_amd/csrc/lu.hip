@@ -221,210 +221,309 @@ __global__ __launch_bounds__(256) void k_panel(typename CT<T>::t *A, long long l
 // Panel factorisation with tournament pivoting (TSLU, the panel step of CALU:
 // Grigori, Demmel & Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)), used by the
 // FP32 factorisations that GMRES-IR refines.  Partial pivoting needs one
-// grid-wide exchange per column; here the 32 pivot rows of a panel are chosen
-//   1. per workgroup: partial pivoting over its own rows selects <= 32 candidates,
-//   2. by a reduction tree: each workgroup stacks up to 8 candidate sets (256
-//      rows, original values) and keeps the 32 rows partial pivoting selects,
-//   3. by one workgroup: partial pivoting on the 32 winners gives their order,
-//      L11 and U11 and the sequential interchanges ipiv,
-// after which L21 = A21 U11^-1 row by row.  Every step is an ordinary launch.
-constexpr int kTsluRows = 256;     // rows a workgroup holds in LDS
-constexpr int kTsluFan = 8;        // candidate sets merged per workgroup
+// grid-wide exchange per column; here the pw <= 32 pivot rows of a panel are chosen
+//   1. per workgroup of 1024 rows: partial pivoting selects <= pw candidates,
+//   2. by a reduction tree: a workgroup stacks up to 32 candidate sets (their
+//      original rows) and keeps the rows partial pivoting selects,
+// until one workgroup remains.  Partial pivoting on the final stack picks its
+// pivots among the winners in order, and eliminating with winners only leaves
+// the winners' rows exactly as partial pivoting on the winners alone would
+// (no loser is ever a pivot row), so the final workgroup also emits
+// L11 \ U11, LAPACK-style sequential interchanges and a row-move plan.
+// k_tslu_apply then moves the rows and forms L21 = A21 U11^-1.
+constexpr int kTsluT = 256;             // rows (threads) per tournament workgroup
+constexpr int kTsluFan = kTsluT / kPW;  // candidate sets merged per workgroup
+constexpr int kPlanMax = 2 * kPW;       // row moves of one panel
 
-// Partial pivoting over up to 256 rows held in registers, one row per thread
-// (v: the row's first kPW entries; rows with used set on entry never pivot).
-// piv[j] (LDS) receives the thread whose row pivots column j (-1 if none is
-// left); the pivot row is broadcast through LDS and the other rows are
-// eliminated in registers.  Largest |Re|+|Im|, smaller thread on ties.
-template <typename T>
-__device__ __forceinline__ void block_gepp(typename CT<T>::t (&v)[kPW], bool used, int pw, int *piv, T *wv, int *wr,
-                                           typename CT<T>::t *prow)
+// Maximum over a wave of unsigned keys (DPP: quad, half-row and row mirrors, then
+// the row broadcasts of lanes 15 and 31), read from lane 63.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v)
 {
-    using V = typename CT<T>::t;
+    int x = (int)v;
+    x = (int)max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    x = (int)max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    x = (int)max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    x = (int)max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false));  // row_mirror
+    x = (int)max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(x, x, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    x = (int)max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(x, x, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return (unsigned)__builtin_amdgcn_readlane(x, 63);
+}
+
+// Partial pivoting over up to kTsluT rows held in registers, one row per thread
+// (v: the row's first kPW entries; rows with used set never pivot).  piv[j]
+// (LDS) receives the thread whose row pivots column j (-1 if none is left).
+// Pivot choice: largest |Re|+|Im| at 2^-12 relative resolution (key = value bits
+// with the low 11 mantissa bits replaced by 2047 - thread), smaller thread on
+// ties; the pivot row is broadcast through LDS, the others are eliminated in
+// registers.  The pivot rows keep their factored entries (L left of the
+// diagonal, U from it).  The columns run in four phases of eight: phase P
+// updates the columns [8P, kPW) under a predicate, which keeps the code small
+// (a full unroll of the 32 x 32 elimination does not fit the instruction cache).
+template <int P>
+__device__ __forceinline__ void gepp_phase(float2 (&v)[kPW], bool &used, int pw, int *piv, unsigned *s_key,
+                                           float2 *prow)
+{
+    constexpr int C0 = 8 * P, NW = kTsluT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int jend = min(C0 + 8, pw);
+#pragma unroll 1
+    for (int j = C0; j < jend; ++j) {
+        // column j of this row (selects per component: a conditional struct copy
+        // makes the compiler move v to scratch memory)
+        float2 vj = v[C0];
 #pragma unroll
-    for (int j = 0; j < kPW; ++j) {
-        if (j < pw) {   // uniform
-            T best = used ? (T)-1 : fabs(v[j].x) + fabs(v[j].y);
-            int brow = used ? INT_MAX : tid;
-            wave_argmax(best, brow);
-            if (lane == 0) { wv[wid] = best; wr[wid] = brow; }
-            __syncthreads();
-            int p = wr[0];
-            best = wv[0];
+        for (int q = 1; q < 8; ++q) {
+            vj.x = (j == C0 + q) ? v[C0 + q].x : vj.x;
+            vj.y = (j == C0 + q) ? v[C0 + q].y : vj.y;
+        }
+        const float mag = fabsf(vj.x) + fabsf(vj.y);
+        const unsigned key = used ? 0u : ((__float_as_uint(mag) & ~2047u) | (unsigned)(2047 - tid));
+        const unsigned wmax = wave_max_u32(key);
+        if (lane == 0) s_key[wid] = wmax;
+        __syncthreads();
+        unsigned kmax = 0;
 #pragma unroll
-            for (int q = 1; q < 4; ++q)
-                if (wv[q] > best || (wv[q] == best && wr[q] < p)) { best = wv[q]; p = wr[q]; }
-            if (tid == 0 && p == INT_MAX) piv[j] = -1;   // no row left
-            if (tid == p) {
-                piv[j] = p;
+        for (int q = 0; q < NW; ++q) kmax = max(kmax, s_key[q]);
+        const int p = kmax ? 2047 - (int)(kmax & 2047u) : -1;
+        if (tid == 0) piv[j] = p;
+        if (tid == p) {
 #pragma unroll
-                for (int c = j; c < kPW; ++c) prow[c] = v[c];
-                used = true;
-            }
-            __syncthreads();
-            if (!used && p != INT_MAX) {
-                const V d = prow[j];
-                if (d.x != (T)0 || d.y != (T)0) {
-                    const T den = d.x * d.x + d.y * d.y;
-                    const V l = cmulv(v[j], V{d.x / den, -d.y / den});
-                    v[j] = l;
+            for (int cc = C0; cc < kPW; ++cc) prow[cc] = v[cc];
+            used = true;
+        }
+        __syncthreads();
+        if (!used && p >= 0) {
+            const float2 d = prow[j];
+            if (d.x != 0.f || d.y != 0.f) {
+                const float den = d.x * d.x + d.y * d.y;
+                const float2 l = cmulv(vj, float2{d.x / den, -d.y / den});
 #pragma unroll
-                    for (int c = j + 1; c < kPW; ++c) v[c] = csubv(v[c], cmulv(l, prow[c]));
+                for (int cc = C0; cc < kPW; ++cc) {
+                    const float2 upd = csubv(v[cc], cmulv(l, prow[cc]));
+                    v[cc].x = cc == j ? l.x : (cc > j ? upd.x : v[cc].x);
+                    v[cc].y = cc == j ? l.y : (cc > j ? upd.y : v[cc].y);
                 }
             }
         }
     }
+}
+
+__device__ __forceinline__ void block_gepp(float2 (&v)[kPW], bool used, int pw, int *piv, unsigned *s_key,
+                                           float2 *prow)
+{
+    gepp_phase<0>(v, used, pw, piv, s_key, prow);
+    gepp_phase<1>(v, used, pw, piv, s_key, prow);
+    gepp_phase<2>(v, used, pw, piv, s_key, prow);
+    gepp_phase<3>(v, used, pw, piv, s_key, prow);
     __syncthreads();
 }
 
-// Stage 1: workgroup g selects up to pw candidates among rows
-// [k0 + g*kTsluRows, k0 + (g+1)*kTsluRows).  Set g of the output: candidate rows
-// (cidx, -1 for none) and their original panel rows (cval, kPW x kPW).
-template <typename T>
-__global__ __launch_bounds__(256) void k_tslu_local(const typename CT<T>::t *A, long long lda, int n, int k0, int pw,
-                                                    int *cidx, typename CT<T>::t *cval)
+// x := x U^-1 for one row x (registers) and U upper triangular (LDS), column by
+// column, in the same four phases.
+template <int P>
+__device__ __forceinline__ void right_trsv_phase(float2 (&x)[kPW], int pw, const float2 (*U)[kPW + 1])
 {
-    using V = typename CT<T>::t;
-    __shared__ int piv[kPW], wr[4];
-    __shared__ T wv[4];
-    __shared__ V prow[kPW];
-    const int g = blockIdx.x, tid = threadIdx.x;
-    const int r0 = k0 + g * kTsluRows, nrows = min(kTsluRows, n - r0);
-    V v[kPW];
-    const bool mine = tid < nrows;
+    constexpr int C0 = 8 * P;
+    const int iend = min(C0 + 8, pw);
+#pragma unroll 1
+    for (int i = C0; i < iend; ++i) {
+        float2 xi = x[C0];
 #pragma unroll
-    for (int c = 0; c < kPW; ++c) v[c] = (mine && c < pw) ? A[(long long)(r0 + tid) * lda + k0 + c] : V{0, 0};
-    block_gepp<T>(v, !mine, pw, piv, wv, wr, prow);
-    for (int idx = tid; idx < kPW * kPW; idx += 256) {
-        const int j = idx / kPW, c = idx % kPW;
-        const int p = j < pw ? piv[j] : -1;
-        if (c == 0) cidx[g * kPW + j] = p < 0 ? -1 : r0 + p;
-        cval[((size_t)g * kPW + j) * kPW + c] = (p >= 0 && c < pw) ? A[(long long)(r0 + p) * lda + k0 + c] : V{0, 0};
+        for (int q = 1; q < 8; ++q) {
+            xi.x = (i == C0 + q) ? x[C0 + q].x : xi.x;
+            xi.y = (i == C0 + q) ? x[C0 + q].y : xi.y;
+        }
+        const float2 d = U[i][i];
+        const float den = d.x * d.x + d.y * d.y;
+        xi = den != 0.f ? float2{(xi.x * d.x + xi.y * d.y) / den, (xi.y * d.x - xi.x * d.y) / den} : float2{0, 0};
+#pragma unroll
+        for (int cc = C0; cc < kPW; ++cc) {
+            const float2 upd = csubv(x[cc], cmulv(xi, U[i][cc]));
+            x[cc].x = cc == i ? xi.x : (cc > i ? upd.x : x[cc].x);
+            x[cc].y = cc == i ? xi.y : (cc > i ? upd.y : x[cc].y);
+        }
     }
 }
 
-// Stage 2: workgroup g merges sets [g*kTsluFan, min((g+1)*kTsluFan, nsets)).
-template <typename T>
-__global__ __launch_bounds__(256) void k_tslu_merge(const int *in_idx, const typename CT<T>::t *in_val, int nsets,
-                                                    int pw, int *out_idx, typename CT<T>::t *out_val)
-{
-    using V = typename CT<T>::t;
-    __shared__ int piv[kPW], wr[4];
-    __shared__ T wv[4];
-    __shared__ V prow[kPW];
-    const int g = blockIdx.x, tid = threadIdx.x;
-    const int s0 = g * kTsluFan, nrows = min(kTsluFan, nsets - s0) * kPW;
-    const int *idx_in = in_idx + (size_t)s0 * kPW;
-    const V *val_in = in_val + (size_t)s0 * kPW * kPW;
-    const bool mine = tid < nrows && idx_in[tid] >= 0;
-    V v[kPW];
-#pragma unroll
-    for (int c = 0; c < kPW; ++c) v[c] = mine ? val_in[(size_t)tid * kPW + c] : V{0, 0};
-    block_gepp<T>(v, !mine, pw, piv, wv, wr, prow);
-    for (int idx = tid; idx < kPW * kPW; idx += 256) {
-        const int j = idx / kPW, c = idx % kPW;
-        const int p = j < pw ? piv[j] : -1;
-        if (c == 0) out_idx[g * kPW + j] = p < 0 ? -1 : idx_in[p];
-        out_val[((size_t)g * kPW + j) * kPW + c] = p >= 0 ? val_in[(size_t)p * kPW + c] : V{0, 0};
-    }
-}
+struct TsluFinal {
+    long long *ipiv;   // rows c .. c+pw receive the sequential interchanges
+    float2 *top;       // [kPW][kPW] L11 \ U11, row j = the row that goes to c + j
+    float2 *oldtop;    // [kPW][kPW] panel entries of rows c .. c+pw before the interchanges
+    int *plan;         // [0] = m, then m destinations, then m sources (rows before the moves)
+    int *info;         // first zero pivot (1-based), 0 if none
+};
 
-// Stage 3 (one workgroup): partial pivoting on the winners orders them and
-// gives L11 \ U11 (top, kPW x kPW, row j = the row that goes to k0 + j) and the
-// sequential interchanges ipiv[k0 .. k0+pw) that bring winner j to row k0 + j.
-template <typename T>
-__global__ __launch_bounds__(256) void k_tslu_finish(const int *widx, const typename CT<T>::t *wval, int k0, int pw,
-                                                     long long *ipiv, typename CT<T>::t *top, int *info)
+// One tournament stage.  MODE 0: workgroup g takes rows [c + g kTsluT, ...) of A;
+// MODE 1: workgroup g merges candidate sets [g kTsluFan, ...) of in_idx / in_val.
+// With one workgroup in the grid the stage is the last and fills fin; otherwise
+// workgroup g writes candidate set g (row ids, -1 for none, and original rows).
+template <int MODE>
+__global__ __launch_bounds__(kTsluT) void k_tslu_stage(const float2 *A, long long lda, int n, int c, int pw,
+                                                       const int *in_idx, const float2 *in_val, int nsets_in,
+                                                       int *out_idx, float2 *out_val, TsluFinal fin)
 {
-    using V = typename CT<T>::t;
-    __shared__ int piv[kPW], wr[4], zero_at;
-    __shared__ T wv[4];
-    __shared__ V prow[kPW];
-    __shared__ long long rows[2 * kPW], at[2 * kPW];   // rows moved so far and their positions
-    const int tid = threadIdx.x;
-    const bool mine = tid < kPW && widx[tid] >= 0;
-    V v[kPW];
+    __shared__ unsigned s_key[kTsluT / 64];
+    __shared__ float2 prow[kPW];
+    __shared__ int piv[kPW], s_rowid[kTsluT], zero_at;
+    __shared__ int s_win[kPW], s_at[kPW], s_pos[kPW];
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const bool last = gridDim.x == 1;
+    int rowid = -1;
+    size_t base = 0;   // MODE 1: first stacked row of this workgroup in in_idx / in_val
+    float2 v[kPW];
+    if (MODE == 0) {
+        const long long r = (long long)c + (long long)g * kTsluT + tid;
+        if (r < n) rowid = (int)r;
 #pragma unroll
-    for (int c = 0; c < kPW; ++c) v[c] = mine ? wval[(size_t)tid * kPW + c] : V{0, 0};
+        for (int cc = 0; cc < kPW; ++cc) v[cc] = (rowid >= 0 && cc < pw) ? A[r * lda + c + cc] : float2{0, 0};
+    } else {
+        base = (size_t)g * kTsluFan * kPW;
+        const int nrows = min(kTsluFan, nsets_in - g * kTsluFan) * kPW;
+        if (tid < nrows) rowid = in_idx[base + tid];
+#pragma unroll
+        for (int cc = 0; cc < kPW; ++cc) v[cc] = rowid >= 0 ? in_val[(base + tid) * kPW + cc] : float2{0, 0};
+    }
+    s_rowid[tid] = rowid;
     if (tid == 0) zero_at = INT_MAX;
-    block_gepp<T>(v, !mine, pw, piv, wv, wr, prow);
-    // the thread whose row pivoted column j writes row j of the factored block
+    block_gepp(v, rowid < 0, pw, piv, s_key, prow);
+    if (!last) {
+        for (int idx = tid; idx < kPW * kPW; idx += kTsluT) {
+            const int j = idx / kPW, cc = idx % kPW;
+            const int p = j < pw ? piv[j] : -1;
+            const int rid = p >= 0 ? s_rowid[p] : -1;
+            if (cc == 0) out_idx[g * kPW + j] = rid;
+            float2 o{0, 0};
+            if (rid >= 0 && cc < pw) o = MODE == 0 ? A[(long long)rid * lda + c + cc] : in_val[(base + p) * kPW + cc];
+            out_val[((size_t)g * kPW + j) * kPW + cc] = o;
+        }
+        return;
+    }
+    // last stage: the owners of the pivot rows write L11 \ U11
     int j_of = -1;
     for (int j = 0; j < pw; ++j)
         if (piv[j] == tid) j_of = j;
     if (j_of >= 0) {
 #pragma unroll
-        for (int c = 0; c < kPW; ++c) top[j_of * kPW + c] = c < pw ? v[c] : V{0, 0};
-#pragma unroll
-        for (int c = 0; c < kPW; ++c)
-            if (c == j_of && v[c].x == (T)0 && v[c].y == (T)0) atomicMin(&zero_at, j_of);
+        for (int cc = 0; cc < kPW; ++cc) {
+            fin.top[j_of * kPW + cc] = cc < pw ? v[cc] : float2{0, 0};
+            if (cc == j_of && v[cc].x == 0.f && v[cc].y == 0.f) atomicMin(&zero_at, j_of);
+        }
     }
-    for (int idx = tid; idx < kPW * kPW; idx += 256)
-        if (idx / kPW >= pw) top[idx] = V{0, 0};
+    for (int idx = tid; idx < kPW * kPW; idx += kTsluT) {
+        const int j = idx / kPW, cc = idx % kPW;
+        if (j >= pw) fin.top[idx] = float2{0, 0};
+        const long long r = (long long)c + j;
+        fin.oldtop[idx] = (j < pw && cc < pw) ? A[r * lda + c + cc] : float2{0, 0};
+    }
+    if (tid < kPW) {
+        const int p = tid < pw ? piv[tid] : -1;
+        s_win[tid] = p >= 0 ? s_rowid[p] : -1;
+        s_at[tid] = c + tid;    // row now at position c + tid
+        s_pos[tid] = c + tid;   // position of row c + tid
+    }
     __syncthreads();
     if (tid == 0) {
-        int nt = 0, first_bad = zero_at;
+        // sequential interchanges bringing winner j to position c + j: rows of the
+        // panel's own positions are tracked in s_at / s_pos; a winner from further
+        // down has not moved before its own step
+        int first_bad = zero_at;
         for (int j = 0; j < pw; ++j) {
-            const long long here = k0 + j;
-            const int w = piv[j];
-            if (w < 0) {   // fewer candidate rows than columns: cannot happen for pw <= n - k0
-                ipiv[here] = here;
+            const int here = c + j, w = s_win[j];
+            if (w < 0) {   // fewer candidates than columns: cannot happen for pw <= n - c
+                fin.ipiv[here] = here;
                 first_bad = min(first_bad, j);
                 continue;
             }
-            const long long target = widx[w];
-            long long p = target, occupant = here;
-            int it = -1, io = -1;
-            for (int q = 0; q < nt; ++q) {
-                if (rows[q] == target) { p = at[q]; it = q; }
-                if (at[q] == here) { occupant = rows[q]; io = q; }
-            }
-            ipiv[here] = p;
+            const int p = (w < c + pw) ? s_pos[w - c] : w;
+            fin.ipiv[here] = p;
             if (p != here) {
-                if (it < 0) { it = nt; rows[nt] = target; ++nt; }
-                if (io < 0) { io = nt; rows[nt] = occupant; ++nt; }
-                at[it] = here;
-                at[io] = p;
+                const int occupant = s_at[j];   // a row of the panel's own positions
+                s_at[j] = w;
+                if (p < c + pw) s_at[p - c] = occupant;
+                if (w < c + pw) s_pos[w - c] = here;
+                s_pos[occupant - c] = p;
             }
         }
-        if (first_bad != INT_MAX && *info == 0) *info = k0 + first_bad + 1;
+        // moves: positions c + j receive the winners; a panel row that ended
+        // below the panel's positions moves there
+        int m = 0;
+        for (int j = 0; j < pw; ++j) {
+            const int w = s_win[j] >= 0 ? s_win[j] : c + j;
+            if (w != c + j) {
+                fin.plan[1 + m] = c + j;
+                fin.plan[1 + kPlanMax + m] = w;
+                ++m;
+            }
+        }
+        for (int t = 0; t < pw; ++t) {
+            if (s_pos[t] >= c + pw) {
+                fin.plan[1 + m] = s_pos[t];
+                fin.plan[1 + kPlanMax + m] = c + t;
+                ++m;
+            }
+        }
+        fin.plan[0] = m;
+        if (first_bad != INT_MAX && *fin.info == 0) *fin.info = c + first_bad + 1;
     }
 }
 
-// Stage 4, after the interchanges: rows k0 .. k0+pw of the panel become
-// L11 \ U11, the rows below L21 = A21 U11^-1 (one thread per row).
-template <typename T>
-__global__ __launch_bounds__(256) void k_tslu_apply(typename CT<T>::t *A, long long lda, int n, int k0, int pw,
-                                                    const typename CT<T>::t *top)
+// After a tournament: workgroups [0, nsw) move the planned rows in the columns
+// [a0, a1) and [b0, b1) (16 columns each, all sources read before any
+// destination is written); the others finish the panel rows r >= c: rows
+// c .. c+pw become L11 \ U11, every other row x (its pre-move panel entries,
+// from oldtop when it was a panel row) becomes L21 = x U11^-1.
+__global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, int n, int c, int pw, const float2 *top,
+                                                    const float2 *oldtop, const int *plan, int a0, int a1, int b0,
+                                                    int b1, int nsw)
 {
-    using V = typename CT<T>::t;
-    __shared__ V U[kPW][kPW + 1];
-    for (int idx = threadIdx.x; idx < kPW * kPW; idx += blockDim.x) U[idx / kPW][idx % kPW] = top[idx];
+    __shared__ int s_dst[kPlanMax], s_src[kPlanMax], s_m;
+    __shared__ float2 s_buf[kPlanMax][16];
+    __shared__ float2 U[kPW][kPW + 1];
+    const int tid = threadIdx.x;
+    if (tid == 0) s_m = plan[0];
+    if (tid < kPlanMax) {
+        s_dst[tid] = plan[1 + tid];
+        s_src[tid] = plan[1 + kPlanMax + tid];
+    }
     __syncthreads();
-    const long long r = (long long)k0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    V *row = A + r * lda + k0;
-    if (r < k0 + pw) {
-        const int j = (int)(r - k0);
-        for (int c = 0; c < pw; ++c) row[c] = U[j][c];
+    const int m = s_m;
+    if ((int)blockIdx.x < nsw) {
+        const int cl = tid & 15, rl = tid >> 4;
+        const int t = blockIdx.x * 16 + cl, w1 = a1 - a0;
+        int col = -1;
+        if (t < w1) col = a0 + t;
+        else if (t < w1 + (b1 - b0)) col = b0 + (t - w1);
+        for (int s = rl; s < m; s += 16)
+            if (col >= 0) s_buf[s][cl] = A[(long long)s_src[s] * lda + col];
+        __syncthreads();
+        for (int s = rl; s < m; s += 16)
+            if (col >= 0) A[(long long)s_dst[s] * lda + col] = s_buf[s][cl];
         return;
     }
-    V x[kPW];
-#pragma unroll
-    for (int c = 0; c < kPW; ++c) x[c] = c < pw ? row[c] : V{0, 0};
-#pragma unroll
-    for (int c = 0; c < kPW; ++c) {
-        V s = x[c];
-#pragma unroll
-        for (int i = 0; i < c; ++i) s = csubv(s, cmulv(x[i], U[i][c]));
-        const V d = U[c][c];
-        const T den = d.x * d.x + d.y * d.y;
-        x[c] = (c < pw && den != (T)0) ? V{(s.x * d.x + s.y * d.y) / den, (s.y * d.x - s.x * d.y) / den} : V{0, 0};
+    for (int idx = tid; idx < kPW * kPW; idx += 256) U[idx / kPW][idx % kPW] = top[idx];
+    __syncthreads();
+    const long long r = (long long)c + (long long)(blockIdx.x - nsw) * 256 + tid;
+    if (r >= n) return;
+    float2 *row = A + r * lda + c;
+    if (r < c + pw) {
+        const int j = (int)(r - c);
+        for (int cc = 0; cc < pw; ++cc) row[cc] = U[j][cc];
+        return;
     }
+    int from = -1;   // panel row whose entries move here
+    for (int s = 0; s < m; ++s)
+        if (s_dst[s] == r) from = s_src[s] - c;
+    const float2 *srow = from >= 0 ? oldtop + (size_t)from * kPW : row;
+    float2 x[kPW];
 #pragma unroll
-    for (int c = 0; c < kPW; ++c)
-        if (c < pw) row[c] = x[c];
+    for (int cc = 0; cc < kPW; ++cc) x[cc] = cc < pw ? srow[cc] : float2{0, 0};
+    right_trsv_phase<0>(x, pw, U);
+    right_trsv_phase<1>(x, pw, U);
+    right_trsv_phase<2>(x, pw, U);
+    right_trsv_phase<3>(x, pw, U);
+#pragma unroll
+    for (int cc = 0; cc < kPW; ++cc)
+        if (cc < pw) row[cc] = x[cc];
 }
 
 // ---------------------------------------------------------------- row interchanges
@@ -1050,8 +1149,9 @@ __global__ __launch_bounds__(kIB) void k_trinv(const float2 *A, long long lda, i
 // c < cs (xin, out: LDS, row-major x nrhs; they may alias).  256 threads: thread t
 // takes row t / 2 and half t % 2 of the columns, whose 64 entries it loads into
 // registers at once (one memory round trip per block instead of one per entry).
-template <bool SUB>
-__device__ void blk_matvec(const float2 *Mx, long long ld, int rs, int cs, const float2 *xin, float2 *out, int nrhs)
+template <bool SUB, int NRHS>
+__device__ __forceinline__ void blk_matvec(const float2 *Mx, long long ld, int rs, int cs, const float2 *xin,
+                                           float2 *out)
 {
     constexpr int H = kIB / 2;
     const int t = threadIdx.x, i = t >> 1, h = t & 1;
@@ -1069,73 +1169,69 @@ __device__ void blk_matvec(const float2 *Mx, long long ld, int rs, int cs, const
 #pragma unroll
         for (int c = 0; c < H; ++c) a[c] = (i < rs && h * H + c < cs) ? row[c] : float2{0, 0};
     }
-    float2 acc[kMaxRhs];
+    float2 acc[NRHS];
 #pragma unroll
-    for (int r = 0; r < kMaxRhs; ++r) acc[r] = {0, 0};
+    for (int r = 0; r < NRHS; ++r) acc[r] = {0, 0};
 #pragma unroll
     for (int c = 0; c < H; ++c) {
 #pragma unroll
-        for (int r = 0; r < kMaxRhs; ++r) {
-            if (r < nrhs) {
-                const float2 x = xin[(h * H + c) * nrhs + r];
-                acc[r].x += a[c].x * x.x - a[c].y * x.y;
-                acc[r].y += a[c].x * x.y + a[c].y * x.x;
-            }
+        for (int r = 0; r < NRHS; ++r) {
+            const float2 x = xin[(h * H + c) * NRHS + r];
+            acc[r].x += a[c].x * x.x - a[c].y * x.y;
+            acc[r].y += a[c].x * x.y + a[c].y * x.x;
         }
     }
 #pragma unroll
-    for (int r = 0; r < kMaxRhs; ++r) {
+    for (int r = 0; r < NRHS; ++r) {
         acc[r].x += __shfl_xor(acc[r].x, 1, 64);
         acc[r].y += __shfl_xor(acc[r].y, 1, 64);
     }
     __syncthreads();
     if (i < rs && h == 0) {
 #pragma unroll
-        for (int r = 0; r < kMaxRhs; ++r) {
-            if (r < nrhs) {
-                float2 v = acc[r];
-                if (SUB) v = {out[i * nrhs + r].x - v.x, out[i * nrhs + r].y - v.y};
-                out[i * nrhs + r] = v;
-            }
+        for (int r = 0; r < NRHS; ++r) {
+            float2 v = acc[r];
+            if (SUB) v = {out[i * NRHS + r].x - v.x, out[i * NRHS + r].y - v.y};
+            out[i * NRHS + r] = v;
         }
     }
     __syncthreads();
 }
 
-template <bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_inv_first(const float2 *Dinv, int n, float2 *X, int nrhs, int blk)
+template <bool LOWER, int NRHS>
+__global__ __launch_bounds__(256) void k_trsv_inv_first(const float2 *Dinv, int n, float2 *X, int blk)
 {
-    __shared__ float2 xs[kIB * kMaxRhs];
+    __shared__ float2 xs[kIB * NRHS];
     const int b0 = blk * kIB, bs = min(kIB, n - b0);
     // rows past the block end read as zero (their matrix entries are zero too, but 0 * garbage is not)
-    for (int t = threadIdx.x; t < kIB * nrhs; t += blockDim.x) xs[t] = t < bs * nrhs ? X[(long long)b0 * nrhs + t] : float2{0, 0};
+    for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x) xs[t] = t < bs * NRHS ? X[(long long)b0 * NRHS + t] : float2{0, 0};
     __syncthreads();
-    blk_matvec<false>(Dinv + (size_t)blk * kIB * kIB, kIB, bs, bs, xs, xs, nrhs);
-    for (int t = threadIdx.x; t < bs * nrhs; t += blockDim.x) X[(long long)b0 * nrhs + t] = xs[t];
+    blk_matvec<false, NRHS>(Dinv + (size_t)blk * kIB * kIB, kIB, bs, bs, xs, xs);
+    for (int t = threadIdx.x; t < bs * NRHS; t += blockDim.x) X[(long long)b0 * NRHS + t] = xs[t];
 }
 
 // Block blk of X is final: every later (LOWER) / earlier block subtracts its
 // product with blk; workgroup 0 owns the adjacent block and finishes it with
 // the inverted diagonal block.
-template <bool LOWER>
+template <bool LOWER, int NRHS>
 __global__ __launch_bounds__(256) void k_trsv_inv_step(const float2 *A, long long lda, int n, const float2 *Dinv,
-                                                       float2 *X, int nrhs, int blk, int nblk)
+                                                       float2 *X, int blk, int nblk)
 {
-    __shared__ float2 xs[kIB * kMaxRhs];
-    __shared__ float2 xb[kIB * kMaxRhs];
+    __shared__ float2 xs[kIB * NRHS];
+    __shared__ float2 xb[kIB * NRHS];
     const int next = LOWER ? blk + 1 : blk - 1;
     const int ch = LOWER ? next + (int)blockIdx.x : next - (int)blockIdx.x;
     if (ch < 0 || ch >= nblk) return;   // uniform per workgroup
     const int c0 = blk * kIB, cs = min(kIB, n - c0);
     const int r0 = ch * kIB, rs = min(kIB, n - r0);
-    for (int t = threadIdx.x; t < kIB * nrhs; t += blockDim.x) {
-        xb[t] = t < cs * nrhs ? X[(long long)c0 * nrhs + t] : float2{0, 0};
-        xs[t] = t < rs * nrhs ? X[(long long)r0 * nrhs + t] : float2{0, 0};
+    for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x) {
+        xb[t] = t < cs * NRHS ? X[(long long)c0 * NRHS + t] : float2{0, 0};
+        xs[t] = t < rs * NRHS ? X[(long long)r0 * NRHS + t] : float2{0, 0};
     }
     __syncthreads();
-    blk_matvec<true>(A + (long long)r0 * lda + c0, lda, rs, cs, xb, xs, nrhs);
-    if (blockIdx.x == 0) blk_matvec<false>(Dinv + (size_t)next * kIB * kIB, kIB, rs, rs, xs, xs, nrhs);
-    for (int t = threadIdx.x; t < rs * nrhs; t += blockDim.x) X[(long long)r0 * nrhs + t] = xs[t];
+    blk_matvec<true, NRHS>(A + (long long)r0 * lda + c0, lda, rs, cs, xb, xs);
+    if (blockIdx.x == 0) blk_matvec<false, NRHS>(Dinv + (size_t)next * kIB * kIB, kIB, rs, rs, xs, xs);
+    for (int t = threadIdx.x; t < rs * NRHS; t += blockDim.x) X[(long long)r0 * NRHS + t] = xs[t];
 }
 
 // r = b - A x in FP64 (A row-major n x n), one wave per row; x, b, r are n x nrhs row-major.
@@ -1283,7 +1379,9 @@ struct LUWork {
     int rpb_max = 0;
     DevBuf<int> plan;                      // row-interchange plan (k_swap_plan)
     DevBuf<int> tidx[2];                   // tournament candidate sets (row ids)
-    DevBuf<typename CT<T>::t> tval[2], top;   // their panel rows; the factored winners
+    DevBuf<float2> tval[2];                // their original panel rows
+    DevBuf<float2> top, oldtop;            // factored winners; panel rows before the moves
+    DevBuf<int> tplan;                     // row moves of the last tournament
 };
 
 template <typename T, int TM, int TN, bool M3, int WPE>
@@ -1362,28 +1460,35 @@ void panel(Ctx &ctx, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, i
     }
 }
 
-// Panel of columns [k0, k0 + pw) by tournament pivoting: selects the pivot rows,
-// writes ipiv[k0 .. k0+pw) and the factored pw x pw block to W.top.  The caller
-// interchanges the rows, then k_tslu_apply finishes the panel.
-template <typename T>
-void tslu_select(Ctx &ctx, LUWork<T> &W, const typename CT<T>::t *A, long long lda, int n, int k0, int pw,
-                 long long *ipiv)
+// Panel of columns [c, c + pw) of the block column [k, k + kb) by tournament
+// pivoting: the stages choose the pivot rows and write ipiv[c .. c+pw); the
+// apply kernel moves the rows in the block's other columns and finishes the panel.
+void tslu_panel(Ctx &ctx, LUWork<float> &W, float2 *A, long long lda, int n, int k, int kb, int c, int pw,
+                long long *ipiv)
 {
     hipStream_t s = ctx.stream;
-    int nsets = (n - k0 + kTsluRows - 1) / kTsluRows;
-    k_tslu_local<T><<<nsets, 256, 0, s>>>(A, lda, n, k0, pw, W.tidx[0].p, W.tval[0].p);
-    check("k_tslu_local");
-    int cur = 0;
-    while (nsets > 1) {
-        const int nout = (nsets + kTsluFan - 1) / kTsluFan;
-        k_tslu_merge<T><<<nout, 256, 0, s>>>(W.tidx[cur].p, W.tval[cur].p, nsets, pw, W.tidx[cur ^ 1].p,
-                                             W.tval[cur ^ 1].p);
-        check("k_tslu_merge");
-        cur ^= 1;
-        nsets = nout;
+    const TsluFinal fin{ipiv, W.top.p, W.oldtop.p, W.tplan.p, W.info.p};
+    {
+        ProfScope prof(ctx, "k_tslu_stage", (double)(n - c) * pw * sizeof(float2));
+        int nsets = (n - c + kTsluT - 1) / kTsluT;
+        k_tslu_stage<0><<<nsets, kTsluT, 0, s>>>(A, lda, n, c, pw, nullptr, nullptr, 0, W.tidx[0].p, W.tval[0].p, fin);
+        check("k_tslu_stage");
+        int cur = 0;
+        while (nsets > 1) {
+            const int nout = (nsets + kTsluFan - 1) / kTsluFan;
+            k_tslu_stage<1><<<nout, kTsluT, 0, s>>>(A, lda, n, c, pw, W.tidx[cur].p, W.tval[cur].p, nsets,
+                                                    W.tidx[cur ^ 1].p, W.tval[cur ^ 1].p, fin);
+            check("k_tslu_stage");
+            cur ^= 1;
+            nsets = nout;
+        }
     }
-    k_tslu_finish<T><<<1, 256, 0, s>>>(W.tidx[cur].p, W.tval[cur].p, k0, pw, ipiv, W.top.p, W.info.p);
-    check("k_tslu_finish");
+    ProfScope prof(ctx, "k_tslu_apply", (double)(n - c) * pw * sizeof(float2) * 2);
+    const int w = (c - k) + (k + kb - (c + pw));
+    const int nsw = (w + 15) / 16;
+    const int grid = nsw + (n - c + 255) / 256;
+    k_tslu_apply<<<grid, 256, 0, s>>>(A, lda, n, c, pw, W.top.p, W.oldtop.p, W.tplan.p, k, c, c + pw, k + kb, nsw);
+    check("k_tslu_apply");
 }
 
 // Trailing update C -= A B through split-bf16 planes (FP32 path only).
@@ -1440,12 +1545,14 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
     // pivoting; FP64 keeps LAPACK's partial pivoting (the same ipiv as zgetrf).
     const bool tournament = sizeof(T) == 4;
     if (tournament) {
-        const size_t sets = (size_t)(n + kTsluRows - 1) / kTsluRows;
+        const size_t sets = (size_t)(n + kTsluT - 1) / kTsluT;
         for (int q = 0; q < 2; ++q) {
             W.tidx[q].alloc(sets * kPW);
             W.tval[q].alloc(sets * kPW * kPW);
         }
         W.top.alloc((size_t)kPW * kPW);
+        W.oldtop.alloc((size_t)kPW * kPW);
+        W.tplan.alloc(1 + 2 * kPlanMax);
     }
     const int NB = 256;   // block column of the right-looking LU (trailing-update depth)
     for (int k = 0; k < n; k += NB) {
@@ -1453,14 +1560,7 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
         for (int sp = 0; sp < kb; sp += kPW) {
             const int c = k + sp, pw = std::min(kPW, kb - sp);
             if constexpr (sizeof(T) == 4) {
-                {
-                    ProfScope prof(ctx, "k_tslu", (double)(n - c) * pw * sizeof(typename CT<T>::t));
-                    tslu_select<T>(ctx, W, A, lda, n, c, pw, ipiv);
-                }
-                laswp<T>(ctx, A, lda, c, c + pw, ipiv, k, k + kb, 0, 0, W.plan.p);
-                ProfScope prof(ctx, "k_tslu_apply", (double)(n - c) * pw * sizeof(typename CT<T>::t) * 2);
-                k_tslu_apply<T><<<(n - c + 255) / 256, 256, 0, s>>>(A, lda, n, c, pw, W.top.p);
-                check("k_tslu_apply");
+                tslu_panel(ctx, W, A, lda, n, k, kb, c, pw, ipiv);
             } else {
                 panel<T>(ctx, W, A, lda, n, c, pw, ipiv);
                 laswp<T>(ctx, A, lda, c, c + pw, ipiv, k, c, c + pw, k + kb, W.plan.p);
@@ -1521,21 +1621,37 @@ void invert_diagonal_blocks(hipStream_t s, const float2 *LU, long long lda, int 
     check("k_trinv");
 }
 
-void lu_solve_inv(hipStream_t s, const float2 *A, long long lda, int n, const float2 *Dinv, float2 *X, int nrhs)
+template <int NRHS>
+void lu_solve_inv_n(hipStream_t s, const float2 *A, long long lda, int n, const float2 *Dinv, float2 *X)
 {
     const int nb = (n + kIB - 1) / kIB;
     const float2 *DL = Dinv, *DU = Dinv + (size_t)nb * kIB * kIB;
-    k_trsv_inv_first<true><<<1, 256, 0, s>>>(DL, n, X, nrhs, 0);
+    k_trsv_inv_first<true, NRHS><<<1, 256, 0, s>>>(DL, n, X, 0);
     check("k_trsv_inv_first");
     for (int b = 0; b + 1 < nb; ++b) {
-        k_trsv_inv_step<true><<<nb - 1 - b, 256, 0, s>>>(A, lda, n, DL, X, nrhs, b, nb);
+        k_trsv_inv_step<true, NRHS><<<nb - 1 - b, 256, 0, s>>>(A, lda, n, DL, X, b, nb);
         check("k_trsv_inv_step");
     }
-    k_trsv_inv_first<false><<<1, 256, 0, s>>>(DU, n, X, nrhs, nb - 1);
+    k_trsv_inv_first<false, NRHS><<<1, 256, 0, s>>>(DU, n, X, nb - 1);
     check("k_trsv_inv_first");
     for (int b = nb - 1; b > 0; --b) {
-        k_trsv_inv_step<false><<<b, 256, 0, s>>>(A, lda, n, DU, X, nrhs, b, nb);
+        k_trsv_inv_step<false, NRHS><<<b, 256, 0, s>>>(A, lda, n, DU, X, b, nb);
         check("k_trsv_inv_step");
+    }
+}
+
+void lu_solve_inv(hipStream_t s, const float2 *A, long long lda, int n, const float2 *Dinv, float2 *X, int nrhs)
+{
+    switch (nrhs) {
+    case 1: lu_solve_inv_n<1>(s, A, lda, n, Dinv, X); break;
+    case 2: lu_solve_inv_n<2>(s, A, lda, n, Dinv, X); break;
+    case 3: lu_solve_inv_n<3>(s, A, lda, n, Dinv, X); break;
+    case 4: lu_solve_inv_n<4>(s, A, lda, n, Dinv, X); break;
+    case 5: lu_solve_inv_n<5>(s, A, lda, n, Dinv, X); break;
+    case 6: lu_solve_inv_n<6>(s, A, lda, n, Dinv, X); break;
+    case 7: lu_solve_inv_n<7>(s, A, lda, n, Dinv, X); break;
+    case 8: lu_solve_inv_n<8>(s, A, lda, n, Dinv, X); break;
+    default: BH_FAIL(BH_ERR_INVALID, "nrhs %d out of range", nrhs);
     }
 }
 
