@@ -145,14 +145,17 @@ struct ProfScope {
     const char *name;
     double work;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    ProfScope(Ctx &c, const char *n, double w) : ctx(c), name(n), work(w) {
+    hipStream_t stream;
+    // brackets work on stream st (default: the context stream)
+    ProfScope(Ctx &c, const char *n, double w, hipStream_t st = nullptr)
+        : ctx(c), name(n), work(w), stream(st ? st : c.stream) {
         if (!ctx.profiling) return;
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return;
-        (void)hipEventRecord(e0, ctx.stream);
+        (void)hipEventRecord(e0, stream);
     }
     ~ProfScope() {
         if (!ctx.profiling || !e0 || !e1) return;
-        (void)hipEventRecord(e1, ctx.stream);
+        (void)hipEventRecord(e1, stream);
         KernelStat &k = ctx.kstats[name];
         k.launches += 1;
         k.work += work;

@@ -1385,14 +1385,14 @@ struct LUWork {
 };
 
 template <typename T, int TM, int TN, bool M3, int WPE>
-void gemm_launch(Ctx &ctx, typename CT<T>::t *A, long long lda, int ar, int ac, int br, int bc, int cr, int cc, int M,
-                 int N, int K)
+void gemm_launch(Ctx &ctx, hipStream_t s, typename CT<T>::t *A, long long lda, int ar, int ac, int br, int bc, int cr,
+                 int cc, int M, int N, int K)
 {
     constexpr int BM = 32 * TM, BN = 32 * TN;
     dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
     BH_REQUIRE(grid.y <= 65535, "gemm grid too large");
-    ProfScope prof(ctx, "k_cgemm", 8.0 * M * N * K);   // real flops of the complex update
-    k_cgemm<T, TM, TN, M3, WPE><<<grid, 256, 0, ctx.stream>>>(A + (long long)ar * lda + ac, lda,
+    ProfScope prof(ctx, "k_cgemm", 8.0 * M * N * K, s);   // real flops of the complex update
+    k_cgemm<T, TM, TN, M3, WPE><<<grid, 256, 0, s>>>(A + (long long)ar * lda + ac, lda,
                                                                A + (long long)br * lda + bc, lda,
                                                                A + (long long)cr * lda + cc, lda, M, N, K);
     check("k_cgemm");
@@ -1401,24 +1401,23 @@ void gemm_launch(Ctx &ctx, typename CT<T>::t *A, long long lda, int ar, int ac, 
 // Tile shapes chosen by measurement on MI355X (N = 12288 LU): FP32 3M products,
 // 128 x 64 complex tiles, two waves per SIMD; FP64 4M, 64 x 64, two waves per SIMD.
 template <typename T>
-void gemm(Ctx &ctx, typename CT<T>::t *A, long long lda, int ar, int ac, int br, int bc, int cr, int cc, int M, int N,
-          int K)
+void gemm(Ctx &ctx, hipStream_t s, typename CT<T>::t *A, long long lda, int ar, int ac, int br, int bc, int cr, int cc,
+          int M, int N, int K)
 {
     if (M <= 0 || N <= 0 || K <= 0) return;
     BH_REQUIRE(K % kBK == 0, "gemm K=%d not a multiple of %d", K, kBK);
-    if constexpr (sizeof(T) == 4) gemm_launch<T, 4, 2, true, 2>(ctx, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
-    else gemm_launch<T, 2, 2, false, 2>(ctx, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
+    if constexpr (sizeof(T) == 4) gemm_launch<T, 4, 2, true, 2>(ctx, s, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
+    else gemm_launch<T, 2, 2, false, 2>(ctx, s, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
 }
 
 template <typename T>
-void laswp(Ctx &ctx, typename CT<T>::t *A, long long lda, int k1, int k2, const long long *ipiv, int c0, int c1,
-           int c2, int c3, int *plan)
+void laswp(Ctx &ctx, hipStream_t s, typename CT<T>::t *A, long long lda, int k1, int k2, const long long *ipiv, int c0,
+           int c1, int c2, int c3, int *plan)
 {
-    hipStream_t s = ctx.stream;
     const int w = std::max(0, c1 - c0) + std::max(0, c3 - c2);
     const int len = k2 - k1;
     if (w <= 0 || len <= 0) return;
-    ProfScope prof(ctx, "k_laswp", 2.0 * 2 * len * w * sizeof(typename CT<T>::t));   // upper bound: every slot moves
+    ProfScope prof(ctx, "k_laswp", 2.0 * 2 * len * w * sizeof(typename CT<T>::t), s);   // upper bound: every slot moves
     BH_REQUIRE(len <= kMaxSwaps, "laswp: %d interchanges exceed %d", len, kMaxSwaps);
     k_swap_plan<<<1, 256, 0, s>>>(ipiv, k1, k2, plan);
     check("k_swap_plan");
@@ -1463,13 +1462,12 @@ void panel(Ctx &ctx, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, i
 // Panel of columns [c, c + pw) of the block column [k, k + kb) by tournament
 // pivoting: the stages choose the pivot rows and write ipiv[c .. c+pw); the
 // apply kernel moves the rows in the block's other columns and finishes the panel.
-void tslu_panel(Ctx &ctx, LUWork<float> &W, float2 *A, long long lda, int n, int k, int kb, int c, int pw,
-                long long *ipiv)
+void tslu_panel(Ctx &ctx, hipStream_t s, LUWork<float> &W, float2 *A, long long lda, int n, int k, int kb, int c,
+                int pw, long long *ipiv)
 {
-    hipStream_t s = ctx.stream;
     const TsluFinal fin{ipiv, W.top.p, W.oldtop.p, W.tplan.p, W.info.p};
     {
-        ProfScope prof(ctx, "k_tslu_stage", (double)(n - c) * pw * sizeof(float2));
+        ProfScope prof(ctx, "k_tslu_stage", (double)(n - c) * pw * sizeof(float2), s);
         int nsets = (n - c + kTsluT - 1) / kTsluT;
         k_tslu_stage<0><<<nsets, kTsluT, 0, s>>>(A, lda, n, c, pw, nullptr, nullptr, 0, W.tidx[0].p, W.tval[0].p, fin);
         check("k_tslu_stage");
@@ -1483,7 +1481,7 @@ void tslu_panel(Ctx &ctx, LUWork<float> &W, float2 *A, long long lda, int n, int
             nsets = nout;
         }
     }
-    ProfScope prof(ctx, "k_tslu_apply", (double)(n - c) * pw * sizeof(float2) * 2);
+    ProfScope prof(ctx, "k_tslu_apply", (double)(n - c) * pw * sizeof(float2) * 2, s);
     const int w = (c - k) + (k + kb - (c + pw));
     const int nsw = (w + 15) / 16;
     const int grid = nsw + (n - c + 255) / 256;
@@ -1492,19 +1490,22 @@ void tslu_panel(Ctx &ctx, LUWork<float> &W, float2 *A, long long lda, int n, int
 }
 
 // Trailing update C -= A B through split-bf16 planes (FP32 path only).
-void gemm_split(Ctx &ctx, float2 *A, long long lda, int ar, int ac, int br, int bc, int cr, int cc, int M, int N, int K,
-                DevBuf<__bf16> &PA, DevBuf<__bf16> &PB)
+// With reuse_a, PA already holds the planes of the same A (same ar, ac, M, K).
+void gemm_split(Ctx &ctx, hipStream_t s, float2 *A, long long lda, int ar, int ac, int br, int bc, int cr, int cc, int M,
+                int N, int K, DevBuf<__bf16> &PA, DevBuf<__bf16> &PB, bool reuse_a = false)
 {
     if (M <= 0 || N <= 0 || K <= 0) return;
     BH_REQUIRE(K % kSplitBK == 0, "split gemm K=%d not a multiple of %d", K, kSplitBK);
-    hipStream_t s = ctx.stream;
+    BH_REQUIRE(!reuse_a || PA.n >= (size_t)4 * M * K, "split gemm: no planes to reuse");
     if (PA.n < (size_t)4 * M * K) PA.alloc((size_t)4 * M * K);
     if (PB.n < (size_t)4 * N * K) PB.alloc((size_t)4 * N * K);
     {
-        ProfScope prof(ctx, "k_split", 8.0 * ((double)M + N) * K * 2.0);
+        ProfScope prof(ctx, "k_split", 8.0 * ((reuse_a ? 0.0 : (double)M) + N) * K * 2.0, s);
         const long long na = (long long)M * K;
-        k_split_rows<<<(unsigned)((na + 255) / 256), 256, 0, s>>>(A + (long long)ar * lda + ac, lda, M, K, PA.p);
-        check("k_split_rows");
+        if (!reuse_a) {
+            k_split_rows<<<(unsigned)((na + 255) / 256), 256, 0, s>>>(A + (long long)ar * lda + ac, lda, M, K, PA.p);
+            check("k_split_rows");
+        }
         dim3 g2((unsigned)((N + 31) / 32), (unsigned)((K + 31) / 32));
         k_split_cols<<<g2, 256, 0, s>>>(A + (long long)br * lda + bc, lda, K, N, PB.p);
         check("k_split_cols");
@@ -1513,11 +1514,49 @@ void gemm_split(Ctx &ctx, float2 *A, long long lda, int ar, int ac, int br, int 
     dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
     BH_REQUIRE(grid.y <= 65535, "gemm grid too large");
     // work: bf16 MFMA flops executed (3 products per real product, 4 real per complex)
-    ProfScope prof(ctx, "k_cgemm_bf16x3", 24.0 * M * N * K);
+    ProfScope prof(ctx, "k_cgemm_bf16x3", 24.0 * M * N * K, s);
     k_cgemm_bf16x3<TM, TN><<<grid, 256, 0, s>>>(PA.p, PB.p, A + (long long)cr * lda + cc, lda, M, N, K);
     check("k_cgemm_bf16x3");
 }
 
+// Block column [k, k + kb): its 32-column panels and the updates inside the block.
+template <typename T>
+void factor_block(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, int k, int kb,
+                  long long *ipiv)
+{
+    for (int sp = 0; sp < kb; sp += kPW) {
+        const int c = k + sp, pw = std::min(kPW, kb - sp);
+        if constexpr (sizeof(T) == 4) {
+            tslu_panel(ctx, s, W, A, lda, n, k, kb, c, pw, ipiv);
+        } else {
+            panel<T>(ctx, W, A, lda, n, c, pw, ipiv);
+            laswp<T>(ctx, s, A, lda, c, c + pw, ipiv, k, c, c + pw, k + kb, W.plan.p);
+        }
+        if (c + pw < k + kb) {
+            trsm_unit<T>(s, A, lda, c, pw, c, c + pw, k + kb - (c + pw));
+            gemm<T>(ctx, s, A, lda, c + pw, c, c, c + pw, c + pw, c + pw, n - (c + pw), k + kb - (c + pw), pw);
+        }
+    }
+}
+
+// U12 = L11^-1 A12 for block row [k, k + kb) over the columns [c0, n).
+template <typename T>
+void block_row_u12(Ctx &ctx, hipStream_t s, typename CT<T>::t *A, long long lda, int n, int k, int kb, int c0)
+{
+    for (int sp = 0; sp < kb; sp += kPW) {
+        const int pw = std::min(kPW, kb - sp);
+        trsm_unit<T>(s, A, lda, k + sp, pw, k + sp, c0, n - c0);
+        if (sp + pw < kb)
+            gemm<T>(ctx, s, A, lda, k + sp + pw, k + sp, k + sp, c0, k + sp + pw, c0, kb - (sp + pw), n - c0, pw);
+    }
+}
+
+// Right-looking blocked LU (block columns of NB = 256, panels of 32).  FP64
+// (LAPACK's partial pivoting) runs on the context stream.  FP32 (tournament
+// pivoting) looks one block ahead: after block k's U12 is formed, the next
+// block column is updated first and factored on a second stream while the rest
+// of the trailing matrix is updated on the context stream; the two touch
+// disjoint columns.
 template <typename T>
 void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *ipiv, int *info_host,
                bool split = false)
@@ -1541,10 +1580,33 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
     W.plan.alloc(4 * kMaxSwaps);
     BH_CHECK_HIP(hipMemsetAsync(W.bar.p, 0, kBarWords * sizeof(unsigned), s));
     BH_CHECK_HIP(hipMemsetAsync(W.info.p, 0, sizeof(int), s));
-    // FP32 factors only precondition GMRES-IR: their panels use tournament
-    // pivoting; FP64 keeps LAPACK's partial pivoting (the same ipiv as zgetrf).
-    const bool tournament = sizeof(T) == 4;
-    if (tournament) {
+    const int NB = 256;   // block column of the right-looking LU (trailing-update depth)
+    int planes_of = -1;   // block whose L21 planes PA holds
+    auto trailing = [&](hipStream_t st, int k, int kb, int c0, int c1) {
+        // A[k+kb:, c0:c1) -= L21 U12[:, c0:c1)
+        if (c1 <= c0 || k + kb >= n) return;
+        if constexpr (sizeof(T) == 4) {
+            if (split) {
+                gemm_split(ctx, st, A, lda, k + kb, k, k, c0, k + kb, c0, n - (k + kb), c1 - c0, kb, PA, PB,
+                           planes_of == k);
+                planes_of = k;
+                return;
+            }
+        }
+        gemm<T>(ctx, st, A, lda, k + kb, k, k, c0, k + kb, c0, n - (k + kb), c1 - c0, kb);
+    };
+    if constexpr (sizeof(T) == 8) {
+        for (int k = 0; k < n; k += NB) {
+            const int kb = std::min(NB, n - k);
+            factor_block<T>(ctx, s, W, A, lda, n, k, kb, ipiv);
+            laswp<T>(ctx, s, A, lda, k, k + kb, ipiv, 0, k, k + kb, n, W.plan.p);
+            if (k + kb < n) {
+                block_row_u12<T>(ctx, s, A, lda, n, k, kb, k + kb);
+                trailing(s, k, kb, k + kb, n);
+            }
+        }
+    } else {
+        // FP32 factors only precondition GMRES-IR: their panels use tournament pivoting
         const size_t sets = (size_t)(n + kTsluT - 1) / kTsluT;
         for (int q = 0; q < 2; ++q) {
             W.tidx[q].alloc(sets * kPW);
@@ -1553,39 +1615,35 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
         W.top.alloc((size_t)kPW * kPW);
         W.oldtop.alloc((size_t)kPW * kPW);
         W.tplan.alloc(1 + 2 * kPlanMax);
-    }
-    const int NB = 256;   // block column of the right-looking LU (trailing-update depth)
-    for (int k = 0; k < n; k += NB) {
-        const int kb = std::min(NB, n - k);
-        for (int sp = 0; sp < kb; sp += kPW) {
-            const int c = k + sp, pw = std::min(kPW, kb - sp);
-            if constexpr (sizeof(T) == 4) {
-                tslu_panel(ctx, W, A, lda, n, k, kb, c, pw, ipiv);
-            } else {
-                panel<T>(ctx, W, A, lda, n, c, pw, ipiv);
-                laswp<T>(ctx, A, lda, c, c + pw, ipiv, k, c, c + pw, k + kb, W.plan.p);
+        hipStream_t s2 = nullptr;
+        hipEvent_t ready = nullptr, done = nullptr;
+        BH_CHECK_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        struct Cleanup {
+            hipStream_t &s2;
+            hipEvent_t &a, &b;
+            ~Cleanup() {
+                if (s2) { (void)hipStreamSynchronize(s2); (void)hipStreamDestroy(s2); }
+                if (a) (void)hipEventDestroy(a);
+                if (b) (void)hipEventDestroy(b);
             }
-            if (c + pw < k + kb) {
-                trsm_unit<T>(s, A, lda, c, pw, c, c + pw, k + kb - (c + pw));
-                gemm<T>(ctx, A, lda, c + pw, c, c, c + pw, c + pw, c + pw, n - (c + pw), k + kb - (c + pw), pw);
-            }
-        }
-        laswp<T>(ctx, A, lda, k, k + kb, ipiv, 0, k, k + kb, n, W.plan.p);
-        if (k + kb < n) {
-            for (int sp = 0; sp < kb; sp += kPW) {
-                const int pw = std::min(kPW, kb - sp);
-                trsm_unit<T>(s, A, lda, k + sp, pw, k + sp, k + kb, n - (k + kb));
-                if (sp + pw < kb)
-                    gemm<T>(ctx, A, lda, k + sp + pw, k + sp, k + sp, k + kb, k + sp + pw, k + kb, kb - (sp + pw),
-                            n - (k + kb), pw);
-            }
-            if constexpr (sizeof(T) == 4) {
-                if (split) {
-                    gemm_split(ctx, A, lda, k + kb, k, k, k + kb, k + kb, k + kb, n - (k + kb), n - (k + kb), kb, PA, PB);
-                    continue;
-                }
-            }
-            gemm<T>(ctx, A, lda, k + kb, k, k, k + kb, k + kb, k + kb, n - (k + kb), n - (k + kb), kb);
+        } cleanup{s2, ready, done};
+        BH_CHECK_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        BH_CHECK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        factor_block<T>(ctx, s, W, A, lda, n, 0, std::min(NB, n), ipiv);
+        for (int k = 0; k < n; k += NB) {
+            const int kb = std::min(NB, n - k);
+            // block k is factored; its interchanges reach the other columns
+            laswp<T>(ctx, s, A, lda, k, k + kb, ipiv, 0, k, k + kb, n, W.plan.p);
+            if (k + kb >= n) break;
+            block_row_u12<T>(ctx, s, A, lda, n, k, kb, k + kb);
+            const int nk = k + kb, nkb = std::min(NB, n - nk);
+            trailing(s, k, kb, nk, nk + nkb);   // the next block column first
+            BH_CHECK_HIP(hipEventRecord(ready, s));
+            BH_CHECK_HIP(hipStreamWaitEvent(s2, ready, 0));
+            factor_block<T>(ctx, s2, W, A, lda, n, nk, nkb, ipiv);
+            BH_CHECK_HIP(hipEventRecord(done, s2));
+            trailing(s, k, kb, nk + nkb, n);    // the rest, concurrently
+            BH_CHECK_HIP(hipStreamWaitEvent(s, done, 0));
         }
     }
     BH_CHECK_HIP(hipMemcpyAsync(info_host, W.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
