@@ -671,6 +671,26 @@ constexpr int kBK = 16;
 // Ci += ArBi + AiBr); M3 = true: three (T1 = ArBr, T2 = AiBi,
 // T3 = (Ar+Ai)(Br+Bi), Cr = T1 - T2, Ci = T3 - T1 - T2), normwise stable and used
 // for the FP32 factorisation that GMRES-IR refines.
+// Tile of a GEMM launched as a 1-D grid of nbm x nbn tiles.  Workgroups are
+// dispatched round-robin over the 8 XCDs (workgroup id mod 8), each with its
+// own L2: the id is first remapped so every XCD owns a contiguous range of
+// tiles, which is then walked in groups of kGroupM tile rows (column-major
+// inside a group), so the workgroups an XCD runs at once share A row panels
+// and B column panels in its L2.
+constexpr int kXcds = 8, kGroupM = 8;
+__device__ __forceinline__ void gemm_tile(int nbm, int nbn, int &tm, int &tn)
+{
+    const int total = nbm * nbn, id = blockIdx.x;
+    const int q = total / kXcds, r = total % kXcds, x = id % kXcds;
+    const int t = x * q + min(x, r) + id / kXcds;
+    const int per_group = kGroupM * nbn;
+    const int first = (t / per_group) * kGroupM;
+    const int gm = min(kGroupM, nbm - first);
+    const int in = t % per_group;
+    tm = first + in % gm;
+    tn = in / gm;
+}
+
 template <typename T, int TM, int TN, bool M3, int WPE>
 __global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__restrict__ Ap, long long lda,
                                                const typename CT<T>::t *__restrict__ Bp, long long ldb,
@@ -688,7 +708,9 @@ __global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__r
     __shared__ V Bs[2][kBK][BN + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
-    const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
+    int tm, tn;
+    gemm_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    const long long m0 = (long long)tm * BM, n0 = (long long)tn * BN;
     const int ar = tid / AROW, ac = (tid % AROW) * APT;
     const int br = tid / BROW, bc = (tid % BROW) * BPT;
     const bool arow_ok = m0 + ar < M;
@@ -767,6 +789,20 @@ __global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__r
     // output row of register r in lane l (column l % 16): the f32 16x16x4 MFMA
     // keeps 4 consecutive rows per lane group (4 (l/16) + r), the f64 one
     // interleaves them (4 r + l/16)
+    // all C reads are issued before any write, so they overlap (one memory
+    // round trip per tile instead of one per element)
+    V cv[TM][TN][4];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ri = sizeof(T) == 4 ? 4 * (lane >> 4) + r : 4 * r + (lane >> 4);
+                const long long gi = m0 + wm * 16 * TM + a * 16 + ri;
+                const long long gj = n0 + wn * 16 * TN + b * 16 + (lane & 15);
+                cv[a][b][r] = (gi < M && gj < N) ? Cp[gi * ldc + gj] : V{0, 0};
+            }
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
@@ -785,7 +821,7 @@ __global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__r
                         re = acc[0][a][b][r];
                         im = acc[1][a][b][r];
                     }
-                    V c = Cp[gi * ldc + gj];
+                    V c = cv[a][b][r];
                     c.x -= re;
                     c.y -= im;
                     Cp[gi * ldc + gj] = c;
@@ -859,7 +895,9 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_bf16x3(const __bf16 *__restric
     __shared__ __align__(16) __bf16 Bs[4][BN][RS];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
-    const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
+    int tm, tn;
+    gemm_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    const long long m0 = (long long)tm * BM, n0 = (long long)tn * BN;
     const long long planeA = (long long)M * K, planeB = (long long)N * K;
     // 16-byte chunks (8 bf16) per chunk of K: A has 4 * BM * 4, B has 4 * BN * 4
     constexpr int ACH = 4 * BM * (BK / 8), BCH = 4 * BN * (BK / 8);
@@ -947,6 +985,17 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_bf16x3(const __bf16 *__restric
         }
         __syncthreads();
     }
+    float2 cv[TM][TN][4];   // all C reads before any write
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long long gi = m0 + wm * 16 * TM + a * 16 + 4 * (lane >> 4) + r;
+                const long long gj = n0 + wn * 16 * TN + b * 16 + (lane & 15);
+                cv[a][b][r] = (gi < M && gj < N) ? C[gi * ldc + gj] : float2{0, 0};
+            }
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
@@ -956,7 +1005,7 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_bf16x3(const __bf16 *__restric
                 const long long gi = m0 + wm * 16 * TM + a * 16 + 4 * (lane >> 4) + r;
                 const long long gj = n0 + wn * 16 * TN + b * 16 + (lane & 15);
                 if (gi < M && gj < N) {
-                    float2 c = C[gi * ldc + gj];
+                    float2 c = cv[a][b][r];
                     c.x -= arr[a][b][r] - aii[a][b][r];
                     c.y -= ari[a][b][r];
                     C[gi * ldc + gj] = c;
@@ -1389,8 +1438,9 @@ void gemm_launch(Ctx &ctx, hipStream_t s, typename CT<T>::t *A, long long lda, i
                  int cc, int M, int N, int K)
 {
     constexpr int BM = 32 * TM, BN = 32 * TN;
-    dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
-    BH_REQUIRE(grid.y <= 65535, "gemm grid too large");
+    const long long tiles = (long long)((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+    BH_REQUIRE(tiles < (1LL << 31), "gemm grid too large");
+    const dim3 grid((unsigned)tiles);
     ProfScope prof(ctx, "k_cgemm", 8.0 * M * N * K, s);   // real flops of the complex update
     k_cgemm<T, TM, TN, M3, WPE><<<grid, 256, 0, s>>>(A + (long long)ar * lda + ac, lda,
                                                                A + (long long)br * lda + bc, lda,
@@ -1511,8 +1561,9 @@ void gemm_split(Ctx &ctx, hipStream_t s, float2 *A, long long lda, int ar, int a
         check("k_split_cols");
     }
     constexpr int TM = 2, TN = 4, BM = 32 * TM, BN = 32 * TN;
-    dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
-    BH_REQUIRE(grid.y <= 65535, "gemm grid too large");
+    const long long tiles = (long long)((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+    BH_REQUIRE(tiles < (1LL << 31), "gemm grid too large");
+    const dim3 grid((unsigned)tiles);
     // work: bf16 MFMA flops executed (3 products per real product, 4 real per complex)
     ProfScope prof(ctx, "k_cgemm_bf16x3", 24.0 * M * N * K, s);
     k_cgemm_bf16x3<TM, TN><<<grid, 256, 0, s>>>(PA.p, PB.p, A + (long long)cr * lda + cc, lda, M, N, K);
