@@ -605,14 +605,14 @@ __global__ __launch_bounds__(256) void k_laswp_apply(typename CT<T>::t *A, long 
 
 // ---------------------------------------------------------------- small triangular solve
 // B (m rows x ncols, m <= 32) := L^{-1} B with L unit lower (m x m), both inside A.
-// One wave per workgroup: lane = column, the column vector lives in LDS.
+// One wave per workgroup, lane = column: the column lives in registers (all
+// its loads in flight at once), L in LDS (every read a broadcast).
 template <typename T>
 __global__ __launch_bounds__(64) void k_trsm_unit(typename CT<T>::t *A, long long lda, int lr, int lc, int m, int br,
                                                   int bc, int ncols)
 {
     using V = typename CT<T>::t;
     __shared__ V Ls[kPW][kPW + 1];
-    __shared__ V xs[kPW][64];
     for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
         const int i = idx / m, j = idx % m;
         Ls[i][j] = A[(long long)(lr + i) * lda + lc + j];
@@ -620,15 +620,24 @@ __global__ __launch_bounds__(64) void k_trsm_unit(typename CT<T>::t *A, long lon
     const int lane = threadIdx.x;
     const int col = blockIdx.x * 64 + lane;
     const bool act = col < ncols;
-    for (int i = 0; i < m; ++i) xs[i][lane] = act ? A[(long long)(br + i) * lda + bc + col] : V{0, 0};
+    V x[kPW];
+#pragma unroll
+    for (int i = 0; i < kPW; ++i) x[i] = (act && i < m) ? A[(long long)(br + i) * lda + bc + col] : V{0, 0};
     __syncthreads();
-    for (int i = 1; i < m; ++i) {
-        V s = xs[i][lane];
-        for (int j = 0; j < i; ++j) s = csubv(s, cmulv(Ls[i][j], xs[j][lane]));
-        xs[i][lane] = s;
+#pragma unroll
+    for (int i = 1; i < kPW; ++i) {
+        if (i < m) {   // uniform
+            V s = x[i];
+#pragma unroll
+            for (int j = 0; j < i; ++j) s = csubv(s, cmulv(Ls[i][j], x[j]));
+            x[i] = s;
+        }
     }
-    if (act)
-        for (int i = 0; i < m; ++i) A[(long long)(br + i) * lda + bc + col] = xs[i][lane];
+    if (act) {
+#pragma unroll
+        for (int i = 0; i < kPW; ++i)
+            if (i < m) A[(long long)(br + i) * lda + bc + col] = x[i];
+    }
 }
 
 // ---------------------------------------------------------------- MFMA complex GEMM
