@@ -724,7 +724,7 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
 
 constexpr int kTile = 64;          // dofs per tile side
 constexpr int kTileCells = 128;    // max cells per tile side
-constexpr int kPairChunk = 256;    // pair blocks staged in LDS at once
+constexpr int kPairChunk = 512;    // pair blocks staged in LDS at once (two per thread)
 
 struct TileSide {
     const int *ptr;       // [ntiles+1] into cells
@@ -777,9 +777,10 @@ __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
     for (int cb = 0; cb < nc; cb += CC) {
         const int cn = min(CC, nc - cb);
         // phase 1: pair blocks of (all row cells) x (column cells cb .. cb+cn)
-        if (tid < nr * cn) {
-            const int rc = tid % nr, cc = tid / nr;
-            far_block<R, OPK, QO, QI>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + tid, kPairChunk);
+#pragma unroll 1
+        for (int t = tid; t < nr * cn; t += 256) {
+            const int rc = t % nr, cc = t / nr;
+            far_block<R, OPK, QO, QI>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kPairChunk);
         }
         __syncthreads();
         // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184
