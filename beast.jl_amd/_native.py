@@ -72,6 +72,7 @@ _SIGS = [
     ("bh_ctx_synchronize", C.c_int, [_P]),
     ("bh_ctx_profile", C.c_int, [_P, C.c_int]),
     ("bh_ctx_kernel_stats", C.c_int, [_P, C.c_char_p, _I64, _D, _D]),
+    ("bh_ctx_profile_only", C.c_int, [_P, C.c_char_p]),
     ("bh_mesh_create", C.c_int, [_P, _D, C.c_int64, _I32, C.c_int64, C.POINTER(_P)]),
     ("bh_mesh_destroy", C.c_int, [_P]),
     ("bh_space_create", C.c_int, [_P, C.c_int64, _I32, _I32, _I32, _D, C.POINTER(_P)]),

@@ -30,8 +30,9 @@ class Context:
     def synchronize(self):
         nat.check(self._lib.bh_ctx_synchronize(self.handle))
 
-    def profile(self, enable=True):
-        """Bracket every library kernel launch with HIP events on this context's stream."""
+    def profile(self, enable=True, only=None):
+        """Bracket library kernel launches with HIP events (all of them, or only kernel ``only``)."""
+        nat.check(self._lib.bh_ctx_profile_only(self.handle, (only or "").encode()))
         nat.check(self._lib.bh_ctx_profile(self.handle, int(bool(enable))))
 
     def kernel_stats(self, name):

@@ -61,6 +61,8 @@ static void ctx_free(Ctx *c)
     if (!c->freed.compare_exchange_strong(expected, true)) return;   // exactly once
     (void)hipSetDevice(c->device);
     resolve_stats(*c);
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    c->event_pool.clear();
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete static_cast<bh_ctx *>(c);
@@ -113,8 +115,8 @@ void resolve_stats(Ctx &ctx)
         for (auto &ev : kv.second.pending) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) kv.second.seconds += ms * 1e-3;
-            (void)hipEventDestroy(ev.first);
-            (void)hipEventDestroy(ev.second);
+            ctx.event_pool.push_back(ev.first);
+            ctx.event_pool.push_back(ev.second);
         }
         kv.second.pending.clear();
     }
@@ -128,6 +130,14 @@ extern "C" int bh_ctx_profile(bh_ctx *ctx, int enable)
     resolve_stats(*ctx);
     if (enable) ctx->kstats.clear();
     ctx->profiling = enable != 0;
+    BH_API_END
+}
+
+extern "C" int bh_ctx_profile_only(bh_ctx *ctx, const char *kernel)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctx != nullptr, "ctx is NULL");
+    ctx->profile_only = kernel ? kernel : "";
     BH_API_END
 }
 

@@ -130,6 +130,8 @@ struct Ctx {
     std::string arch;
     bool profiling = false;
     std::map<std::string, KernelStat> kstats;
+    std::vector<hipEvent_t> event_pool;   // timing events reused across launches
+    std::string profile_only;             // when set, only this kernel name is timed
     // objects (meshes, matrices) created on this context; destruction of the
     // context is deferred until the last of them is gone
     std::atomic<int> refs{0};
@@ -150,8 +152,21 @@ struct ProfScope {
     ProfScope(Ctx &c, const char *n, double w, hipStream_t st = nullptr)
         : ctx(c), name(n), work(w), stream(st ? st : c.stream) {
         if (!ctx.profiling) return;
-        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return;
+        if (!ctx.profile_only.empty() && ctx.profile_only != name) return;
+        e0 = take();
+        e1 = take();
+        if (!e0 || !e1) return;
         (void)hipEventRecord(e0, stream);
+    }
+    hipEvent_t take() {
+        hipEvent_t e = nullptr;
+        if (!ctx.event_pool.empty()) {
+            e = ctx.event_pool.back();
+            ctx.event_pool.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+        }
+        return e;
     }
     ~ProfScope() {
         if (!ctx.profiling || !e0 || !e1) return;

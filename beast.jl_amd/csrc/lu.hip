@@ -536,6 +536,7 @@ __global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, in
 // (slot len + i for the first i naming that row).  plan[s] = the row whose
 // content ends at slot s, plan[2 len + s] = the slot's row (-1: slot unused).
 constexpr int kMaxSwaps = 256;
+constexpr int kNBFP32 = 512;   // FP32 LU block depth
 
 __global__ __launch_bounds__(256) void k_swap_plan(const long long *ipiv, int k1, int k2, int *plan)
 {
@@ -1640,7 +1641,9 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
     W.plan.alloc(4 * kMaxSwaps);
     BH_CHECK_HIP(hipMemsetAsync(W.bar.p, 0, kBarWords * sizeof(unsigned), s));
     BH_CHECK_HIP(hipMemsetAsync(W.info.p, 0, sizeof(int), s));
-    const int NB = 256;   // block column of the right-looking LU (trailing-update depth)
+    // block column of the right-looking LU (trailing-update depth); the FP32
+    // path's deeper blocks halve the C traffic per flop of the trailing update
+    const int NB = sizeof(T) == 4 ? kNBFP32 : 256;
     int planes_of = -1;   // block whose L21 planes PA holds
     auto trailing = [&](hipStream_t st, int k, int kb, int c0, int c1) {
         // A[k+kb:, c0:c1) -= L21 U12[:, c0:c1)
@@ -1692,8 +1695,10 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
         factor_block<T>(ctx, s, W, A, lda, n, 0, std::min(NB, n), ipiv);
         for (int k = 0; k < n; k += NB) {
             const int kb = std::min(NB, n - k);
-            // block k is factored; its interchanges reach the other columns
-            laswp<T>(ctx, s, A, lda, k, k + kb, ipiv, 0, k, k + kb, n, W.plan.p);
+            // block k is factored; its interchanges reach the other columns (in
+            // pieces of at most kMaxSwaps interchanges, applied in order)
+            for (int q = k; q < k + kb; q += kMaxSwaps)
+                laswp<T>(ctx, s, A, lda, q, std::min(q + kMaxSwaps, k + kb), ipiv, 0, k, k + kb, n, W.plan.p);
             if (k + kb >= n) break;
             block_row_u12<T>(ctx, s, A, lda, n, k, kb, k + kb);
             const int nk = k + kb, nkb = std::min(NB, n - nk);

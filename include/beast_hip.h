@@ -95,6 +95,10 @@ int bh_ctx_synchronize(bh_ctx *ctx);
  * bytes, see DESIGN) the launches were given.  Enabling clears the counters. */
 int bh_ctx_profile(bh_ctx *ctx, int enable);
 int bh_ctx_kernel_stats(bh_ctx *ctx, const char *kernel, int64_t *launches, double *seconds, double *work);
+/* Measurement (no counterpart in the reference): restrict profiling to the
+ * launches of one kernel name (NULL or "" restores all), so that timing a
+ * dominant kernel adds no events around the others. */
+int bh_ctx_profile_only(bh_ctx *ctx, const char *kernel);
 
 /* Triangle mesh: vertices[3*nv] (x,y,z per vertex), cells[3*nf] 0-based vertex
  * indices in chart order (CompScienceMeshes chart: r(u,v) = u v1 + v v2 + (1-u-v) v3). */
