@@ -174,6 +174,13 @@ def main():
     else:
         peak = PEAK_FP32_MFMA
         work_note = "8 M N K real flops per launch (complex update, counted as 4 real products)"
+    # HBM bytes per launch of the dominant kernel, from the committed counter passes
+    # (FETCH_SIZE / WRITE_SIZE scaled by a copy kernel) for this workload only
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{dom}_traffic_{args.config.lower()}.json")
+    if os.path.exists(tpath) and args.lu == "fp32split":
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("traffic_bytes_per_launch")
     out = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * value, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
@@ -187,7 +194,8 @@ def main():
         "phases": {key: float(np.median([p[key] for p in phases])) for key in phases[0]},
         "roofline": {"kernel": dom + " (LU trailing updates)", "bound": "mfma",
                      "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None, "work": work_note,
+                     "frac": achieved / peak, "traffic": traffic, "traffic_unit": "bytes per launch",
+                     "work": work_note,
                      "launches": g_launch // max(1, args.steps), "avg_launch_ms": 1e3 * g_sec / max(1, g_launch)},
         "kernels": dict(breakdown, k_far_tiles={"launches": f_launch, "ms": round(1e3 * f_sec, 3),
                                                 "gflops_ref_formulation": f_work / max(f_sec, 1e-30) / 1e9}),
