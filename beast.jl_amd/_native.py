@@ -20,8 +20,8 @@ BH_ERR_UNSUPPORTED = -5
 BH_ERR_SINGULAR = -6
 BH_ERR_NOCONV = -7
 
-OP_SINGLELAYER, OP_DOUBLELAYER = 0, 1
-QS_WILTON_SAUTER, QS_SAUTER, QS_DOUBLENUM = 0, 1, 2
+OP_SINGLELAYER, OP_DOUBLELAYER, OP_NCROSS = 0, 1, 2
+QS_WILTON_SAUTER, QS_SAUTER, QS_DOUBLENUM, QS_SINGLENUM = 0, 1, 2, 3
 FP64, FP32, FP32_SPLIT = 0, 1, 2
 
 
@@ -86,6 +86,7 @@ _SIGS = [
                                   _I32, _I32, _D]),
     ("bh_assemble_planewave", C.c_int, [_P, _P, _D, _D, _D, _D, _D, C.c_int, C.c_int, _D]),
     ("bh_matrix_assemble", C.c_int, [_P, C.POINTER(bh_operator), C.POINTER(bh_quadstrat), _P, C.POINTER(_P)]),
+    ("bh_matrix_accumulate", C.c_int, [_P, C.POINTER(bh_operator), C.POINTER(bh_quadstrat), _P, _D]),
     ("bh_matrix_from_host", C.c_int, [_P, C.c_int64, _D, C.c_int64, C.POINTER(_P)]),
     ("bh_matrix_destroy", C.c_int, [_P]),
     ("bh_matrix_size", C.c_int, [_P, _I64]),

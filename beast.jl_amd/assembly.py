@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _native as nat
 from .operators import (DoubleNumQStrat, DoubleNumSauterQstrat, DoubleNumWiltonSauterQStrat,
-                        LinearCombinationOfOperators, MWDoubleLayer3D, MWSingleLayer3D, PlaneWaveMW,
+                        LinearCombinationOfOperators, MWDoubleLayer3D, MWSingleLayer3D, NCross, PlaneWaveMW,
                         SingleNumQStrat, TangTraceMW, defaultquadstrat, resolve_quadstrat)
 from .rt import RTBasis
 
@@ -116,6 +116,9 @@ def op_struct(op):
         s.kind = nat.OP_DOUBLELAYER
         s.alpha[:] = [op.alpha.real, op.alpha.imag]
         s.beta[:] = [0.0, 0.0]
+    elif isinstance(op, NCross):
+        s.kind = nat.OP_NCROSS
+        return s
     else:
         raise nat.UnsupportedOnGPU(nat.BH_ERR_UNSUPPORTED, f"operator {type(op).__name__} has no GPU path")
     g = complex(op.gamma)
@@ -125,7 +128,7 @@ def op_struct(op):
 
 def qs_struct(qs):
     s = nat.bh_quadstrat()
-    if isinstance(qs, (DoubleNumWiltonSauterQStrat, DoubleNumSauterQstrat, DoubleNumQStrat)):
+    if isinstance(qs, (DoubleNumWiltonSauterQStrat, DoubleNumSauterQstrat, DoubleNumQStrat, SingleNumQStrat)):
         s.kind = qs.kind
         for i, v in enumerate(qs.params):
             s.params[i] = int(v)
@@ -251,14 +254,31 @@ class DeviceMatrix:
 
     @classmethod
     def assemble(cls, op, X, quadstrat=defaultquadstrat, ctx=None):
+        """Resident matrix of ``op`` on X x X; a linear combination is the first term
+        scaled, then the others accumulated in place (operator.jl:294-306)."""
         ctx = ctx or default_context()
-        if isinstance(op, LinearCombinationOfOperators):
-            raise nat.UnsupportedOnGPU(nat.BH_ERR_UNSUPPORTED, "resident assembly takes a single operator")
-        qs = resolve_quadstrat(quadstrat, op, X, X)
+        lib = nat.load()
+        xh = space_handle(X, ctx)
         h = C.c_void_p()
-        nat.check(nat.load().bh_matrix_assemble(ctx.handle, C.byref(op_struct(op)), C.byref(qs_struct(qs)),
-                                                space_handle(X, ctx), C.byref(h)))
-        return cls(h, X.numfunctions(), ctx)
+        terms = list(_terms(op))
+        # a term with coefficient 1 first (assembled in place, the others accumulated)
+        ones = [i for i, (c, _) in enumerate(terms) if complex(c) == 1.0]
+        if ones:
+            terms.insert(0, terms.pop(ones[0]))
+        c0, t0 = terms[0]
+        qs = resolve_quadstrat(quadstrat, t0, X, X)
+        nat.check(lib.bh_matrix_assemble(ctx.handle, C.byref(op_struct(t0)), C.byref(qs_struct(qs)), xh, C.byref(h)))
+        M = cls(h, X.numfunctions(), ctx)
+        if complex(c0) != 1.0:
+            # the first term assembled unscaled: rescale by accumulating (c0 - 1) times it
+            d = complex(c0) - 1.0
+            sc = (C.c_double * 2)(d.real, d.imag)
+            nat.check(lib.bh_matrix_accumulate(M.h, C.byref(op_struct(t0)), C.byref(qs_struct(qs)), xh, sc))
+        for coef, term in terms[1:]:
+            qs = resolve_quadstrat(quadstrat, term, X, X)
+            sc = (C.c_double * 2)(complex(coef).real, complex(coef).imag)
+            nat.check(lib.bh_matrix_accumulate(M.h, C.byref(op_struct(term)), C.byref(qs_struct(qs)), xh, sc))
+        return M
 
     @classmethod
     def from_host(cls, Z, ctx=None):

@@ -253,6 +253,12 @@ extern "C" int bh_space_create(bh_mesh *mesh, int64_t ndofs, const int32_t *shap
     hipStream_t st = mesh->ctx->stream;
     s->d_e2d.upload(s->e2d.data(), s->e2d.size(), st);
     s->d_e2c.upload(s->e2c.data(), s->e2c.size(), st);
+    s->d_ptr.upload(s->ptr.data(), s->ptr.size(), st);
+    if (!s->cell.empty()) {
+        s->d_cell.upload(s->cell.data(), s->cell.size(), st);
+        s->d_ref.upload(s->ref.data(), s->ref.size(), st);
+        s->d_coeff.upload(s->coeff.data(), s->coeff.size(), st);
+    }
     BH_CHECK_HIP(hipStreamSynchronize(st));
     *out = s.release();
     BH_API_END

@@ -878,7 +878,7 @@ Strategy parse_qs(const bh_quadstrat *qs)
 OpParams parse_op(const bh_operator *op)
 {
     BH_REQUIRE(op != nullptr, "operator is NULL");
-    BH_REQUIRE(op->kind == BH_OP_MW_SINGLELAYER || op->kind == BH_OP_MW_DOUBLELAYER,
+    BH_REQUIRE(op->kind == BH_OP_MW_SINGLELAYER || op->kind == BH_OP_MW_DOUBLELAYER || op->kind == BH_OP_NCROSS,
                "operator kind %d is not supported", op->kind);
     OpParams p{op->kind, op->gamma[0], op->gamma[1], op->alpha[0], op->alpha[1], op->beta[0], op->beta[1]};
     for (double v : {p.gr, p.gi, p.ar, p.ai, p.br, p.bi})
@@ -1128,9 +1128,95 @@ void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long lon
 }  // namespace
 
 // Assemble into a device buffer (column-major or row-major, ld given).
+// NCross on RT x RT over one mesh (identityop.jl:20-22 with the matched-mesh
+// loop of localop.jl:114-146): Z[m, n] += scale a (sum_q w_q f_i(x_q) . (n x f_j(x_q))) b
+// over the cells m and n share, f_i = (x - v_i) / (2A).  One thread per test
+// dof, which alone writes its row, in shape order (deterministic).
+template <typename R, bool ROWMAJOR>
+__global__ __launch_bounds__(256) void k_ncross(const double *__restrict__ geo, const double *__restrict__ qp, int Q,
+                                                int ndofs, const int *__restrict__ ptr, const int *__restrict__ cell,
+                                                const int *__restrict__ ref, const double *__restrict__ coeff,
+                                                const int *__restrict__ e2d, const double *__restrict__ e2c, int kmax,
+                                                cx<R> *Z, long long ld, double sr, double si)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= ndofs) return;
+    for (int k = ptr[m]; k < ptr[m + 1]; ++k) {
+        const int c = cell[k], i = ref[k];
+        const double a = coeff[k];
+        if (a == 0.0) continue;
+        const double *g = geo + (size_t)c * kGeo;
+        const double *nrm = g + kGeoN;
+        const double s2a = 1.0 / (2.0 * g[kGeoA]);
+        double loc[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < Q; ++q) {
+            const double *pq = qp + ((size_t)c * Q + q) * 4;
+            const double x[3] = {pq[0] + g[kGeoC], pq[1] + g[kGeoC + 1], pq[2] + g[kGeoC + 2]};
+            const double fi[3] = {(x[0] - g[3 * i]) * s2a, (x[1] - g[3 * i + 1]) * s2a, (x[2] - g[3 * i + 2]) * s2a};
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double fj[3] = {(x[0] - g[3 * j]) * s2a, (x[1] - g[3 * j + 1]) * s2a,
+                                      (x[2] - g[3 * j + 2]) * s2a};
+                double nf[3];
+                cross3(nf, nrm, fj);
+                loc[j] += pq[3] * dot3(fi, nf);
+            }
+        }
+        for (int j = 0; j < 3; ++j)
+            for (int kb = 0; kb < kmax; ++kb) {
+                const int n = e2d[((size_t)c * 3 + j) * kmax + kb];
+                if (n < 0) continue;
+                const double v = a * loc[j] * e2c[((size_t)c * 3 + j) * kmax + kb];
+                cx<R> &z = Z[ROWMAJOR ? (long long)m * ld + n : (long long)n * ld + m];
+                z.r += (R)(sr * v);
+                z.i += (R)(si * v);
+            }
+    }
+}
+
+void assemble_ncross(Ctx &ctx, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ, long long ld,
+                     bool rowmajor, bool accumulate, double sr, double si)
+{
+    BH_REQUIRE(qs != nullptr, "quadstrat is NULL");
+    if (qs->kind != BH_QS_SINGLENUM)
+        BH_FAIL(BH_ERR_UNSUPPORTED, "NCross needs SingleNumQStrat (quadstrat kind %d given)", qs->kind);
+    if (X.mesh != Y.mesh) BH_FAIL(BH_ERR_UNSUPPORTED, "NCross on different test and trial meshes has no GPU path");
+    const int degree = qs->params[0];
+    BH_REQUIRE(degree >= 1 && degree <= 8, "SingleNumQStrat: triangle rule must be degree 1..8");
+    const int64_t M = X.ndofs, N = Y.ndofs;
+    const size_t es = fp32 ? 8 : 16;
+    if (!accumulate) {
+        if (rowmajor) BH_CHECK_HIP(hipMemset2DAsync(dZ, ld * es, 0, N * es, M, ctx.stream));
+        else BH_CHECK_HIP(hipMemset2DAsync(dZ, ld * es, 0, M * es, N, ctx.stream));
+    }
+    int Q = 0;
+    const double *qp = (const double *)quad_points(*X.mesh, degree, &Q, false, ctx.stream);
+    const unsigned grid = (unsigned)((M + 255) / 256);
+    ProfScope prof(ctx, "k_ncross", (double)M * 2 * Q * 3 * 12.0);
+#define BH_NCROSS(R, RM)                                                                                          \
+    k_ncross<R, RM><<<grid, 256, 0, ctx.stream>>>(X.mesh->d_geo.p, qp, Q, (int)M, X.d_ptr.p, X.d_cell.p, X.d_ref.p, \
+                                                  X.d_coeff.p, Y.d_e2d.p, Y.d_e2c.p, Y.kmax, (cx<R> *)dZ, ld, sr, si)
+    if (fp32) {
+        if (rowmajor) BH_NCROSS(float, true);
+        else BH_NCROSS(float, false);
+    } else {
+        if (rowmajor) BH_NCROSS(double, true);
+        else BH_NCROSS(double, false);
+    }
+#undef BH_NCROSS
+    check_launch("k_ncross");
+    BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
+}
+
 void assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ,
                      long long ld, bool rowmajor, bool accumulate, double sr, double si)
 {
+    BH_REQUIRE(op != nullptr, "operator is NULL");
+    if (op->kind == BH_OP_NCROSS) {
+        if (X.ndofs == 0 || Y.ndofs == 0) return;
+        assemble_ncross(ctx, qs, fp32, X, Y, dZ, ld, rowmajor, accumulate, sr, si);
+        return;
+    }
     Plan P;
     make_plan(ctx, op, qs, *X.mesh, *Y.mesh, fp32, P);
     build_tiles(X, ctx.stream);

@@ -209,6 +209,8 @@ struct Space {
     std::vector<double> e2c;        // [nf][3][kmax]
     DevBuf<int32_t> d_e2d;
     DevBuf<double> d_e2c;
+    DevBuf<int32_t> d_ptr, d_cell, d_ref;   // the shapes in CSR form, on the device
+    DevBuf<double> d_coeff;
     // dof tiles for the far-field gather kernel
     bool tiles_built = false;
     int64_t ntiles = 0;

@@ -62,6 +62,22 @@ extern "C" int bh_matrix_assemble(bh_ctx *ctx, const bh_operator *op, const bh_q
     BH_API_END
 }
 
+extern "C" int bh_matrix_accumulate(bh_matrix *M, const bh_operator *op, const bh_quadstrat *qs, const bh_space *X,
+                                    const double *scale)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(M && op && X, "NULL argument");
+    BH_REQUIRE(X->ndofs == M->n, "space has %lld dofs, matrix is %lld x %lld", (long long)X->ndofs, (long long)M->n,
+               (long long)M->n);
+    BH_REQUIRE(X->mesh->ctx == M->ctx, "space belongs to another context");
+    std::lock_guard<std::mutex> lock(M->mu);
+    bh_space *sp = const_cast<bh_space *>(X);
+    const double sr = scale ? scale[0] : 1.0, si = scale ? scale[1] : 0.0;
+    assemble_device(*M->ctx, op, qs, false, *sp, *sp, M->a64.p, M->n, true, true, sr, si);
+    M->factored = -1;   // earlier factors describe the old matrix
+    BH_API_END
+}
+
 extern "C" int bh_matrix_from_host(bh_ctx *ctx, int64_t n, const double *A, int64_t ld, bh_matrix **out)
 {
     BH_API_BEGIN

@@ -79,6 +79,7 @@ _gamma(op) = BEAST.isstatic(op.gamma) ? 0.0 : op.gamma
 
 bhop(op::BEAST.MWSingleLayer3D) = BHOperator(0, c2(_gamma(op)), c2(op.α), c2(op.β))
 bhop(op::BEAST.MWDoubleLayer3D) = BHOperator(1, c2(_gamma(op)), c2(op.alpha), (0.0, 0.0))
+bhop(op::BEAST.NCross) = BHOperator(2, (0.0, 0.0), (0.0, 0.0), (0.0, 0.0))
 
 pad8(t) = ntuple(i -> i <= length(t) ? Int32(t[i]) : Int32(0), 8)
 bhqs(q::BEAST.DoubleNumWiltonSauterQStrat) = BHQuadStrat(0, pad8((q.outer_rule_far, q.inner_rule_far,
@@ -88,6 +89,7 @@ bhqs(q::BEAST.DoubleNumSauterQstrat) = BHQuadStrat(1, pad8((q.outer_rule, q.inne
     q.sauter_schwab_common_tetr, q.sauter_schwab_common_face, q.sauter_schwab_common_edge,
     q.sauter_schwab_common_vert)))
 bhqs(q::BEAST.DoubleNumQStrat) = BHQuadStrat(2, pad8((q.outer_rule, q.inner_rule)))
+bhqs(q::BEAST.SingleNumQStrat) = BHQuadStrat(3, pad8((q.quad_rule,)))
 bhqs(q) = throw(Unsupported("quadrature strategy $(typeof(q)) has no GPU path"))
 
 # mesh and space handles, cached per (object, context)
@@ -123,7 +125,7 @@ function spacehandle(X::BEAST.RTBasis, ctx)
     end
 end
 
-const GPUOp = Union{BEAST.MWSingleLayer3D,BEAST.MWDoubleLayer3D}
+const GPUOp = Union{BEAST.MWSingleLayer3D,BEAST.MWDoubleLayer3D,BEAST.NCross}
 
 """
     gpu_assemble!(Z, op, X, Y, qs; scale=1, row_offset=0, col_offset=0)
@@ -187,6 +189,17 @@ function DeviceMatrix(Z::Matrix{ComplexF64}; ctx=context())
 end
 
 const PRECISIONS = Dict(:fp64 => Cint(0), :fp32 => Cint(1), :fp32split => Cint(2))
+
+"""`accumulate!(M, op, X; scale)`: M += scale * op(X, X) in HBM (a further term of a
+linear combination, e.g. the N/2 of the MFIE K + N/2)."""
+function accumulate!(M::DeviceMatrix, op::GPUOp, X::BEAST.RTBasis; scale=1.0, quadstrat=BEAST.defaultquadstrat,
+        ctx=context())
+    s = Float64[real(scale), imag(scale)]
+    check(ccall((:bh_matrix_accumulate, libbeast_hip), Cint,
+        (Ptr{Cvoid}, Ref{BHOperator}, Ref{BHQuadStrat}, Ptr{Cvoid}, Ptr{Float64}),
+        M.ptr, Ref(bhop(op)), Ref(bhqs(quadstrat(op, X, X))), spacehandle(X, ctx), s))
+    return M
+end
 
 """`factor!(M, :fp64)` (getrf), `factor!(M, :fp32)` (FP32 LU for GMRES-IR) or
 `factor!(M, :fp32split)` (FP32 LU whose trailing updates use split-bf16 products)."""

@@ -58,18 +58,23 @@ class DoubleNumQStrat(AbstractQuadStrat):
 
 
 class SingleNumQStrat(AbstractQuadStrat):
+    kind = 3
+
     def __init__(self, quad_rule):
         self.quad_rule = quad_rule
+        self.params = (quad_rule,)
 
     def __repr__(self):
         return "SingleNumQStrat(%d)" % self.quad_rule
 
 
 def defaultquadstrat(op, X=None, Y=None):
-    """defaultquadstrat for Maxwell operators on RT x RT (mwops.jl:55) and for
-    functionals (excitation.jl:6)."""
+    """defaultquadstrat for Maxwell operators on RT x RT (mwops.jl:55), for local
+    operators on RT x RT (identityop.jl:66-67) and for functionals (excitation.jl:6)."""
     if isinstance(op, Functional):
         return SingleNumQStrat(8)
+    if isinstance(op, LocalOperator):
+        return SingleNumQStrat(6)
     return DoubleNumWiltonSauterQStrat(2, 3, 6, 7, 5, 5, 4, 3)
 
 
@@ -140,6 +145,18 @@ class MWDoubleLayer3D(IntegralOperator):
 
     def __repr__(self):
         return f"MWDoubleLayer3D(alpha={self.alpha}, gamma={self.gamma})"
+
+
+class LocalOperator(IntegralOperator):
+    """Operators whose matrix couples only functions sharing a cell (localop.jl:4)."""
+
+
+class NCross(LocalOperator):
+    """NCross (identityop.jl:14-22): the identity with the trial function rotated,
+    integrand g . (n x f) for test g and trial f."""
+
+    def __repr__(self):
+        return "NCross()"
 
 
 class LinearCombinationOfOperators(IntegralOperator):

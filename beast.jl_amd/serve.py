@@ -26,7 +26,7 @@ import numpy as np
 
 from . import assembly as asm
 from .mesh import meshcube, meshellipsoid, meshsphere
-from .operators import Maxwell3D, cross, n as nvec
+from .operators import Maxwell3D, NCross, cross, n as nvec
 from .rt import raviartthomas
 
 
@@ -86,7 +86,7 @@ def batched_requests(seed=4, count=64, nmin=500, nmax=8000, kinds=("efie",)):
     return reqs
 
 
-def poisson_requests(seed=5, rate=20.0, count=200, nmin=500, nmax=30000, kinds=("efie",), repeat_prob=0.3):
+def poisson_requests(seed=5, rate=20.0, count=200, nmin=500, nmax=30000, kinds=("efie", "mfie"), repeat_prob=0.3):
     """Serving workload: Poisson arrivals; a share of requests repeat an earlier (geometry, k) with new RHS."""
     rng = np.random.default_rng(seed)
     reqs, t, seen = [], 0.0, []
@@ -145,9 +145,14 @@ class Server:
         self._reserved += nbytes
         return True
 
-    def _operator(self, kind, k):
+    @staticmethod
+    def _operator(kind, k):
+        """EFIE: T.  MFIE: K + N/2 with N = NCross, both tested with RT (examples/mfie.jl
+        tests with Buffa-Christiansen functions; the RT-tested form keeps one space)."""
         if kind == "efie":
             return Maxwell3D.singlelayer(wavenumber=k)
+        if kind == "mfie":
+            return Maxwell3D.doublelayer(wavenumber=k) + 0.5 * NCross()
         raise ValueError(f"request kind {kind!r} is not served")
 
     def process(self, req, ctx):
@@ -181,7 +186,7 @@ class Server:
                     self._reserved -= nbytes
                     self._resident[key] = (M, nbytes)
                     self._lock.notify_all()
-            B = np.stack([self._rhs(X, req.wavenumber, d, p, ctx) for d, p in req.rhs], axis=1)
+            B = np.stack([self._rhs(X, req.kind, req.wavenumber, d, p, ctx) for d, p in req.rhs], axis=1)
             x = M.solve(B, tol=self.tol, maxit=self.maxit) if self.precision == "fp32" else M.solve(B)
             req.info = dict(M.stats or {})
             req.result = x
@@ -192,9 +197,12 @@ class Server:
         req.done = time.perf_counter()
 
     @staticmethod
-    def _rhs(X, k, d, p, ctx):
-        E = Maxwell3D.planewave(direction=d, polarization=p, wavenumber=k)
-        return asm.assemble_rhs(cross(cross(nvec, E), nvec), X, ctx=ctx)
+    def _rhs(X, kind, k, d, p, ctx):
+        """EFIE: (n x E) x n.  MFIE: (n x H) x n with H = -curl E / (i mu omega), which for
+        the plane wave p exp(-i k d.x) (eps = mu = 1, omega = k) is the plane wave d x p."""
+        pol = p if kind == "efie" else np.cross(d, p)
+        F = Maxwell3D.planewave(direction=d, polarization=pol, wavenumber=k)
+        return asm.assemble_rhs(cross(cross(nvec, F), nvec), X, ctx=ctx)
 
     def run(self, requests, realtime=True):
         """Serve ``requests`` (arrival times in seconds); returns the summary statistics."""

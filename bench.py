@@ -218,7 +218,7 @@ def main():
             srv.close()
             out["serving"] = dict(st, workload=f"{args.serve_requests} requests, Poisson 20/s (seed 5), geometry mix "
                                               "with N log-uniform in [500, 30000], 30% repeats on resident factors, "
-                                              "EFIE, mixed precision")
+                                              "EFIE or MFIE (K + N/2, RT-tested) with p = 0.5, mixed precision")
             log(f"bench: serving {st}")
     if not args.no_cpu_baseline:
         log("bench: CPU baseline sample")

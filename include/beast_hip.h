@@ -37,8 +37,10 @@ enum bh_status {
     BH_ERR_NOCONV = -7        /* iterative refinement did not reach the tolerance */
 };
 
-/* Operator kinds (src/maxwell/maxwell.jl:16-79). */
-enum bh_op_kind { BH_OP_MW_SINGLELAYER = 0, BH_OP_MW_DOUBLELAYER = 1 };
+/* Operator kinds (src/maxwell/maxwell.jl:16-79; BH_OP_NCROSS is the local
+ * operator NCross, src/identityop.jl:14-22: <f, n x g> on common cells of one
+ * mesh, gamma/alpha/beta unused, quadstrat BH_QS_SINGLENUM). */
+enum bh_op_kind { BH_OP_MW_SINGLELAYER = 0, BH_OP_MW_DOUBLELAYER = 1, BH_OP_NCROSS = 2 };
 
 /* Quadrature strategy kinds; the integer parameters follow the field order of
  * the reference structs:
@@ -48,9 +50,11 @@ enum bh_op_kind { BH_OP_MW_SINGLELAYER = 0, BH_OP_MW_DOUBLELAYER = 1 };
  *   BH_QS_SAUTER         DoubleNumSauterQstrat(outer, inner, ss_tetr, ss_face,
  *                        ss_edge, ss_vert)   src/quadrature/doublenumsauterqstrat.jl:1-8
  *   BH_QS_DOUBLENUM      DoubleNumQStrat(outer, inner)   src/quadrature/doublenumqstrat.jl:1-4
+ *   BH_QS_SINGLENUM      SingleNumQStrat(quad_rule)   src/quadrature/quadstrats.jl:48-50
+ *                        (local operators; RT x RT default 6, src/identityop.jl:66-67)
  * Triangle rule parameters are Dunavant polynomial degrees 1..8; Sauter-Schwab
  * parameters are Gauss-Legendre orders 1..16. */
-enum bh_qs_kind { BH_QS_WILTON_SAUTER = 0, BH_QS_SAUTER = 1, BH_QS_DOUBLENUM = 2 };
+enum bh_qs_kind { BH_QS_WILTON_SAUTER = 0, BH_QS_SAUTER = 1, BH_QS_DOUBLENUM = 2, BH_QS_SINGLENUM = 3 };
 
 /* Arithmetic.  Assembly: BH_FP64 everywhere, or BH_FP32 for the far-field
  * double-Gauss only (classification, Sauter-Schwab and Wilton blocks stay FP64
@@ -161,6 +165,10 @@ typedef struct {
 /* Assemble op on (X, X) into a resident FP64 matrix (row-major in HBM). */
 int bh_matrix_assemble(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, const bh_space *X,
                        bh_matrix **out);
+/* Z += scale * op(X, X) on a resident FP64 matrix (before it is factored); the
+ * resident form of a LinearCombinationOfOperators term (operator.jl:294-306). */
+int bh_matrix_accumulate(bh_matrix *M, const bh_operator *op, const bh_quadstrat *qs, const bh_space *X,
+                         const double *scale);
 /* Resident matrix from host data (column-major complex double, n x n). */
 int bh_matrix_from_host(bh_ctx *ctx, int64_t n, const double *A, int64_t ld, bh_matrix **out);
 int bh_matrix_destroy(bh_matrix *M);

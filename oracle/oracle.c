@@ -27,6 +27,7 @@
  *                       which is not available here)
  *   scatter             src/integralop.jl:146-184, src/operator.jl:168-184
  *   plane-wave RHS      src/excitation.jl:38-115,149-198, src/maxwell/mwexc.jl:36-42
+ *   NCross              src/identityop.jl:14-22, src/localop.jl:114-146,362-386
  *
  * Parity pins (see tests/test_oracle_*.py): Dunavant exactness, Wilton vs brute
  * force, DQ vs Wilton-SE on far pairs, SS vs Wilton-SE at the reference test
@@ -717,6 +718,70 @@ int or_assemble(int op_kind, const double *g, const double *al, const double *be
     free(tm);
     (void)nds;
     return err;
+}
+
+/* ------------------------------------------------------------------ */
+/* NCross on RT x RT over one mesh (identityop.jl:20-22; the matched-   */
+/* mesh loop localop.jl:114-146 with cellinteractions_matched!,         */
+/* localop.jl:362-386; SingleNumQStrat(degree), identityop.jl:66-67):   */
+/*   locmat[i,j] = sum_q w_q f_i(x_q) . (n x f_j(x_q)) per cell,         */
+/*   Z[m,n] += scale * a * locmat[i,j] * b.                              */
+/* Z column-major, complex interleaved, ld = ntest.                      */
+/* ------------------------------------------------------------------ */
+int or_ncross(const double *V, const int32_t *C, int nf, int nd, const int32_t *tptr, const int32_t *tcell,
+              const int32_t *tref, const double *tcoef, int ns, const int32_t *sptr, const int32_t *scell,
+              const int32_t *sref, const double *scoef, int degree, const double *scale, double *Z)
+{
+    int Ks, Kt;
+    slot_t *sm = build_cellmap(nf, ns, sptr, scell, sref, scoef, &Ks);
+    slot_t *tm = build_cellmap(nf, nd, tptr, tcell, tref, tcoef, &Kt);
+    const cplx sc = scale[0] + I * scale[1];
+    for (int p = 0; p < nf; ++p) {
+        double tv[9];
+        for (int a = 0; a < 3; ++a)
+            for (int q = 0; q < 3; ++q) tv[3 * a + q] = V[3 * C[3 * p + a] + q];
+        double xp[3 * OR_MAXQ], wx[OR_MAXQ];
+        const int np = tri_points(tv, degree, xp, wx);
+        if (np < 0) { free(sm); free(tm); return -1; }
+        const double A = tri_area(tv);
+        double t1[3], t2[3], nrm[3];
+        sub3(t1, tv, tv + 6);
+        sub3(t2, tv + 3, tv + 6);
+        cross3(nrm, t1, t2);
+        const double nn = norm3(nrm);
+        for (int q = 0; q < 3; ++q) nrm[q] /= nn;
+        double loc[3][3] = {{0}};
+        for (int k = 0; k < np; ++k) {
+            const double *x = xp + 3 * k;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double fi[3], fj[3], nf_[3];
+                    for (int q = 0; q < 3; ++q) {
+                        fi[q] = (x[q] - tv[3 * i + q]) / (2.0 * A);
+                        fj[q] = (x[q] - tv[3 * j + q]) / (2.0 * A);
+                    }
+                    cross3(nf_, nrm, fj);
+                    loc[i][j] += wx[k] * dot3(fi, nf_);
+                }
+        }
+        /* store(a * locmat[i,j] * b, m, n) over the test and trial slots of the cell */
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int ka = 0; ka < Kt; ++ka) {
+                    const slot_t ta = tm[(size_t)(3 * p + i) * Kt + ka];
+                    if (ta.dof < 0) continue;
+                    for (int kb = 0; kb < Ks; ++kb) {
+                        const slot_t sb = sm[(size_t)(3 * p + j) * Ks + kb];
+                        if (sb.dof < 0) continue;
+                        const cplx v = sc * (ta.c * loc[i][j] * sb.c);
+                        Z[2 * ((size_t)ta.dof + (size_t)nd * sb.dof)] += creal(v);
+                        Z[2 * ((size_t)ta.dof + (size_t)nd * sb.dof) + 1] += cimag(v);
+                    }
+                }
+    }
+    free(sm);
+    free(tm);
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -47,6 +47,9 @@ def lib():
         L.or_assemble.restype = C.c_int
         L.or_planewave.argtypes = [_D, _I32, C.c_int, C.c_int, _I32, _I32, _I32, _D, _D, _D, _D, _D, _D, C.c_int, _D]
         L.or_planewave.restype = C.c_int
+        L.or_ncross.argtypes = [_D, _I32, C.c_int, C.c_int, _I32, _I32, _I32, _D, C.c_int, _I32, _I32, _I32, _D,
+                                C.c_int, _D, _D]
+        L.or_ncross.restype = C.c_int
         _lib = L
     return _lib
 
@@ -140,6 +143,23 @@ def assemble(kind, gamma, alpha, beta, qs_kind, qs, X, Y, want_classes=False):
     if rc != 0:
         raise AssertionError("hits <= 3")
     return (Z, cls) if want_classes else Z
+
+
+def ncross(X, Y, degree=6, scale=1.0):
+    """Dense NCross matrix (test X, trial Y on the same mesh), column-major complex128."""
+    if X.geo is not Y.geo:
+        raise ValueError("NCross oracle: test and trial spaces must share the mesh")
+    m = X.geo
+    V, Cc = np.ascontiguousarray(m.vertices, float), np.ascontiguousarray(m.cells, np.int32)
+    nt, tp, tc, tr, tco = _space_arrays(X)
+    ns, sp, sc, sr, sco = _space_arrays(Y)
+    Z = np.zeros((nt, ns), dtype=np.complex128, order="F")
+    i32 = lambda t: t.ctypes.data_as(_I32)
+    rc = lib().or_ncross(_d(V), i32(Cc), m.numcells, nt, i32(tp), i32(tc), i32(tr), _d(tco),
+                         ns, i32(sp), i32(sc), i32(sr), _d(sco), int(degree), _d(_c2(scale)), Z.ctypes.data_as(_D))
+    if rc != 0:
+        raise ValueError("bad rule degree")
+    return Z
 
 
 def planewave(X, direction, polarization, gamma, amplitude=1.0, degree=8):

@@ -196,3 +196,31 @@ def test_planewave_rhs(bj, orc, golden):
     b = bj.assemble(e, X)
     bo = orc.planewave(X, bj.z_hat, bj.x_hat, 1j * k)
     assert relfro(b, bo) < 1e-13
+
+
+@pytest.mark.parametrize("mesh", ["sphere2.in", "cube"])
+def test_ncross_matches_oracle(bj, orc, golden, mesh):
+    """NCross (identityop.jl:20-22) on RT x RT, host mode, FP64: within 1e-12 of the oracle."""
+    m = bj.readmesh(os.path.join(golden, mesh)) if mesh.endswith(".in") else bj.meshcube(3)
+    X = bj.raviartthomas(m)
+    Zg = bj.assemble(bj.NCross(), X, X)
+    Zo = orc.ncross(X, X)
+    assert relfro(Zg, Zo) < 1e-12
+
+
+def test_mfie_resident_combination(bj, orc, golden):
+    """Resident K + N/2 (a linear combination: K assembled, N/2 accumulated in place) equals the
+    host assembly of K plus half the oracle's NCross; then an MFIE solve converges."""
+    m = bj.readmesh(os.path.join(golden, "sphere2.in"))
+    X = bj.raviartthomas(m)
+    k = 1.0
+    K = bj.Maxwell3D.doublelayer(wavenumber=k)
+    M = bj.DeviceMatrix.assemble(K + 0.5 * bj.NCross(), X)
+    Zref = bj.assemble(K, X, X) + 0.5 * orc.ncross(X, X)
+    assert relfro(M.download(), Zref) < 1e-13
+    d, p = np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0])
+    H = bj.Maxwell3D.planewave(direction=d, polarization=np.cross(d, p), wavenumber=k)
+    b = bj.assemble(bj.cross(bj.cross(bj.n, H), bj.n), X)
+    M.factor("fp32split")
+    u = M.solve(b, tol=1e-12)
+    assert np.linalg.norm(Zref @ u - b) / np.linalg.norm(b) < 1e-11

@@ -51,3 +51,21 @@ def test_admission_budget(bj):
     assert stats["max_resident_gb"] * 1e9 <= 2.2 * sv.Server.footprint(n_max)
     with pytest.raises(MemoryError):
         sv.Server(workers=1, budget_bytes=1e3).run(sv.batched_requests(seed=1, count=1), realtime=False)
+
+
+def test_mfie_requests_alone_and_in_mix(bj):
+    """MFIE requests (K + N/2, RHS from the magnetic plane wave) mixed with EFIE ones: each
+    converges and gives the same bits alone and in the mix."""
+    sv = _serve(bj)
+    reqs = sv.batched_requests(seed=12, count=8, nmin=300, nmax=1200, kinds=("efie", "mfie"))
+    assert {r.kind for r in reqs} == {"efie", "mfie"}
+    srv = sv.Server(workers=3, tol=1e-12)
+    srv.run(reqs, realtime=False)
+    srv.close()
+    for r in [q for q in reqs if q.kind == "mfie"][:2]:
+        assert r.info["residual"] <= 1e-12
+        alone = sv.Request(r.rid, 0.0, r.shape, r.dofs, r.kind, r.wavenumber, r.rhs)
+        s1 = sv.Server(workers=1, tol=1e-12)
+        s1.run([alone], realtime=False)
+        s1.close()
+        assert np.array_equal(alone.result, r.result), r.rid

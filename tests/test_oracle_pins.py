@@ -173,3 +173,46 @@ def test_farfield_golden(bj, orc, golden):
     u = orc.lu_solve(Z, b)
     a = brute.farfield_magnitudes(bj, u, X, np.linspace(0, np.pi, 7))
     assert np.abs(a - np.array(brute.FARFIELD_GOLDEN)).max() < 1e-3
+
+
+def _ncross_closed_form(X):
+    """NCross on RT x RT from the closed form of its cell integral: f_i . (n x f_j) =
+    n . ((x - v_j) x (x - v_i)) / (4 A^2) is affine in x, so its integral over the cell is
+    A n . (c x (v_j - v_i) + v_j x v_i) / (4 A^2) with c the centroid (no quadrature)."""
+    m = X.geo
+    V, C = np.asarray(m.vertices, float), np.asarray(m.cells)
+    N = X.numfunctions()
+    Z = np.zeros((N, N))
+    ptr, cell, ref, coef = (np.asarray(a) for a in (X.shape_ptr, X.shape_cell, X.shape_ref, X.shape_coeff))
+    by_cell = {}
+    for d in range(N):
+        for k in range(ptr[d], ptr[d + 1]):
+            by_cell.setdefault(int(cell[k]), []).append((d, int(ref[k]), float(coef[k])))
+    for c, shapes in by_cell.items():
+        v = V[C[c]]
+        t = np.cross(v[0] - v[2], v[1] - v[2])
+        A = 0.5 * np.linalg.norm(t)
+        nrm = t / np.linalg.norm(t)
+        ctr = v.mean(axis=0)
+        for (m_, i, a) in shapes:
+            for (n_, j, b) in shapes:
+                loc = nrm @ (np.cross(ctr, v[j] - v[i]) + np.cross(v[j], v[i])) / (4 * A)
+                Z[m_, n_] += a * loc * b
+    return Z
+
+
+def test_ncross_oracle_closed_form(bj, orc, golden):
+    """The oracle's NCross (identityop.jl:20-22 integrated with SingleNumQStrat(6) as localop.jl
+    does) equals the closed-form cell integrals, for the default rule and the 1-point rule (the
+    integrand is affine, so every rule is exact), and is antisymmetric on RT x RT."""
+    m = bj.readmesh(os.path.join(golden, "sphere45.in"))
+    X = bj.raviartthomas(m)
+    Zc = _ncross_closed_form(X)
+    Z6 = orc.ncross(X, X, degree=6)
+    Z1 = orc.ncross(X, X, degree=1)
+    assert np.abs(Z6.imag).max() == 0.0
+    assert np.abs(Z6.real - Zc).max() < 1e-14 * np.abs(Zc).max()
+    assert np.abs(Z1.real - Zc).max() < 1e-14 * np.abs(Zc).max()
+    assert np.abs(Z6 + Z6.T).max() < 1e-14 * np.abs(Zc).max()
+    Zs = orc.ncross(X, X, degree=6, scale=0.5j)
+    assert np.abs(Zs - 0.5j * Z6).max() < 1e-16
