@@ -4,19 +4,36 @@
 // solves with getrs through LinearAlgebra (src/solvers/lusolver.jl:26-34,
 // BEAST.lu in src/solvers/lu.jl:29-47).  Here the matrix stays in HBM, row-major
 // interleaved complex, and is factored by a right-looking blocked LU
-// (Golub & Van Loan, Matrix Computations, Alg. 3.4.x) with:
-//   k_panel       32-column panel factorisation as one cooperative kernel: each
-//                 workgroup keeps its rows of the panel in LDS; per column one grid
-//                 barrier exchanges the per-workgroup pivot candidates (value, row
-//                 and the row itself), so every workgroup swaps and updates locally.
-//                 Pivot choice is LAPACK's: largest |Re|+|Im|, first row on ties.
-//   k_laswp       row interchanges, one thread per column (coalesced in row-major)
-//   k_trsm_unit   32-row unit-lower triangular solve, one thread per column
-//   k_cgemm       C -= A B on MFMA (v_mfma_{f32,f64}_16x16x4): the complex product
-//                 as four real contractions with FP32/FP64 accumulation.
-// Solves: permutation gather, blocked forward/backward substitution
-// (k_trsv_step), FP64 matvec with the resident FP64 matrix, and GMRES-IR
-// (Carson & Higham, SIAM J. Sci. Comput. 40 (2018)) when the factors are FP32.
+// (Golub & Van Loan, Matrix Computations, Alg. 3.4.x).
+//
+// FP64 factors (block depth 256) keep LAPACK's pivots exactly:
+//   k_panel         32-column panel as one cooperative kernel: each workgroup keeps
+//                   its rows in LDS; per column one grid barrier exchanges the
+//                   per-workgroup pivot candidates (value, row, row contents).
+//                   Pivot choice is LAPACK's: largest |Re|+|Im|, first row on ties.
+// FP32 factors (block depth 512) pivot each 32-column panel by tournament
+// (communication-avoiding LU, Grigori, Demmel & Xiang, SIAM J. Matrix Anal. Appl.
+// 32 (2011)):
+//   k_tslu_stage    stage 0: 256 rows per workgroup, one per thread in registers,
+//                   reduced to 32 candidates by partial pivoting; later stages merge
+//                   8 candidate sets; the last emits L11\U11, ipiv and a row plan.
+//   k_tslu_apply    moves the planned rows in the block's other columns and turns
+//                   the panel's rows into L21 = A21 U11^-1.
+//   The next block column is factored on a second stream while the trailing
+//   update of the current one runs (look-ahead by one block).
+// Shared by both:
+//   k_swap_plan, k_laswp_apply  row interchanges: the sequential swaps replayed
+//                   once on row indices, then each moved row moved once.
+//   k_trsm_unit     32-row unit-lower solve for U12, column in registers.
+//   k_cgemm         C -= A B on MFMA (v_mfma_{f32,f64}_16x16x4): FP64 updates and
+//                   the FP32 updates inside a block column.
+//   k_cgemm_bf16x3  FP32 trailing update from split-bf16 planes (hi*hi + hi*lo +
+//                   lo*hi on v_mfma_f32_16x16x32_bf16, FP32 accumulation).
+// Solves: permutation gather, then for FP64 factors blocked substitution
+// (k_trsv_step); for FP32 factors products with the inverted 128x128 diagonal
+// blocks (k_trinv, k_trsv_inv_step).  FP32 factors precondition flexible GMRES
+// inside iterative refinement in FP64 (GMRES-IR, Carson & Higham, SIAM J. Sci.
+// Comput. 40 (2018)), whose residuals use the resident FP64 matrix.
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
