@@ -271,72 +271,100 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v)
 // (LDS) receives the thread whose row pivots column j (-1 if none is left).
 // Pivot choice: largest |Re|+|Im| at 2^-12 relative resolution (key = value bits
 // with the low 11 mantissa bits replaced by 2047 - thread), smaller thread on
-// ties; the pivot row is broadcast through LDS, the others are eliminated in
-// registers.  The pivot rows keep their factored entries (L left of the
-// diagonal, U from it).  The columns run in four phases of eight: phase P
-// updates the columns [8P, kPW) under a predicate, which keeps the code small
-// (a full unroll of the 32 x 32 elimination does not fit the instruction cache).
+// ties.  One barrier per column: each wave finds its best key by DPP and that
+// lane stores its row in the wave's slot (slots and keys alternate between two
+// buffers by column parity), so after the barrier every thread reads the
+// winning wave's row and eliminates with it in registers.  The pivot rows keep
+// their factored entries (L left of the diagonal, U from it).  The columns run
+// in eight phases of four: phase P updates the columns [4P, kPW), those of the
+// phase under a predicate (a full unroll of the 32 x 32 elimination does not fit
+// the instruction cache).
 template <int P>
-__device__ __forceinline__ void gepp_phase(float2 (&v)[kPW], bool &used, int pw, int *piv, unsigned *s_key,
-                                           float2 *prow)
+__device__ __forceinline__ void gepp_phase(float2 (&v)[kPW], bool &used, int pw, int *piv, unsigned (*s_key)[4],
+                                           float2 (*s_row)[4][kPW])
 {
-    constexpr int C0 = 8 * P, NW = kTsluT / 64;
+    constexpr int C0 = 4 * P, NW = kTsluT / 64;
+    static_assert(NW <= 4, "one key slot per wave");
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int jend = min(C0 + 8, pw);
+    const int jend = min(C0 + 4, pw);
 #pragma unroll 1
     for (int j = C0; j < jend; ++j) {
+        const int par = j & 1;
         // column j of this row (selects per component: a conditional struct copy
         // makes the compiler move v to scratch memory)
         float2 vj = v[C0];
 #pragma unroll
-        for (int q = 1; q < 8; ++q) {
+        for (int q = 1; q < 4; ++q) {
             vj.x = (j == C0 + q) ? v[C0 + q].x : vj.x;
             vj.y = (j == C0 + q) ? v[C0 + q].y : vj.y;
         }
         const float mag = fabsf(vj.x) + fabsf(vj.y);
         const unsigned key = used ? 0u : ((__float_as_uint(mag) & ~2047u) | (unsigned)(2047 - tid));
         const unsigned wmax = wave_max_u32(key);
-        if (lane == 0) s_key[wid] = wmax;
-        __syncthreads();
-        unsigned kmax = 0;
+        if (key == wmax && wmax != 0u) {
 #pragma unroll
-        for (int q = 0; q < NW; ++q) kmax = max(kmax, s_key[q]);
+            for (int cc = C0; cc < kPW; ++cc) s_row[par][wid][cc] = v[cc];
+        }
+        if (lane == 0) s_key[par][wid] = wmax;
+        __syncthreads();
+        unsigned kmax = s_key[par][0];
+        int wbest = 0;
+#pragma unroll
+        for (int q = 1; q < NW; ++q) {
+            const unsigned kq = s_key[par][q];
+            wbest = kq > kmax ? q : wbest;
+            kmax = kq > kmax ? kq : kmax;
+        }
         const int p = kmax ? 2047 - (int)(kmax & 2047u) : -1;
         if (tid == 0) piv[j] = p;
-        if (tid == p) {
-#pragma unroll
-            for (int cc = C0; cc < kPW; ++cc) prow[cc] = v[cc];
-            used = true;
-        }
-        __syncthreads();
+        if (tid == p) used = true;
         if (!used && p >= 0) {
-            const float2 d = prow[j];
+            float2 pr[kPW];   // the pivot row, read in one batch
+#pragma unroll
+            for (int cc = C0; cc < kPW; ++cc) pr[cc] = s_row[par][wbest][cc];
+            float2 d = pr[C0];
+#pragma unroll
+            for (int q = 1; q < 4; ++q) {
+                d.x = (j == C0 + q) ? pr[C0 + q].x : d.x;
+                d.y = (j == C0 + q) ? pr[C0 + q].y : d.y;
+            }
             if (d.x != 0.f || d.y != 0.f) {
-                const float den = d.x * d.x + d.y * d.y;
-                const float2 l = cmulv(vj, float2{d.x / den, -d.y / den});
+                // multiplier vj / d through the hardware reciprocal of |d|^2
+                const float rden = __builtin_amdgcn_rcpf(fmaf(d.x, d.x, d.y * d.y));
+                const float2 l{fmaf(vj.x, d.x, vj.y * d.y) * rden, fmaf(vj.y, d.x, -vj.x * d.y) * rden};
 #pragma unroll
                 for (int cc = C0; cc < kPW; ++cc) {
-                    const float2 upd = csubv(v[cc], cmulv(l, prow[cc]));
-                    v[cc].x = cc == j ? l.x : (cc > j ? upd.x : v[cc].x);
-                    v[cc].y = cc == j ? l.y : (cc > j ? upd.y : v[cc].y);
+                    const float ux = fmaf(l.y, pr[cc].y, fmaf(-l.x, pr[cc].x, v[cc].x));
+                    const float uy = fmaf(-l.y, pr[cc].x, fmaf(-l.x, pr[cc].y, v[cc].y));
+                    if (cc < C0 + 4) {
+                        v[cc].x = cc == j ? l.x : (cc > j ? ux : v[cc].x);
+                        v[cc].y = cc == j ? l.y : (cc > j ? uy : v[cc].y);
+                    } else {
+                        v[cc].x = ux;
+                        v[cc].y = uy;
+                    }
                 }
             }
         }
     }
 }
 
-__device__ __forceinline__ void block_gepp(float2 (&v)[kPW], bool used, int pw, int *piv, unsigned *s_key,
-                                           float2 *prow)
+__device__ __forceinline__ void block_gepp(float2 (&v)[kPW], bool used, int pw, int *piv, unsigned (*s_key)[4],
+                                           float2 (*s_row)[4][kPW])
 {
-    gepp_phase<0>(v, used, pw, piv, s_key, prow);
-    gepp_phase<1>(v, used, pw, piv, s_key, prow);
-    gepp_phase<2>(v, used, pw, piv, s_key, prow);
-    gepp_phase<3>(v, used, pw, piv, s_key, prow);
+    gepp_phase<0>(v, used, pw, piv, s_key, s_row);
+    gepp_phase<1>(v, used, pw, piv, s_key, s_row);
+    gepp_phase<2>(v, used, pw, piv, s_key, s_row);
+    gepp_phase<3>(v, used, pw, piv, s_key, s_row);
+    gepp_phase<4>(v, used, pw, piv, s_key, s_row);
+    gepp_phase<5>(v, used, pw, piv, s_key, s_row);
+    gepp_phase<6>(v, used, pw, piv, s_key, s_row);
+    gepp_phase<7>(v, used, pw, piv, s_key, s_row);
     __syncthreads();
 }
 
 // x := x U^-1 for one row x (registers) and U upper triangular (LDS), column by
-// column, in the same four phases.
+// column, in four phases of eight; row i of U is read in one batch.
 template <int P>
 __device__ __forceinline__ void right_trsv_phase(float2 (&x)[kPW], int pw, const float2 (*U)[kPW + 1])
 {
@@ -344,20 +372,31 @@ __device__ __forceinline__ void right_trsv_phase(float2 (&x)[kPW], int pw, const
     const int iend = min(C0 + 8, pw);
 #pragma unroll 1
     for (int i = C0; i < iend; ++i) {
-        float2 xi = x[C0];
+        float2 ur[kPW];
+#pragma unroll
+        for (int cc = C0; cc < kPW; ++cc) ur[cc] = U[i][cc];
+        float2 xi = x[C0], d = ur[C0];
 #pragma unroll
         for (int q = 1; q < 8; ++q) {
             xi.x = (i == C0 + q) ? x[C0 + q].x : xi.x;
             xi.y = (i == C0 + q) ? x[C0 + q].y : xi.y;
+            d.x = (i == C0 + q) ? ur[C0 + q].x : d.x;
+            d.y = (i == C0 + q) ? ur[C0 + q].y : d.y;
         }
-        const float2 d = U[i][i];
-        const float den = d.x * d.x + d.y * d.y;
-        xi = den != 0.f ? float2{(xi.x * d.x + xi.y * d.y) / den, (xi.y * d.x - xi.x * d.y) / den} : float2{0, 0};
+        const float den = fmaf(d.x, d.x, d.y * d.y);
+        const float rden = den != 0.f ? __builtin_amdgcn_rcpf(den) : 0.f;
+        xi = float2{fmaf(xi.x, d.x, xi.y * d.y) * rden, fmaf(xi.y, d.x, -xi.x * d.y) * rden};
 #pragma unroll
         for (int cc = C0; cc < kPW; ++cc) {
-            const float2 upd = csubv(x[cc], cmulv(xi, U[i][cc]));
-            x[cc].x = cc == i ? xi.x : (cc > i ? upd.x : x[cc].x);
-            x[cc].y = cc == i ? xi.y : (cc > i ? upd.y : x[cc].y);
+            const float ux = fmaf(xi.y, ur[cc].y, fmaf(-xi.x, ur[cc].x, x[cc].x));
+            const float uy = fmaf(-xi.y, ur[cc].x, fmaf(-xi.x, ur[cc].y, x[cc].y));
+            if (cc < C0 + 8) {
+                x[cc].x = cc == i ? xi.x : (cc > i ? ux : x[cc].x);
+                x[cc].y = cc == i ? xi.y : (cc > i ? uy : x[cc].y);
+            } else {
+                x[cc].x = ux;
+                x[cc].y = uy;
+            }
         }
     }
 }
@@ -370,50 +409,133 @@ struct TsluFinal {
     int *info;         // first zero pivot (1-based), 0 if none
 };
 
-// One tournament stage.  MODE 0: workgroup g takes rows [c + g kTsluT, ...) of A;
-// MODE 1: workgroup g merges candidate sets [g kTsluFan, ...) of in_idx / in_val.
-// With one workgroup in the grid the stage is the last and fills fin; otherwise
-// workgroup g writes candidate set g (row ids, -1 for none, and original rows).
-template <int MODE>
-__global__ __launch_bounds__(kTsluT) void k_tslu_stage(const float2 *A, long long lda, int n, int c, int pw,
-                                                       const int *in_idx, const float2 *in_val, int nsets_in,
-                                                       int *out_idx, float2 *out_val, TsluFinal fin)
+template <typename T, int TM, int TN, bool M3>
+__device__ __forceinline__ void cgemm_tile(int id, const typename CT<T>::t *__restrict__ Ap, long long lda,
+                                           const typename CT<T>::t *__restrict__ Bp, long long ldb,
+                                           typename CT<T>::t *Cp, long long ldc, int M, int N, int K);
+
+// Counters and candidate sets of the tournament.  Level l holds n_l sets
+// (n_0 = workgroups of the first stage, n_{l+1} = ceil(n_l / kTsluFan)); its sets
+// start at set offset n_0 + ... + n_{l-1} and its group counters at
+// ceil(n_0 / kTsluFan) + ... + ceil(n_{l-1} / kTsluFan).
+struct TsluWork {
+    int *cnt;      // arrivals per group; every counter is zero between panels
+    int *idx;      // [sets][kPW] candidate row ids, -1 for none
+    float2 *val;   // [sets][kPW][kPW] their panel entries (after the previous panel's update)
+};
+
+// The tournament of the panel [c, c + pw) as one launch, and the previous
+// panel's update of the block columns to its right.
+// Workgroups [0, nsets0) each take kTsluT rows (r >= c): with prev set they first
+// apply the previous panel's update to their rows of the panel columns
+// (A[r, c:c+pw) -= L21(r) U12, L21 = A[r, c-kPW:c), U12 = A[c-kPW:c, c:c+pw)) and
+// write it back, then select <= pw candidates by partial pivoting.  A workgroup
+// publishes its set and counts itself into its group of kTsluFan sets (agent-scope
+// release / acquire); the last to arrive merges the group's sets at the next
+// level, and so on, so no workgroup waits for another.  The workgroup that
+// partial-pivots the last single set writes L11 \ U11, ipiv, the panel rows
+// before the moves and the row-move plan.
+// Workgroups [nsets0, ...) are the tiles of A[c:, c+kPW : c+kPW+upd_n) -=
+// L21 U12 with the same L21 and U12 (the rest of the previous panel's update).
+__global__ __launch_bounds__(kTsluT, 2) void k_tslu_panel(float2 *A, long long lda, int n, int c, int pw, int nsets0,
+                                                          int prev, int upd_n, TsluWork tw, TsluFinal fin)
 {
-    __shared__ unsigned s_key[kTsluT / 64];
-    __shared__ float2 prow[kPW];
-    __shared__ int piv[kPW], s_rowid[kTsluT], zero_at;
+    if ((int)blockIdx.x >= nsets0) {
+        cgemm_tile<float, 4, 2, true>((int)blockIdx.x - nsets0, A + (long long)c * lda + c - kPW, lda,
+                                      A + (long long)(c - kPW) * lda + c + kPW, lda, A + (long long)c * lda + c + kPW,
+                                      lda, n - c, upd_n, kPW);
+        return;
+    }
+    __shared__ unsigned s_key[2][4];
+    __shared__ __align__(16) float2 s_row[2][4][kPW];
+    __shared__ int piv[kPW], s_rowid[kTsluT], zero_at, s_next;
     __shared__ int s_win[kPW], s_at[kPW], s_pos[kPW];
-    const int g = blockIdx.x, tid = threadIdx.x;
-    const bool last = gridDim.x == 1;
+    __shared__ __align__(16) float2 s_u[kPW][kPW];
+    const int tid = threadIdx.x;
+    int g = blockIdx.x, nl = nsets0, off = 0, coff = 0;
     int rowid = -1;
-    size_t base = 0;   // MODE 1: first stacked row of this workgroup in in_idx / in_val
-    float2 v[kPW];
-    if (MODE == 0) {
+    float2 v[kPW], orig[kPW];
+    {
         const long long r = (long long)c + (long long)g * kTsluT + tid;
         if (r < n) rowid = (int)r;
 #pragma unroll
         for (int cc = 0; cc < kPW; ++cc) v[cc] = (rowid >= 0 && cc < pw) ? A[r * lda + c + cc] : float2{0, 0};
-    } else {
-        base = (size_t)g * kTsluFan * kPW;
-        const int nrows = min(kTsluFan, nsets_in - g * kTsluFan) * kPW;
-        if (tid < nrows) rowid = in_idx[base + tid];
+        if (prev) {
+            for (int idx = tid; idx < kPW * kPW; idx += kTsluT) {
+                const int i = idx / kPW, cc = idx % kPW;
+                s_u[i][cc] = cc < pw ? A[(long long)(c - kPW + i) * lda + c + cc] : float2{0, 0};
+            }
+            __syncthreads();
+            if (rowid >= 0) {
+                const float2 *lrow = A + r * lda + c - kPW;
+                float2 l = lrow[0];
+#pragma unroll 1
+                for (int i = 0; i < kPW; ++i) {
+                    const float2 ln = lrow[min(i + 1, kPW - 1)];   // the next multiplier, loaded ahead
+                    float2 ur[kPW];                                 // row i of U12, read in one batch
 #pragma unroll
-        for (int cc = 0; cc < kPW; ++cc) v[cc] = rowid >= 0 ? in_val[(base + tid) * kPW + cc] : float2{0, 0};
+                    for (int cc = 0; cc < kPW; ++cc) ur[cc] = s_u[i][cc];
+#pragma unroll
+                    for (int cc = 0; cc < kPW; ++cc) {
+                        v[cc].x = fmaf(l.y, ur[cc].y, fmaf(-l.x, ur[cc].x, v[cc].x));
+                        v[cc].y = fmaf(-l.y, ur[cc].x, fmaf(-l.x, ur[cc].y, v[cc].y));
+                    }
+                    l = ln;
+                }
+#pragma unroll
+                for (int cc = 0; cc < kPW; ++cc)
+                    if (cc < pw) A[r * lda + c + cc] = v[cc];
+            }
+        }
     }
+#pragma unroll
+    for (int cc = 0; cc < kPW; ++cc) orig[cc] = v[cc];
     s_rowid[tid] = rowid;
     if (tid == 0) zero_at = INT_MAX;
-    block_gepp(v, rowid < 0, pw, piv, s_key, prow);
-    if (!last) {
-        for (int idx = tid; idx < kPW * kPW; idx += kTsluT) {
-            const int j = idx / kPW, cc = idx % kPW;
-            const int p = j < pw ? piv[j] : -1;
-            const int rid = p >= 0 ? s_rowid[p] : -1;
-            if (cc == 0) out_idx[g * kPW + j] = rid;
-            float2 o{0, 0};
-            if (rid >= 0 && cc < pw) o = MODE == 0 ? A[(long long)rid * lda + c + cc] : in_val[(base + p) * kPW + cc];
-            out_val[((size_t)g * kPW + j) * kPW + cc] = o;
+    for (;;) {
+        block_gepp(v, rowid < 0, pw, piv, s_key, s_row);
+        if (nl == 1) break;
+        // publish this set: the owner of pivot j writes its row id and entries
+        const size_t set = (size_t)(off + g);
+        int j_of = -1;
+        for (int j = 0; j < pw; ++j)
+            if (piv[j] == tid) j_of = j;
+        if (j_of >= 0) {
+            tw.idx[set * kPW + j_of] = rowid;
+#pragma unroll
+            for (int cc = 0; cc < kPW; ++cc) tw.val[(set * kPW + j_of) * kPW + cc] = orig[cc];
         }
-        return;
+        if (tid < kPW && (tid >= pw || piv[tid] < 0)) tw.idx[set * kPW + tid] = -1;
+        __syncthreads();
+        if (tid == 0) {
+            const int grp = g / kTsluFan, members = min(kTsluFan, nl - grp * kTsluFan);
+            __threadfence();
+            const int arrived = __hip_atomic_fetch_add(tw.cnt + coff + grp, 1, __ATOMIC_ACQ_REL,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+            int next = -1;
+            if (arrived == members - 1) {
+                __hip_atomic_store(tw.cnt + coff + grp, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __threadfence();
+                next = grp;
+            }
+            s_next = next;
+        }
+        __syncthreads();
+        const int grp = s_next;
+        if (grp < 0) return;
+        // the last of its group: merge the group's sets one level up
+        const size_t first = (size_t)off + (size_t)grp * kTsluFan;
+        const int nrows = min(kTsluFan, nl - grp * kTsluFan) * kPW;
+        coff += (nl + kTsluFan - 1) / kTsluFan;
+        off += nl;
+        nl = (nl + kTsluFan - 1) / kTsluFan;
+        g = grp;
+        rowid = tid < nrows ? tw.idx[first * kPW + tid] : -1;
+#pragma unroll
+        for (int cc = 0; cc < kPW; ++cc) v[cc] = rowid >= 0 ? tw.val[(first * kPW + tid) * kPW + cc] : float2{0, 0};
+#pragma unroll
+        for (int cc = 0; cc < kPW; ++cc) orig[cc] = v[cc];
+        s_rowid[tid] = rowid;
     }
     // last stage: the owners of the pivot rows write L11 \ U11
     int j_of = -1;
@@ -515,6 +637,27 @@ __global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, in
         __syncthreads();
         for (int s = rl; s < m; s += 16)
             if (col >= 0) A[(long long)s_dst[s] * lda + col] = s_buf[s][cl];
+        if ((int)blockIdx.x * 16 + 16 <= w1 || b1 <= b0) return;   // no columns right of the panel
+        // the panel's block row in the columns to its right: U12 = L11^-1 (moved rows c .. c+pw),
+        // L11 the unit lower part of top; one thread per column, the column in registers
+        for (int idx = tid; idx < kPW * kPW; idx += 256) U[idx / kPW][idx % kPW] = top[idx];
+        __syncthreads();
+        if (rl != 0 || t < w1 || col < 0) return;
+        float2 x[kPW];
+#pragma unroll
+        for (int i = 0; i < kPW; ++i) x[i] = i < pw ? A[(long long)(c + i) * lda + col] : float2{0, 0};
+#pragma unroll
+        for (int i = 1; i < kPW; ++i) {
+            if (i < pw) {   // uniform
+                float2 acc = x[i];
+#pragma unroll
+                for (int j = 0; j < i; ++j) acc = csubv(acc, cmulv(U[i][j], x[j]));
+                x[i] = acc;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kPW; ++i)
+            if (i < pw) A[(long long)(c + i) * lda + col] = x[i];
         return;
     }
     for (int idx = tid; idx < kPW * kPW; idx += 256) U[idx / kPW][idx % kPW] = top[idx];
@@ -705,9 +848,9 @@ constexpr int kBK = 16;
 // inside a group), so the workgroups an XCD runs at once share A row panels
 // and B column panels in its L2.
 constexpr int kXcds = 8, kGroupM = 8;
-__device__ __forceinline__ void gemm_tile(int nbm, int nbn, int &tm, int &tn)
+__device__ __forceinline__ void gemm_tile(int id, int nbm, int nbn, int &tm, int &tn)
 {
-    const int total = nbm * nbn, id = blockIdx.x;
+    const int total = nbm * nbn;
     const int q = total / kXcds, r = total % kXcds, x = id % kXcds;
     const int t = x * q + min(x, r) + id / kXcds;
     const int per_group = kGroupM * nbn;
@@ -718,10 +861,11 @@ __device__ __forceinline__ void gemm_tile(int nbm, int nbn, int &tm, int &tn)
     tn = in / gm;
 }
 
-template <typename T, int TM, int TN, bool M3, int WPE>
-__global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__restrict__ Ap, long long lda,
-                                               const typename CT<T>::t *__restrict__ Bp, long long ldb,
-                                               typename CT<T>::t *Cp, long long ldc, int M, int N, int K)
+// One tile (linear index id, 256 threads) of the GEMM below.
+template <typename T, int TM, int TN, bool M3>
+__device__ __forceinline__ void cgemm_tile(int id, const typename CT<T>::t *__restrict__ Ap, long long lda,
+                                           const typename CT<T>::t *__restrict__ Bp, long long ldb,
+                                           typename CT<T>::t *Cp, long long ldc, int M, int N, int K)
 {
     using V = typename CT<T>::t;
     using v4 = typename MfmaVec<T>::v4;
@@ -736,7 +880,7 @@ __global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__r
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
     int tm, tn;
-    gemm_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    gemm_tile(id, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
     const long long m0 = (long long)tm * BM, n0 = (long long)tn * BN;
     const int ar = tid / AROW, ac = (tid % AROW) * APT;
     const int br = tid / BROW, bc = (tid % BROW) * BPT;
@@ -856,6 +1000,14 @@ __global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__r
             }
 }
 
+template <typename T, int TM, int TN, bool M3, int WPE>
+__global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__restrict__ Ap, long long lda,
+                                                    const typename CT<T>::t *__restrict__ Bp, long long ldb,
+                                                    typename CT<T>::t *Cp, long long ldc, int M, int N, int K)
+{
+    cgemm_tile<T, TM, TN, M3>(blockIdx.x, Ap, lda, Bp, ldb, Cp, ldc, M, N, K);
+}
+
 // ---------------------------------------------------------------- split-bf16 GEMM
 // Mixed-path trailing update C -= A B with FP32 complex C.  Each FP32 value x is
 // split into bf16 hi = bf16(x) and lo = bf16(x - hi); a real product is taken as
@@ -912,28 +1064,30 @@ __global__ __launch_bounds__(256) void k_split_cols(const float2 *__restrict__ B
 }
 
 // C[M x N] -= A B from split planes PA[4][M][K], PB[4][N][K]; K a multiple of 32.
-// 4 waves in 2 x 2, wave tile (16 TM) x (16 TN) complex.
-template <int TM, int TN>
-__global__ __launch_bounds__(256, 2) void k_cgemm_bf16x3(const __bf16 *__restrict__ PA, const __bf16 *__restrict__ PB,
-                                                         float2 *C, long long ldc, int M, int N, int K)
+// WM x WN waves, wave tile (16 TM) x (16 TN) complex.
+template <int TM, int TN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *__restrict__ PA,
+                                                                 const __bf16 *__restrict__ PB, float2 *C,
+                                                                 long long ldc, int M, int N, int K)
 {
-    constexpr int BM = 32 * TM, BN = 32 * TN, BK = kSplitBK, RS = BK + kSplitPad;
+    constexpr int NT = 64 * WM * WN;
+    constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN, BK = kSplitBK, RS = BK + kSplitPad;
     __shared__ __align__(16) __bf16 As[4][BM][RS];
     __shared__ __align__(16) __bf16 Bs[4][BN][RS];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int wm = wv >> 1, wn = wv & 1;
+    const int wm = wv / WN, wn = wv % WN;
     int tm, tn;
-    gemm_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    gemm_tile(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
     const long long m0 = (long long)tm * BM, n0 = (long long)tn * BN;
     const long long planeA = (long long)M * K, planeB = (long long)N * K;
     // 16-byte chunks (8 bf16) per chunk of K: A has 4 * BM * 4, B has 4 * BN * 4
     constexpr int ACH = 4 * BM * (BK / 8), BCH = 4 * BN * (BK / 8);
-    constexpr int AP = (ACH + 255) / 256, BP = (BCH + 255) / 256;
+    constexpr int AP = (ACH + NT - 1) / NT, BP = (BCH + NT - 1) / NT;
     uint4 ra[AP], rb[BP];
     auto load = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < AP; ++q) {
-            const int c = tid + 256 * q;
+            const int c = tid + NT * q;
             if (c < ACH) {
                 const int p = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, g = rem % 4;
                 const long long row = m0 + r;
@@ -942,7 +1096,7 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_bf16x3(const __bf16 *__restric
         }
 #pragma unroll
         for (int q = 0; q < BP; ++q) {
-            const int c = tid + 256 * q;
+            const int c = tid + NT * q;
             if (c < BCH) {
                 const int p = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, g = rem % 4;
                 const long long col = n0 + r;
@@ -953,7 +1107,7 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_bf16x3(const __bf16 *__restric
     auto store = [&]() {
 #pragma unroll
         for (int q = 0; q < AP; ++q) {
-            const int c = tid + 256 * q;
+            const int c = tid + NT * q;
             if (c < ACH) {
                 const int p = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, g = rem % 4;
                 *(uint4 *)&As[p][r][8 * g] = ra[q];
@@ -961,7 +1115,7 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_bf16x3(const __bf16 *__restric
         }
 #pragma unroll
         for (int q = 0; q < BP; ++q) {
-            const int c = tid + 256 * q;
+            const int c = tid + NT * q;
             if (c < BCH) {
                 const int p = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, g = rem % 4;
                 *(uint4 *)&Bs[p][r][8 * g] = rb[q];
@@ -1454,8 +1608,9 @@ struct LUWork {
     int maxG = 0;
     int rpb_max = 0;
     DevBuf<int> plan;                      // row-interchange plan (k_swap_plan)
-    DevBuf<int> tidx[2];                   // tournament candidate sets (row ids)
-    DevBuf<float2> tval[2];                // their original panel rows
+    DevBuf<int> tcnt;                      // tournament group counters (zero between panels)
+    DevBuf<int> tidx;                      // tournament candidate sets (row ids), all levels
+    DevBuf<float2> tval;                   // their panel rows
     DevBuf<float2> top, oldtop;            // factored winners; panel rows before the moves
     DevBuf<int> tplan;                     // row moves of the last tournament
 };
@@ -1537,26 +1692,24 @@ void panel(Ctx &ctx, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, i
 }
 
 // Panel of columns [c, c + pw) of the block column [k, k + kb) by tournament
-// pivoting: the stages choose the pivot rows and write ipiv[c .. c+pw); the
-// apply kernel moves the rows in the block's other columns and finishes the panel.
+// pivoting.  One launch applies the previous panel's update to the block's
+// columns right of it (prev) and runs the tournament, which writes ipiv[c .. c+pw);
+// the apply launch moves the rows in the block's other columns, finishes the
+// panel rows and forms the panel's block row U12 right of the panel.
 void tslu_panel(Ctx &ctx, hipStream_t s, LUWork<float> &W, float2 *A, long long lda, int n, int k, int kb, int c,
-                int pw, long long *ipiv)
+                int pw, long long *ipiv, bool prev)
 {
     const TsluFinal fin{ipiv, W.top.p, W.oldtop.p, W.tplan.p, W.info.p};
+    const TsluWork tw{W.tcnt.p, W.tidx.p, W.tval.p};
     {
-        ProfScope prof(ctx, "k_tslu_stage", (double)(n - c) * pw * sizeof(float2), s);
-        int nsets = (n - c + kTsluT - 1) / kTsluT;
-        k_tslu_stage<0><<<nsets, kTsluT, 0, s>>>(A, lda, n, c, pw, nullptr, nullptr, 0, W.tidx[0].p, W.tval[0].p, fin);
-        check("k_tslu_stage");
-        int cur = 0;
-        while (nsets > 1) {
-            const int nout = (nsets + kTsluFan - 1) / kTsluFan;
-            k_tslu_stage<1><<<nout, kTsluT, 0, s>>>(A, lda, n, c, pw, W.tidx[cur].p, W.tval[cur].p, nsets,
-                                                    W.tidx[cur ^ 1].p, W.tval[cur ^ 1].p, fin);
-            check("k_tslu_stage");
-            cur ^= 1;
-            nsets = nout;
-        }
+        const int nsets0 = (n - c + kTsluT - 1) / kTsluT;
+        const int upd_n = prev ? std::max(0, k + kb - (c + kPW)) : 0;
+        const long long tiles = upd_n > 0 ? (long long)((n - c + 127) / 128) * ((upd_n + 63) / 64) : 0;
+        BH_REQUIRE(nsets0 + tiles < (1LL << 31), "tournament grid too large");
+        ProfScope prof(ctx, "k_tslu_panel", (double)(n - c) * pw * sizeof(float2) * (prev ? 3 : 1), s);
+        k_tslu_panel<<<(unsigned)(nsets0 + tiles), kTsluT, 0, s>>>(A, lda, n, c, pw, nsets0, prev ? 1 : 0, upd_n, tw,
+                                                                  fin);
+        check("k_tslu_panel");
     }
     ProfScope prof(ctx, "k_tslu_apply", (double)(n - c) * pw * sizeof(float2) * 2, s);
     const int w = (c - k) + (k + kb - (c + pw));
@@ -1587,13 +1740,14 @@ void gemm_split(Ctx &ctx, hipStream_t s, float2 *A, long long lda, int ar, int a
         k_split_cols<<<g2, 256, 0, s>>>(A + (long long)br * lda + bc, lda, K, N, PB.p);
         check("k_split_cols");
     }
-    constexpr int TM = 2, TN = 4, BM = 32 * TM, BN = 32 * TN;
-    const long long tiles = (long long)((N + BN - 1) / BN) * ((M + BM - 1) / BM);
-    BH_REQUIRE(tiles < (1LL << 31), "gemm grid too large");
-    const dim3 grid((unsigned)tiles);
     // work: bf16 MFMA flops executed (3 products per real product, 4 real per complex)
     ProfScope prof(ctx, "k_cgemm_bf16x3", 24.0 * M * N * K, s);
-    k_cgemm_bf16x3<TM, TN><<<grid, 256, 0, s>>>(PA.p, PB.p, A + (long long)cr * lda + cc, lda, M, N, K);
+    // 128 x 128 complex tiles, 8 waves (4 x 2) of 32 x 64
+    constexpr int TM = 2, TN = 4, WM = 4, WN = 2, BM = 16 * TM * WM, BN = 16 * TN * WN;
+    const long long tiles = (long long)((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+    BH_REQUIRE(tiles < (1LL << 31), "gemm grid too large");
+    k_cgemm_bf16x3<TM, TN, WM, WN><<<dim3((unsigned)tiles), 64 * WM * WN, 0, s>>>(
+        PA.p, PB.p, A + (long long)cr * lda + cc, lda, M, N, K);
     check("k_cgemm_bf16x3");
 }
 
@@ -1605,14 +1759,16 @@ void factor_block(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, l
     for (int sp = 0; sp < kb; sp += kPW) {
         const int c = k + sp, pw = std::min(kPW, kb - sp);
         if constexpr (sizeof(T) == 4) {
-            tslu_panel(ctx, s, W, A, lda, n, k, kb, c, pw, ipiv);
+            // the tournament launch carries the previous panel's update, the apply
+            // launch this panel's block row
+            tslu_panel(ctx, s, W, A, lda, n, k, kb, c, pw, ipiv, sp > 0);
         } else {
             panel<T>(ctx, W, A, lda, n, c, pw, ipiv);
             laswp<T>(ctx, s, A, lda, c, c + pw, ipiv, k, c, c + pw, k + kb, W.plan.p);
-        }
-        if (c + pw < k + kb) {
-            trsm_unit<T>(s, A, lda, c, pw, c, c + pw, k + kb - (c + pw));
-            gemm<T>(ctx, s, A, lda, c + pw, c, c, c + pw, c + pw, c + pw, n - (c + pw), k + kb - (c + pw), pw);
+            if (c + pw < k + kb) {
+                trsm_unit<T>(s, A, lda, c, pw, c, c + pw, k + kb - (c + pw));
+                gemm<T>(ctx, s, A, lda, c + pw, c, c, c + pw, c + pw, c + pw, n - (c + pw), k + kb - (c + pw), pw);
+            }
         }
     }
 }
@@ -1687,11 +1843,12 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
         }
     } else {
         // FP32 factors only precondition GMRES-IR: their panels use tournament pivoting
-        const size_t sets = (size_t)(n + kTsluT - 1) / kTsluT;
-        for (int q = 0; q < 2; ++q) {
-            W.tidx[q].alloc(sets * kPW);
-            W.tval[q].alloc(sets * kPW * kPW);
-        }
+        // candidate sets of all levels: n_0 + n_1 + ... < 2 n_0 + levels
+        const size_t sets = 2 * ((size_t)(n + kTsluT - 1) / kTsluT) + 8;
+        W.tcnt.alloc(sets);
+        W.tidx.alloc(sets * kPW);
+        W.tval.alloc(sets * kPW * kPW);
+        BH_CHECK_HIP(hipMemsetAsync(W.tcnt.p, 0, sets * sizeof(int), s));
         W.top.alloc((size_t)kPW * kPW);
         W.oldtop.alloc((size_t)kPW * kPW);
         W.tplan.alloc(1 + 2 * kPlanMax);
