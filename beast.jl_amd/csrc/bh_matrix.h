@@ -11,8 +11,11 @@ struct bh_matrix {
     bh::DevBuf<double> a64;        // FP64 matrix, row-major interleaved complex (n*n*2)
     bh::DevBuf<double> lu64;       // FP64 LU factors (when factored in FP64)
     bh::DevBuf<float> lu32;        // FP32 LU factors (when factored in FP32)
-    bh::DevBuf<float> dinv32;      // inverted 128 x 128 diagonal blocks of the FP32 factors (L, then U)
+    bh::DevBuf<float> dinv32;      // FP32 factors: inverted 128 x 128 diagonal blocks of L and U, then G = Dinv x (block beside it)
     bh::DevBuf<float> x32;         // FP32 solve vector (n x 8 complex)
+    bh::DevBuf<int> tsync;         // FP32 solves: ticket counter, error flag, one ready flag per 128-row block
+    long long tsync_tickets = 0;   // tickets handed out so far
+    int tsync_epoch = 0;           // value of the ready flags of the last solve
     bh::DevBuf<long long> ipiv;    // 0-based pivot rows (sequential interchanges)
     bh::DevBuf<int> perm;          // (P b)[i] = b[perm[i]]
     long long lda = 0;             // leading dimension of the factors

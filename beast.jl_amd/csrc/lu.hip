@@ -31,7 +31,7 @@
 //                   lo*hi on v_mfma_f32_16x16x32_bf16, FP32 accumulation).
 // Solves: permutation gather, then for FP64 factors blocked substitution
 // (k_trsv_step); for FP32 factors products with the inverted 128x128 diagonal
-// blocks (k_trinv, k_trsv_inv_step).  FP32 factors precondition flexible GMRES
+// blocks (k_trinv, k_trsv_flag).  FP32 factors precondition flexible GMRES
 // inside iterative refinement in FP64 (GMRES-IR, Carson & Higham, SIAM J. Sci.
 // Comput. 40 (2018)), whose residuals use the resident FP64 matrix.
 #include <hip/hip_cooperative_groups.h>
@@ -696,7 +696,12 @@ __global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, in
 // (slot len + i for the first i naming that row).  plan[s] = the row whose
 // content ends at slot s, plan[2 len + s] = the slot's row (-1: slot unused).
 constexpr int kMaxSwaps = 256;
-constexpr int kNBFP32 = 512;   // FP32 LU block depth
+// FP32 LU block depth (trailing-update depth): large matrices take 1024, whose
+// halves of kNBInner are factored one after the other on the block-column
+// stream; that halves the C traffic of the trailing updates, which pays once
+// the trailing matrix is large enough to hide the longer block-column chain.
+constexpr int kNBFP32 = 1024, kNBFP32Small = 512, kNBDeepFrom = 32768;
+constexpr int kNBInner = 512;
 
 __global__ __launch_bounds__(256) void k_swap_plan(const long long *ipiv, int k1, int k2, int *plan)
 {
@@ -1379,13 +1384,12 @@ __global__ __launch_bounds__(kIB) void k_trinv(const float2 *A, long long lda, i
 // c < cs (xin, out: LDS, row-major x nrhs; they may alias).  256 threads: thread t
 // takes row t / 2 and half t % 2 of the columns, whose 64 entries it loads into
 // registers at once (one memory round trip per block instead of one per entry).
-template <bool SUB, int NRHS>
-__device__ __forceinline__ void blk_matvec(const float2 *Mx, long long ld, int rs, int cs, const float2 *xin,
-                                           float2 *out)
+constexpr int kBlkHalf = kIB / 2;
+
+__device__ __forceinline__ void blk_load(const float2 *Mx, long long ld, int rs, int cs, float2 (&a)[kBlkHalf])
 {
-    constexpr int H = kIB / 2;
+    constexpr int H = kBlkHalf;
     const int t = threadIdx.x, i = t >> 1, h = t & 1;
-    float2 a[H];
     const float2 *row = Mx + (long long)min(i, max(rs - 1, 0)) * ld + h * H;
     if (rs == kIB && cs == kIB) {
         const float4 *row4 = (const float4 *)row;   // 16-byte aligned: ld and block origins are multiples of 64
@@ -1399,6 +1403,13 @@ __device__ __forceinline__ void blk_matvec(const float2 *Mx, long long ld, int r
 #pragma unroll
         for (int c = 0; c < H; ++c) a[c] = (i < rs && h * H + c < cs) ? row[c] : float2{0, 0};
     }
+}
+
+template <bool SUB, int NRHS>
+__device__ __forceinline__ void blk_apply(const float2 (&a)[kBlkHalf], int rs, const float2 *xin, float2 *out)
+{
+    constexpr int H = kBlkHalf;
+    const int t = threadIdx.x, i = t >> 1, h = t & 1;
     float2 acc[NRHS];
 #pragma unroll
     for (int r = 0; r < NRHS; ++r) acc[r] = {0, 0};
@@ -1428,40 +1439,147 @@ __device__ __forceinline__ void blk_matvec(const float2 *Mx, long long ld, int r
     __syncthreads();
 }
 
-template <bool LOWER, int NRHS>
-__global__ __launch_bounds__(256) void k_trsv_inv_first(const float2 *Dinv, int n, float2 *X, int blk)
+template <bool SUB, int NRHS>
+__device__ __forceinline__ void blk_matvec(const float2 *Mx, long long ld, int rs, int cs, const float2 *xin,
+                                           float2 *out)
 {
-    __shared__ float2 xs[kIB * NRHS];
-    const int b0 = blk * kIB, bs = min(kIB, n - b0);
-    // rows past the block end read as zero (their matrix entries are zero too, but 0 * garbage is not)
-    for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x) xs[t] = t < bs * NRHS ? X[(long long)b0 * NRHS + t] : float2{0, 0};
-    __syncthreads();
-    blk_matvec<false, NRHS>(Dinv + (size_t)blk * kIB * kIB, kIB, bs, bs, xs, xs);
-    for (int t = threadIdx.x; t < bs * NRHS; t += blockDim.x) X[(long long)b0 * NRHS + t] = xs[t];
+    float2 a[kBlkHalf];
+    blk_load(Mx, ld, rs, cs, a);
+    blk_apply<SUB, NRHS>(a, rs, xin, out);
 }
 
-// Block blk of X is final: every later (LOWER) / earlier block subtracts its
-// product with blk; workgroup 0 owns the adjacent block and finishes it with
-// the inverted diagonal block.
+// G[i] = Dinv[i] M[i][i -/+ 1] for the blocks with a neighbour (LOWER: the
+// block left of the diagonal of L, else the block right of the diagonal of U),
+// so the solve step that waits for the neighbour is one product.  Workgroup
+// (i, y) forms rows [16 y, 16 y + 16) of G[i]; thread t: row t / 16, columns
+// 8 (t % 16) .. + 8.
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_blk_coupling(const float2 *A, long long lda, int n, const float2 *Dinv,
+                                                      float2 *G)
+{
+    __shared__ float2 Ds[16][kIB];
+    __shared__ float2 Ms[32][kIB];
+    const int nb = (n + kIB - 1) / kIB;
+    const int i = blockIdx.x, j = LOWER ? i - 1 : i + 1;
+    if (j < 0 || j >= nb) return;   // uniform
+    const int r0 = i * kIB, rs = min(kIB, n - r0), c0 = j * kIB, cs = min(kIB, n - c0);
+    const int tid = threadIdx.x, rr = tid >> 4, cg = (tid & 15) * 8, y0 = blockIdx.y * 16;
+    for (int idx = tid; idx < 16 * kIB; idx += 256) {
+        const int r = idx / kIB, k = idx % kIB;
+        Ds[r][k] = (y0 + r < rs && k < rs) ? Dinv[(size_t)i * kIB * kIB + (size_t)(y0 + r) * kIB + k] : float2{0, 0};
+    }
+    float2 acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = {0, 0};
+    for (int k0 = 0; k0 < kIB; k0 += 32) {
+        __syncthreads();
+        for (int idx = tid; idx < 32 * kIB; idx += 256) {
+            const int k = idx / kIB, c = idx % kIB;
+            Ms[k][c] = (k0 + k < rs && c < cs) ? A[(long long)(r0 + k0 + k) * lda + c0 + c] : float2{0, 0};
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < 32; ++k) {
+            const float2 d = Ds[rr][k0 + k];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float2 m = Ms[k][cg + c];
+                acc[c].x = fmaf(d.x, m.x, fmaf(-d.y, m.y, acc[c].x));
+                acc[c].y = fmaf(d.x, m.y, fmaf(d.y, m.x, acc[c].y));
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) G[(size_t)i * kIB * kIB + (size_t)(y0 + rr) * kIB + cg + c] = acc[c];
+}
+
+// One launch per triangle.  Workgroups take their block in the order they
+// start (a ticket from an atomic counter, so every block a workgroup waits for
+// belongs to a workgroup that has already started); block i subtracts the
+// products with the blocks it depends on as their ready flags appear, applies
+// its inverted diagonal block and publishes its rows with a ready flag (agent
+// scope release / acquire).  A wait that exceeds kTrsvSpin polls sets the
+// error word and gives up, so every workgroup ends.
+constexpr long long kTrsvSpin = 1LL << 24;
+
 template <bool LOWER, int NRHS>
-__global__ __launch_bounds__(256) void k_trsv_inv_step(const float2 *A, long long lda, int n, const float2 *Dinv,
-                                                       float2 *X, int blk, int nblk)
+__global__ __launch_bounds__(256) void k_trsv_flag(const float2 *A, long long lda, int n, const float2 *Dinv,
+                                                   const float2 *G, float2 *X, int *sync, int ticket0, int epoch)
 {
     __shared__ float2 xs[kIB * NRHS];
     __shared__ float2 xb[kIB * NRHS];
-    const int next = LOWER ? blk + 1 : blk - 1;
-    const int ch = LOWER ? next + (int)blockIdx.x : next - (int)blockIdx.x;
-    if (ch < 0 || ch >= nblk) return;   // uniform per workgroup
-    const int c0 = blk * kIB, cs = min(kIB, n - c0);
-    const int r0 = ch * kIB, rs = min(kIB, n - r0);
-    for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x) {
-        xb[t] = t < cs * NRHS ? X[(long long)c0 * NRHS + t] : float2{0, 0};
-        xs[t] = t < rs * NRHS ? X[(long long)r0 * NRHS + t] : float2{0, 0};
-    }
+    __shared__ int s_idx, s_ready;
+    int *counter = sync, *err = sync + 1, *flag = sync + 2;
+    const int nblk = (n + kIB - 1) / kIB;
+    if (threadIdx.x == 0)
+        s_idx = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - ticket0;
     __syncthreads();
-    blk_matvec<true, NRHS>(A + (long long)r0 * lda + c0, lda, rs, cs, xb, xs);
-    if (blockIdx.x == 0) blk_matvec<false, NRHS>(Dinv + (size_t)next * kIB * kIB, kIB, rs, rs, xs, xs);
+    const int idx = s_idx;
+    if (idx < 0 || idx >= nblk) return;   // cannot happen: one ticket per workgroup
+    const int blk = LOWER ? idx : nblk - 1 - idx;
+    const int r0 = blk * kIB, rs = min(kIB, n - r0);
+    auto wait_flag = [&](int j) {   // thread 0
+        long long it = 0;
+        while (__hip_atomic_load(flag + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+            if (++it > kTrsvSpin) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    };
+    for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x) xs[t] = t < rs * NRHS ? X[(long long)r0 * NRHS + t] : float2{0, 0};
+    __syncthreads();
+    // every dependency but the adjacent block, taken as their ready flags appear
+    const int early = max(idx - 1, 0);
+    int done = 0;
+    while (done < early) {
+        if (threadIdx.x == 0) {
+            wait_flag(LOWER ? done : nblk - 1 - done);
+            // blocks publish in order, so every further flag already set is usable too
+            int ready = done + 1;
+            while (ready < early &&
+                   __hip_atomic_load(flag + (LOWER ? ready : nblk - 1 - ready), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) == epoch)
+                ++ready;
+            __threadfence();   // acquire: the rows of those blocks
+            s_ready = ready;
+        }
+        __syncthreads();
+        const int ready = s_ready;
+        for (; done < ready; ++done) {
+            const int j = LOWER ? done : nblk - 1 - done;
+            const int c0 = j * kIB, cs = min(kIB, n - c0);
+            for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x)
+                xb[t] = t < cs * NRHS ? X[(long long)c0 * NRHS + t] : float2{0, 0};
+            __syncthreads();
+            blk_matvec<true, NRHS>(A + (long long)r0 * lda + c0, lda, rs, cs, xb, xs);
+        }
+    }
+    // y = Dinv x without the adjacent block; then x = y - G x_adjacent, with G in
+    // registers before the wait so only the product follows it
+    blk_matvec<false, NRHS>(Dinv + (size_t)blk * kIB * kIB, kIB, rs, rs, xs, xs);
+    if (idx > 0) {
+        const int j = LOWER ? blk - 1 : blk + 1;
+        const int c0 = j * kIB, cs = min(kIB, n - c0);
+        float2 a[kBlkHalf];
+        blk_load(G + (size_t)blk * kIB * kIB, kIB, rs, cs, a);
+        if (threadIdx.x == 0) {
+            wait_flag(j);
+            __threadfence();   // acquire: the adjacent block's rows
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x)
+            xb[t] = t < cs * NRHS ? X[(long long)c0 * NRHS + t] : float2{0, 0};
+        __syncthreads();
+        blk_apply<true, NRHS>(a, rs, xb, xs);
+    }
     for (int t = threadIdx.x; t < rs * NRHS; t += blockDim.x) X[(long long)r0 * NRHS + t] = xs[t];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();   // release: this block's rows
+        __hip_atomic_store(flag + blk, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // r = b - A x in FP64 (A row-major n x n), one wave per row; x, b, r are n x nrhs row-major.
@@ -1613,6 +1731,9 @@ struct LUWork {
     DevBuf<float2> tval;                   // their panel rows
     DevBuf<float2> top, oldtop;            // factored winners; panel rows before the moves
     DevBuf<int> tplan;                     // row moves of the last tournament
+    DevBuf<int> plan_side;                 // row-interchange plan of the block-column stream
+    DevBuf<__bf16> pa_side, pb_side;       // split planes of the block-column stream
+    DevBuf<__bf16> pa_u12;                 // split planes of L for the block row
 };
 
 template <typename T, int TM, int TN, bool M3, int WPE>
@@ -1751,11 +1872,34 @@ void gemm_split(Ctx &ctx, hipStream_t s, float2 *A, long long lda, int ar, int a
     check("k_cgemm_bf16x3");
 }
 
+// Block row of rows [k, k + kb): A[k:k+kb, c0:c1) := L11^-1 A[k:k+kb, c0:c1) (with PB,
+// FP32 blocks deeper than kNBInner solve their halves and join them with one
+// split-bf16 product through PB's planes).
+template <typename T>
+void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, int k, int kb,
+                   int c0, int c1, DevBuf<__bf16> *PB = nullptr);
+
 // Block column [k, k + kb): its 32-column panels and the updates inside the block.
 template <typename T>
 void factor_block(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, int k, int kb,
                   long long *ipiv)
 {
+    if constexpr (sizeof(T) == 4) {
+        if (kb > kNBInner) {
+            // left half; its interchanges, block row and update reach the right half; right
+            // half; its interchanges reach the left half
+            const int h = kNBInner;
+            factor_block<T>(ctx, s, W, A, lda, n, k, h, ipiv);
+            for (int q = k; q < k + h; q += kMaxSwaps)
+                laswp<T>(ctx, s, A, lda, q, std::min(q + kMaxSwaps, k + h), ipiv, k + h, k + kb, 0, 0, W.plan_side.p);
+            block_row_u12<T>(ctx, s, W, A, lda, n, k, h, k + h, k + kb);
+            gemm_split(ctx, s, A, lda, k + h, k, k, k + h, k + h, k + h, n - (k + h), kb - h, h, W.pa_side, W.pb_side);
+            factor_block<T>(ctx, s, W, A, lda, n, k + h, kb - h, ipiv);
+            for (int q = k + h; q < k + kb; q += kMaxSwaps)
+                laswp<T>(ctx, s, A, lda, q, std::min(q + kMaxSwaps, k + kb), ipiv, k, k + h, 0, 0, W.plan_side.p);
+            return;
+        }
+    }
     for (int sp = 0; sp < kb; sp += kPW) {
         const int c = k + sp, pw = std::min(kPW, kb - sp);
         if constexpr (sizeof(T) == 4) {
@@ -1775,13 +1919,25 @@ void factor_block(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, l
 
 // U12 = L11^-1 A12 for block row [k, k + kb) over the columns [c0, n).
 template <typename T>
-void block_row_u12(Ctx &ctx, hipStream_t s, typename CT<T>::t *A, long long lda, int n, int k, int kb, int c0)
+void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, int k, int kb,
+                   int c0, int c1, DevBuf<__bf16> *PB)
 {
+    if (c1 <= c0) return;
+    if constexpr (sizeof(T) == 4) {
+        if (kb > kNBInner && PB) {
+            // top half, its update of the bottom half (split-bf16 products), bottom half
+            const int h = kNBInner;
+            block_row_u12<T>(ctx, s, W, A, lda, n, k, h, c0, c1);
+            gemm_split(ctx, s, A, lda, k + h, k, k, c0, k + h, c0, kb - h, c1 - c0, h, W.pa_u12, *PB);
+            block_row_u12<T>(ctx, s, W, A, lda, n, k + h, kb - h, c0, c1);
+            return;
+        }
+    }
     for (int sp = 0; sp < kb; sp += kPW) {
         const int pw = std::min(kPW, kb - sp);
-        trsm_unit<T>(s, A, lda, k + sp, pw, k + sp, c0, n - c0);
+        trsm_unit<T>(s, A, lda, k + sp, pw, k + sp, c0, c1 - c0);
         if (sp + pw < kb)
-            gemm<T>(ctx, s, A, lda, k + sp + pw, k + sp, k + sp, c0, k + sp + pw, c0, kb - (sp + pw), n - c0, pw);
+            gemm<T>(ctx, s, A, lda, k + sp + pw, k + sp, k + sp, c0, k + sp + pw, c0, kb - (sp + pw), c1 - c0, pw);
     }
 }
 
@@ -1812,11 +1968,12 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
     W.bar.alloc(kBarWords);
     W.info.alloc(1);
     W.plan.alloc(4 * kMaxSwaps);
+    W.plan_side.alloc(4 * kMaxSwaps);
     BH_CHECK_HIP(hipMemsetAsync(W.bar.p, 0, kBarWords * sizeof(unsigned), s));
     BH_CHECK_HIP(hipMemsetAsync(W.info.p, 0, sizeof(int), s));
     // block column of the right-looking LU (trailing-update depth); the FP32
     // path's deeper blocks halve the C traffic per flop of the trailing update
-    const int NB = sizeof(T) == 4 ? kNBFP32 : 256;
+    const int NB = sizeof(T) == 4 ? (n >= kNBDeepFrom ? kNBFP32 : kNBFP32Small) : 256;
     int planes_of = -1;   // block whose L21 planes PA holds
     auto trailing = [&](hipStream_t st, int k, int kb, int c0, int c1) {
         // A[k+kb:, c0:c1) -= L21 U12[:, c0:c1)
@@ -1837,7 +1994,7 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
             factor_block<T>(ctx, s, W, A, lda, n, k, kb, ipiv);
             laswp<T>(ctx, s, A, lda, k, k + kb, ipiv, 0, k, k + kb, n, W.plan.p);
             if (k + kb < n) {
-                block_row_u12<T>(ctx, s, A, lda, n, k, kb, k + kb);
+                block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, k + kb, n);
                 trailing(s, k, kb, k + kb, n);
             }
         }
@@ -1874,7 +2031,7 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
             for (int q = k; q < k + kb; q += kMaxSwaps)
                 laswp<T>(ctx, s, A, lda, q, std::min(q + kMaxSwaps, k + kb), ipiv, 0, k, k + kb, n, W.plan.p);
             if (k + kb >= n) break;
-            block_row_u12<T>(ctx, s, A, lda, n, k, kb, k + kb);
+            block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, k + kb, n, &PB);
             const int nk = k + kb, nkb = std::min(NB, n - nk);
             trailing(s, k, kb, nk, nk + nkb);   // the next block column first
             BH_CHECK_HIP(hipEventRecord(ready, s));
@@ -1912,42 +2069,56 @@ void lu_solve_inplace(hipStream_t s, const typename CT<T>::t *A, long long lda, 
 void invert_diagonal_blocks(hipStream_t s, const float2 *LU, long long lda, int n, float2 *Dinv)
 {
     const int nb = (n + kIB - 1) / kIB;
+    const size_t bsz = (size_t)nb * kIB * kIB;
     k_trinv<true><<<nb, kIB, 0, s>>>(LU, lda, n, Dinv);
     check("k_trinv");
-    k_trinv<false><<<nb, kIB, 0, s>>>(LU, lda, n, Dinv + (size_t)nb * kIB * kIB);
+    k_trinv<false><<<nb, kIB, 0, s>>>(LU, lda, n, Dinv + bsz);
     check("k_trinv");
+    k_blk_coupling<true><<<dim3(nb, kIB / 16), 256, 0, s>>>(LU, lda, n, Dinv, Dinv + 2 * bsz);
+    check("k_blk_coupling");
+    k_blk_coupling<false><<<dim3(nb, kIB / 16), 256, 0, s>>>(LU, lda, n, Dinv + bsz, Dinv + 3 * bsz);
+    check("k_blk_coupling");
 }
+
+// Synchronisation words of the one-launch triangular solves of one matrix.
+struct TrsvSync {
+    int *words;           // [0] ticket counter, [1] error flag, [2 ..] ready flags
+    long long &tickets;   // tickets handed out so far
+    int &epoch;           // ready-flag value of the last triangle
+};
 
 template <int NRHS>
-void lu_solve_inv_n(hipStream_t s, const float2 *A, long long lda, int n, const float2 *Dinv, float2 *X)
+void lu_solve_inv_n(hipStream_t s, const float2 *A, long long lda, int n, const float2 *Dinv, float2 *X, TrsvSync &ts)
 {
     const int nb = (n + kIB - 1) / kIB;
-    const float2 *DL = Dinv, *DU = Dinv + (size_t)nb * kIB * kIB;
-    k_trsv_inv_first<true, NRHS><<<1, 256, 0, s>>>(DL, n, X, 0);
-    check("k_trsv_inv_first");
-    for (int b = 0; b + 1 < nb; ++b) {
-        k_trsv_inv_step<true, NRHS><<<nb - 1 - b, 256, 0, s>>>(A, lda, n, DL, X, b, nb);
-        check("k_trsv_inv_step");
-    }
-    k_trsv_inv_first<false, NRHS><<<1, 256, 0, s>>>(DU, n, X, nb - 1);
-    check("k_trsv_inv_first");
-    for (int b = nb - 1; b > 0; --b) {
-        k_trsv_inv_step<false, NRHS><<<b, 256, 0, s>>>(A, lda, n, DU, X, b, nb);
-        check("k_trsv_inv_step");
+    const size_t bsz = (size_t)nb * kIB * kIB;
+    const float2 *DL = Dinv, *DU = Dinv + bsz, *GL = Dinv + 2 * bsz, *GU = Dinv + 3 * bsz;
+    for (int tri = 0; tri < 2; ++tri) {
+        if (ts.tickets + nb >= (1LL << 30) || ts.epoch >= (1 << 30)) {   // restart the counters
+            BH_CHECK_HIP(hipMemsetAsync(ts.words, 0, (size_t)(nb + 2) * sizeof(int), s));
+            ts.tickets = 0;
+            ts.epoch = 0;
+        }
+        const int ticket0 = (int)ts.tickets, epoch = ++ts.epoch;
+        ts.tickets += nb;
+        if (tri == 0) k_trsv_flag<true, NRHS><<<nb, 256, 0, s>>>(A, lda, n, DL, GL, X, ts.words, ticket0, epoch);
+        else k_trsv_flag<false, NRHS><<<nb, 256, 0, s>>>(A, lda, n, DU, GU, X, ts.words, ticket0, epoch);
+        check("k_trsv_flag");
     }
 }
 
-void lu_solve_inv(hipStream_t s, const float2 *A, long long lda, int n, const float2 *Dinv, float2 *X, int nrhs)
+void lu_solve_inv(hipStream_t s, const float2 *A, long long lda, int n, const float2 *Dinv, float2 *X, int nrhs,
+                  TrsvSync &ts)
 {
     switch (nrhs) {
-    case 1: lu_solve_inv_n<1>(s, A, lda, n, Dinv, X); break;
-    case 2: lu_solve_inv_n<2>(s, A, lda, n, Dinv, X); break;
-    case 3: lu_solve_inv_n<3>(s, A, lda, n, Dinv, X); break;
-    case 4: lu_solve_inv_n<4>(s, A, lda, n, Dinv, X); break;
-    case 5: lu_solve_inv_n<5>(s, A, lda, n, Dinv, X); break;
-    case 6: lu_solve_inv_n<6>(s, A, lda, n, Dinv, X); break;
-    case 7: lu_solve_inv_n<7>(s, A, lda, n, Dinv, X); break;
-    case 8: lu_solve_inv_n<8>(s, A, lda, n, Dinv, X); break;
+    case 1: lu_solve_inv_n<1>(s, A, lda, n, Dinv, X, ts); break;
+    case 2: lu_solve_inv_n<2>(s, A, lda, n, Dinv, X, ts); break;
+    case 3: lu_solve_inv_n<3>(s, A, lda, n, Dinv, X, ts); break;
+    case 4: lu_solve_inv_n<4>(s, A, lda, n, Dinv, X, ts); break;
+    case 5: lu_solve_inv_n<5>(s, A, lda, n, Dinv, X, ts); break;
+    case 6: lu_solve_inv_n<6>(s, A, lda, n, Dinv, X, ts); break;
+    case 7: lu_solve_inv_n<7>(s, A, lda, n, Dinv, X, ts); break;
+    case 8: lu_solve_inv_n<8>(s, A, lda, n, Dinv, X, ts); break;
     default: BH_FAIL(BH_ERR_INVALID, "nrhs %d out of range", nrhs);
     }
 }
@@ -1991,7 +2162,7 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
         lu_factor<float>(ctx, (float2 *)M->lu32.p, lda, (int)n, M->ipiv.p, &info, precision == BH_FP32_SPLIT);
         if (info == 0) {
             const size_t nb = (size_t)(n + kIB - 1) / kIB;
-            M->dinv32.alloc(2 * nb * kIB * kIB * 2);
+            M->dinv32.alloc(4 * nb * kIB * kIB * 2);   // L and U diagonal inverses, then their couplings
             invert_diagonal_blocks(ctx.stream, (const float2 *)M->lu32.p, lda, (int)n, (float2 *)M->dinv32.p);
         }
     }
@@ -2035,8 +2206,16 @@ void apply_factors(bh_matrix *M, const double2 *B, double2 *Xd, int nrhs)
         k_permute_in<float><<<g, 256, 0, s>>>(B, n, nrhs, M->perm.p, x32);
         check("k_permute_in");
         {
+            const int nb = (n + kIB - 1) / kIB;
+            if (M->tsync.n < (size_t)nb + 2) {
+                M->tsync.alloc((size_t)nb + 2);
+                BH_CHECK_HIP(hipMemsetAsync(M->tsync.p, 0, ((size_t)nb + 2) * sizeof(int), s));
+                M->tsync_tickets = 0;
+                M->tsync_epoch = 0;
+            }
+            TrsvSync ts{M->tsync.p, M->tsync_tickets, M->tsync_epoch};
             ProfScope prof(*M->ctx, "k_trsv_inv", 8.0 * n * (double)n);   // one read of both FP32 factors
-            lu_solve_inv(s, (const float2 *)M->lu32.p, M->lda, n, (const float2 *)M->dinv32.p, x32, nrhs);
+            lu_solve_inv(s, (const float2 *)M->lu32.p, M->lda, n, (const float2 *)M->dinv32.p, x32, nrhs, ts);
         }
         k_widen<float><<<g, 256, 0, s>>>(x32, cnt, Xd);
         check("k_widen");
@@ -2254,7 +2433,17 @@ extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, doub
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     BH_CHECK_HIP(hipMemcpyAsync(hx.data(), dx.p, sizeof(double2) * cnt, hipMemcpyDeviceToHost, s));
+    int trsv_err = 0;
+    if (M->factored == BH_FP32 && M->tsync.p)
+        BH_CHECK_HIP(hipMemcpyAsync(&trsv_err, M->tsync.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     BH_CHECK_HIP(hipStreamSynchronize(s));
+    if (trsv_err) {   // a triangular-solve wait gave up: results are void, the counters restart
+        BH_CHECK_HIP(hipMemsetAsync(M->tsync.p, 0, M->tsync.n * sizeof(int), s));
+        BH_CHECK_HIP(hipStreamSynchronize(s));
+        M->tsync_tickets = 0;
+        M->tsync_epoch = 0;
+        BH_FAIL(BH_ERR_HIP, "FP32 triangular solve: a block wait exceeded its limit");
+    }
     for (int r = 0; r < q; ++r)
         for (int i = 0; i < n; ++i) {
             X[2 * ((size_t)r * n + i)] = hx[(size_t)i * q + r].x;

@@ -123,3 +123,23 @@ def test_farfield_golden_gpu(bj, golden):
     u = M.solve(b, tol=1e-12)
     a = brute.farfield_magnitudes(bj, u, X, np.linspace(0, np.pi, 7))
     assert np.abs(a - np.array(brute.FARFIELD_GOLDEN)).max() < 1e-3
+
+
+def test_fp32split_deep_blocks_match_fp64(bj):
+    """N >= 32768 takes 1024-deep FP32 blocks, each factored as two 512 halves whose
+    interchanges, block row and split-bf16 update join them: the mixed solution must
+    still meet the north-star bound against the plain FP64 path."""
+    mesh = bj.meshsphere(1.0, nv=11000)
+    X = bj.raviartthomas(mesh)
+    assert X.numfunctions() >= 32768
+    k = 2.0
+    t = bj.Maxwell3D.singlelayer(wavenumber=k)
+    E = bj.Maxwell3D.planewave(direction=bj.x_hat, polarization=bj.z_hat, wavenumber=k)
+    b = bj.assemble(bj.cross(bj.cross(bj.n, E), bj.n), X)
+    M = bj.DeviceMatrix.assemble(t, X).factor("fp32split")
+    x32 = M.solve(b, tol=1e-11, maxit=10)
+    assert M.stats["residual"] <= 1e-11
+    del M
+    M = bj.DeviceMatrix.assemble(t, X).factor("fp64")
+    x64 = M.solve(b)
+    assert relfro(x32, x64) < 1e-8
