@@ -114,6 +114,131 @@ __global__ void k_class_lists(const int4 *__restrict__ entries, int n, int *__re
     lists[(size_t)c * cap + slot] = e;
 }
 
+// Grid-restricted classification.  A pair without touching vertices is near
+// only if dmin^2 < 0.64 area_s (max(dmin^2 |gamma|^2, dmin^2 / (16 area_s)) <
+// 0.2^2 needs the second term below it), and touching needs dmin < dtol; so a
+// pair whose centroids are at least D = sqrt(0.64 max area_s) + rho_t + rho_s
+// apart (rho: the largest vertex-to-centroid distance of either mesh, D widened
+// by a relative margin far above rounding) is far.  Trial cells are binned by
+// centroid into a uniform grid of cell size >= D; a test cell classifies only
+// the trial cells of the 27 grid cells around its own, with the same
+// classify_pair as the full scan, and writes its non-far pairs in ascending
+// trial order, so the compacted list is the full scan's exactly.
+struct PairGridView {
+    const int *start;   // [ncells + 1] into cells
+    const int *cells;   // trial cell ids, ascending within each grid cell
+    int nx, ny, nz;
+    double ox, oy, oz, inv_h;
+};
+constexpr int kGridFillCap = 1024;   // non-far pairs of one test cell held in LDS
+
+// The <= 27 neighbouring grid cells of centroid c: their ranges in cells[] and
+// prefix counts (thread 0 fills, the caller synchronises).
+__device__ void grid_ranges(const PairGridView &g, const double *c, int *b0, int *pre)
+{
+    const int ix = min(max((int)floor((c[0] - g.ox) * g.inv_h), 0), g.nx - 1);
+    const int iy = min(max((int)floor((c[1] - g.oy) * g.inv_h), 0), g.ny - 1);
+    const int iz = min(max((int)floor((c[2] - g.oz) * g.inv_h), 0), g.nz - 1);
+    int r = 0, tot = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = ix + dx, y = iy + dy, z = iz + dz;
+                int len = 0, b = 0;
+                if (x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz) {
+                    const int cell = (z * g.ny + y) * g.nx + x;
+                    b = g.start[cell];
+                    len = g.start[cell + 1] - b;
+                }
+                b0[r] = b;
+                pre[r] = tot;
+                tot += len;
+                ++r;
+            }
+    pre[27] = tot;
+}
+
+__device__ __forceinline__ int grid_candidate(const PairGridView &g, const int *b0, const int *pre, int idx)
+{
+    int r = 0;
+    while (pre[r + 1] <= idx) ++r;
+    return g.cells[b0[r] + idx - pre[r]];
+}
+
+__global__ __launch_bounds__(256) void k_classify_count_grid(const double *__restrict__ tgeo, int nft,
+                                                             const double *__restrict__ sgeo, PairGridView g,
+                                                             double k2, int mode, int *__restrict__ counts,
+                                                             int *__restrict__ err)
+{
+    __shared__ int wsum[4], b0[27], pre[28];
+    const int p = blockIdx.x;
+    double tv[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tv[k] = tgeo[(size_t)p * kGeo + k];
+    if (threadIdx.x == 0) grid_ranges(g, tgeo + (size_t)p * kGeo + kGeoC, b0, pre);
+    __syncthreads();
+    int cnt = 0, bad = 0;
+    for (int idx = threadIdx.x; idx < pre[27]; idx += blockDim.x) {
+        const int q = grid_candidate(g, b0, pre, idx);
+        const double *sp = sgeo + (size_t)q * kGeo;
+        double sv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sv[k] = sp[k];
+        int pk;
+        const int c = classify_pair(tv, sv, sp[kGeoA], k2, mode, &pk);
+        if (c < 0) bad = 1;
+        else cnt += (c != kFar);
+    }
+    if (bad) atomicOr(err, 1);
+    for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[p] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(256) void k_classify_fill_grid(const double *__restrict__ tgeo, int nft,
+                                                            const double *__restrict__ sgeo, PairGridView g,
+                                                            double k2, int mode, const int *__restrict__ offsets,
+                                                            int4 *__restrict__ entries)
+{
+    __shared__ int b0[27], pre[28], s_n;
+    __shared__ int s_q[kGridFillCap], s_c[kGridFillCap], s_pk[kGridFillCap];
+    const int p = blockIdx.x;
+    double tv[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tv[k] = tgeo[(size_t)p * kGeo + k];
+    if (threadIdx.x == 0) {
+        grid_ranges(g, tgeo + (size_t)p * kGeo + kGeoC, b0, pre);
+        s_n = 0;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < pre[27]; idx += blockDim.x) {
+        const int q = grid_candidate(g, b0, pre, idx);
+        const double *sp = sgeo + (size_t)q * kGeo;
+        double sv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sv[k] = sp[k];
+        int pk;
+        const int c = classify_pair(tv, sv, sp[kGeoA], k2, mode, &pk);
+        if (c > kFar) {   // c < 0 is reported by the count pass
+            const int slot = atomicAdd(&s_n, 1);
+            if (slot < kGridFillCap) {
+                s_q[slot] = q;
+                s_c[slot] = c;
+                s_pk[slot] = pk;
+            }
+        }
+    }
+    __syncthreads();
+    const int m = min(s_n, kGridFillCap), base = offsets[p];   // s_n <= kGridFillCap: the host checks the counts
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+        const int q = s_q[e];
+        int rank = 0;
+        for (int f = 0; f < m; ++f) rank += s_q[f] < q;
+        entries[base + rank] = make_int4(p, q, s_c[e], s_pk[e]);
+    }
+}
+
 // Full class matrix (debug / bit-exact test entry).
 __global__ void k_classify_matrix(const double *__restrict__ tgeo, int nft, const double *__restrict__ sgeo, int nfs,
                                   double k2, int mode, int8_t *__restrict__ out)
@@ -935,6 +1060,67 @@ void check_launch(const char *what)
     if (e != hipSuccess) BH_FAIL(BH_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
 }
 
+// Trial-cell grid for the restricted classification (see k_classify_count_grid);
+// empty (use == false) when the meshes are too small for it to pay.
+struct PairGrid {
+    bool use = false;
+    DevBuf<int> start, cells;
+    PairGridView view{};
+};
+
+void build_pair_grid(const Mesh &tm, const Mesh &sm, int mode, hipStream_t s, PairGrid &pg)
+{
+    const int nft = (int)tm.nf, nfs = (int)sm.nf;
+    if ((long long)nft * nfs < (1LL << 22)) return;
+    auto rho_of = [](const Mesh &m) {
+        double r = 0;
+        for (int64_t f = 0; f < m.nf; ++f) {
+            const double *g = m.geo.data() + (size_t)f * kGeo;
+            for (int i = 0; i < 3; ++i) {
+                const double dx = g[3 * i] - g[kGeoC], dy = g[3 * i + 1] - g[kGeoC + 1], dz = g[3 * i + 2] - g[kGeoC + 2];
+                r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+            }
+        }
+        return r;
+    };
+    double amax = 0, lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int q = 0; q < nfs; ++q) {
+        const double *g = sm.geo.data() + (size_t)q * kGeo;
+        amax = std::max(amax, g[kGeoA]);
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], g[kGeoC + d]);
+            hi[d] = std::max(hi[d], g[kGeoC + d]);
+        }
+    }
+    const double dtol = 1.0e3 * 2.220446049250313e-16;
+    const double reach = std::max(mode == kModeWiltonSauter ? std::sqrt(0.64 * amax) : 0.0, dtol);
+    double h = (reach + rho_of(tm) + rho_of(sm)) * (1.0 + 1e-6);
+    if (!(h > 0) || !std::isfinite(h)) return;
+    const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+    h = std::max(h, ext / 128.0);   // at most 128^3 grid cells
+    int nd[3];
+    for (int d = 0; d < 3; ++d) nd[d] = std::max(1, (int)std::floor((hi[d] - lo[d]) / h) + 1);
+    const long long ncell = (long long)nd[0] * nd[1] * nd[2];
+    std::vector<int> start(ncell + 1, 0), cells(nfs), key(nfs);
+    const double inv_h = 1.0 / h;
+    for (int q = 0; q < nfs; ++q) {
+        const double *c = sm.geo.data() + (size_t)q * kGeo + kGeoC;
+        int ix[3];
+        for (int d = 0; d < 3; ++d) ix[d] = std::min(std::max((int)std::floor((c[d] - lo[d]) * inv_h), 0), nd[d] - 1);
+        key[q] = (ix[2] * nd[1] + ix[1]) * nd[0] + ix[0];
+        ++start[key[q] + 1];
+    }
+    for (long long i = 0; i < ncell; ++i) start[i + 1] += start[i];
+    std::vector<int> fillp(start.begin(), start.end() - 1);
+    for (int q = 0; q < nfs; ++q) cells[fillp[key[q]]++] = q;   // ascending q within a grid cell
+    pg.start.alloc(ncell + 1);
+    pg.cells.alloc(nfs);
+    pg.start.upload(start.data(), start.size(), s);
+    pg.cells.upload(cells.data(), cells.size(), s);
+    pg.view = PairGridView{pg.start.p, pg.cells.p, nd[0], nd[1], nd[2], lo[0], lo[1], lo[2], inv_h};
+    pg.use = true;
+}
+
 // K0 + K2 + K3: classification, compaction and the non-far blocks.
 void build_near(Ctx &ctx, Mesh &tm, Mesh &sm, const Strategy &st, const OpParams &op, NearData &nd)
 {
@@ -950,8 +1136,13 @@ void build_near(Ctx &ctx, Mesh &tm, Mesh &sm, const Strategy &st, const OpParams
     const double k2 = op.gr * op.gr + op.gi * op.gi;
     DevBuf<int> counts(nft), err(1);
     BH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    PairGrid pg;
+    build_pair_grid(tm, sm, st.mode, s, pg);
     ProfScope prof_c(ctx, "k_classify_count", (double)nft * nfs);
-    k_classify_count<<<nft, 256, 0, s>>>(tm.d_geo.p, nft, sm.d_geo.p, nfs, k2, st.mode, counts.p, err.p);
+    if (pg.use)
+        k_classify_count_grid<<<nft, 256, 0, s>>>(tm.d_geo.p, nft, sm.d_geo.p, pg.view, k2, st.mode, counts.p, err.p);
+    else
+        k_classify_count<<<nft, 256, 0, s>>>(tm.d_geo.p, nft, sm.d_geo.p, nfs, k2, st.mode, counts.p, err.p);
     check_launch("k_classify_count");
     std::vector<int> hc(nft);
     int herr = 0;
@@ -959,12 +1150,20 @@ void build_near(Ctx &ctx, Mesh &tm, Mesh &sm, const Strategy &st, const OpParams
     BH_CHECK_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
     BH_CHECK_HIP(hipStreamSynchronize(s));
     if (herr) BH_FAIL(BH_ERR_ASSERT, "AssertionError: hits <= 3 (two cells share more than 3 vertex positions)");
-    for (int p = 0; p < nft; ++p) off[p + 1] = off[p] + hc[p];
+    int hmax = 0;
+    for (int p = 0; p < nft; ++p) {
+        off[p + 1] = off[p] + hc[p];
+        hmax = std::max(hmax, hc[p]);
+    }
     nd.nent = off[nft];
     nd.nptr.upload(off.data(), off.size(), s);
     if (nd.nent == 0) return;
     nd.entries.alloc(nd.nent);
-    k_classify_fill<<<nft, 256, 0, s>>>(tm.d_geo.p, nft, sm.d_geo.p, nfs, k2, st.mode, nd.nptr.p, nd.entries.p);
+    if (pg.use && hmax <= kGridFillCap)
+        k_classify_fill_grid<<<nft, 256, 0, s>>>(tm.d_geo.p, nft, sm.d_geo.p, pg.view, k2, st.mode, nd.nptr.p,
+                                                  nd.entries.p);
+    else
+        k_classify_fill<<<nft, 256, 0, s>>>(tm.d_geo.p, nft, sm.d_geo.p, nfs, k2, st.mode, nd.nptr.p, nd.entries.p);
     check_launch("k_classify_fill");
     DevBuf<int> counters(5), lists((size_t)5 * nd.nent);
     BH_CHECK_HIP(hipMemsetAsync(counters.p, 0, 5 * sizeof(int), s));

@@ -224,3 +224,23 @@ def test_mfie_resident_combination(bj, orc, golden):
     M.factor("fp32split")
     u = M.solve(b, tol=1e-12)
     assert np.linalg.norm(Zref @ u - b) / np.linalg.norm(b) < 1e-11
+
+
+@pytest.mark.parametrize("k", [1.0, 6.0])
+def test_grid_classification_large_meshes(bj, orc, k):
+    """Meshes with more than 2^22 cell pairs classify through the trial-cell grid;
+    the matrix must equal the oracle's (which classifies every pair), also for two
+    distinct, nearly touching meshes."""
+    s1 = bj.meshsphere(1.0, nv=1100)
+    assert s1.numcells ** 2 >= (1 << 22)
+    X = bj.raviartthomas(s1)
+    t = bj.Maxwell3D.singlelayer(wavenumber=k)
+    qs = bj.defaultquadstrat(t, X, X)
+    rows = np.arange(0, X.numfunctions(), 11)
+    Xs = X.subset(rows)
+    assert relfro(bj.assemble(t, Xs, X), _oracle_Z(orc, t, qs, Xs, X)) < 1e-12
+    s2 = bj.Mesh(s1.vertices * 0.985, s1.cells)   # a second sphere just inside the first
+    Y = bj.raviartthomas(s2)
+    Ys = Y.subset(np.arange(0, Y.numfunctions(), 13))
+    for op in _ops(bj, k):
+        assert relfro(bj.assemble(op, Xs, Ys), _oracle_Z(orc, op, qs, Xs, Ys)) < 1e-12
