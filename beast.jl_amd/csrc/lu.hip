@@ -696,12 +696,18 @@ __global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, in
 // (slot len + i for the first i naming that row).  plan[s] = the row whose
 // content ends at slot s, plan[2 len + s] = the slot's row (-1: slot unused).
 constexpr int kMaxSwaps = 256;
-// FP32 LU block depth (trailing-update depth): large matrices take 1024, whose
-// halves of kNBInner are factored one after the other on the block-column
-// stream; that halves the C traffic of the trailing updates, which pays once
-// the trailing matrix is large enough to hide the longer block-column chain.
-constexpr int kNBFP32 = 1024, kNBFP32Small = 512, kNBDeepFrom = 32768;
+// FP32 LU block depth (trailing-update depth): while the trailing matrix is
+// large the blocks are 1024 deep, factored as two halves of kNBInner one after
+// the other on the block-column stream; that halves the C traffic of the
+// trailing updates, which pays while the trailing update hides the longer
+// block-column chain.  Smaller trailing matrices take 512-deep blocks.
+constexpr int kNBFP32 = 1024, kNBFP32Small = 512, kNBDeepAbove = 24576;
 constexpr int kNBInner = 512;
+
+inline int fp32_block_depth(int n, int k)
+{
+    return n - k > kNBDeepAbove ? kNBFP32 : kNBFP32Small;
+}
 
 __global__ __launch_bounds__(256) void k_swap_plan(const long long *ipiv, int k1, int k2, int *plan)
 {
@@ -803,6 +809,92 @@ __global__ __launch_bounds__(64) void k_trsm_unit(typename CT<T>::t *A, long lon
 #pragma unroll
         for (int i = 0; i < kPW; ++i)
             if (i < m) A[(long long)(br + i) * lda + bc + col] = x[i];
+    }
+}
+
+// X := L^-1 X on the rows [k, k + kb) (kb <= kNBInner) of the columns
+// [c0, c0 + ncols), L the unit lower triangle of A[k:k+kb, k:k+kb): the FP32
+// block row in one launch.  Workgroup g keeps columns [c0 + 16 g, + 16) of the
+// rows in LDS and walks the 32-row blocks p: threads 0..15 solve block p with the
+// diagonal tile of L (their column in registers), then every thread holds
+// column c = t % 16 of the solved block in registers and updates its rows
+// below (rows 32 (p + 1) + t / 16 + 16 m), reading the rows of L from L2.
+constexpr int kStripW = 16;
+constexpr int kNBInnerMax = 512;
+constexpr int kBlockRowOneLaunch = 12288;   // widest block row solved by k_blockrow_trsm
+
+__global__ __launch_bounds__(256) void k_blockrow_trsm(float2 *A, long long lda, int k, int kb, int c0, int ncols)
+{
+    __shared__ float2 X[kNBInnerMax][kStripW];
+    __shared__ float2 Lt[kPW][kPW + 1];
+    const int tid = threadIdx.x, c = tid % kStripW, g = tid / kStripW;
+    const int col0 = c0 + (int)blockIdx.x * kStripW;
+    const int w = min(kStripW, c0 + ncols - col0);
+    for (int idx = tid; idx < kb * kStripW; idx += 256) {
+        const int r = idx / kStripW, cc = idx % kStripW;
+        X[r][cc] = cc < w ? A[(long long)(k + r) * lda + col0 + cc] : float2{0, 0};
+    }
+    const int np = (kb + kPW - 1) / kPW;
+    for (int p = 0; p < np; ++p) {
+        const int r0 = p * kPW, pr = min(kPW, kb - r0);
+        for (int idx = tid; idx < kPW * kPW; idx += 256) {
+            const int i = idx / kPW, j = idx % kPW;
+            Lt[i][j] = (i < pr && j < pr) ? A[(long long)(k + r0 + i) * lda + k + r0 + j] : float2{0, 0};
+        }
+        __syncthreads();
+        if (tid < kStripW) {
+            float2 x[kPW];
+#pragma unroll
+            for (int i = 0; i < kPW; ++i) x[i] = X[r0 + i < kb ? r0 + i : 0][tid];
+#pragma unroll
+            for (int i = 1; i < kPW; ++i) {
+                if (i < pr) {   // uniform
+                    float2 acc = x[i];
+#pragma unroll
+                    for (int j = 0; j < i; ++j) acc = csubv(acc, cmulv(Lt[i][j], x[j]));
+                    x[i] = acc;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kPW; ++i)
+                if (i < pr) X[r0 + i][tid] = x[i];
+        }
+        __syncthreads();
+        if (r0 + pr < kb) {
+            float2 xp[kPW];   // column c of the solved block
+#pragma unroll
+            for (int j = 0; j < kPW; ++j) xp[j] = j < pr ? X[r0 + j][c] : float2{0, 0};
+            // the rows below in tiles of 32: the tile of L through LDS (coalesced),
+            // thread (c, g) updates rows g and g + 16 of the tile
+            for (int q0 = r0 + pr; q0 < kb; q0 += kPW) {
+                const int qr = min(kPW, kb - q0);
+                __syncthreads();
+                for (int idx = tid; idx < kPW * kPW; idx += 256) {
+                    const int i = idx / kPW, j = idx % kPW;
+                    Lt[i][j] = (i < qr && j < pr) ? A[(long long)(k + q0 + i) * lda + k + r0 + j] : float2{0, 0};
+                }
+                __syncthreads();
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int i = g + 16 * h;
+                    if (i < qr) {
+                        float2 acc = X[q0 + i][c];
+#pragma unroll
+                        for (int j = 0; j < kPW; ++j) {
+                            const float2 l = Lt[i][j];
+                            acc.x = fmaf(-l.x, xp[j].x, fmaf(l.y, xp[j].y, acc.x));
+                            acc.y = fmaf(-l.x, xp[j].y, fmaf(-l.y, xp[j].x, acc.y));
+                        }
+                        X[q0 + i][c] = acc;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < kb * kStripW; idx += 256) {
+        const int r = idx / kStripW, cc = idx % kStripW;
+        if (cc < w) A[(long long)(k + r) * lda + col0 + cc] = X[r][cc];
     }
 }
 
@@ -1933,6 +2025,17 @@ void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, 
             return;
         }
     }
+    if constexpr (sizeof(T) == 4) {
+        // narrow block rows in one launch (latency-bound as separate 32-row steps);
+        // wide ones keep the MFMA updates below
+        const int ncols = c1 - c0;
+        if (kb <= kNBInnerMax && ncols <= kBlockRowOneLaunch) {
+            ProfScope prof(ctx, "k_blockrow_trsm", 4.0 * kb * (double)kb * ncols, s);   // kb^2 / 2 complex MACs per column
+            k_blockrow_trsm<<<(ncols + kStripW - 1) / kStripW, 256, 0, s>>>(A, lda, k, kb, c0, ncols);
+            check("k_blockrow_trsm");
+            return;
+        }
+    }
     for (int sp = 0; sp < kb; sp += kPW) {
         const int pw = std::min(kPW, kb - sp);
         trsm_unit<T>(s, A, lda, k + sp, pw, k + sp, c0, c1 - c0);
@@ -1973,7 +2076,7 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
     BH_CHECK_HIP(hipMemsetAsync(W.info.p, 0, sizeof(int), s));
     // block column of the right-looking LU (trailing-update depth); the FP32
     // path's deeper blocks halve the C traffic per flop of the trailing update
-    const int NB = sizeof(T) == 4 ? (n >= kNBDeepFrom ? kNBFP32 : kNBFP32Small) : 256;
+    const int NB = sizeof(T) == 4 ? kNBFP32 : 256;   // deepest block (workspace sizes)
     int planes_of = -1;   // block whose L21 planes PA holds
     auto trailing = [&](hipStream_t st, int k, int kb, int c0, int c1) {
         // A[k+kb:, c0:c1) -= L21 U12[:, c0:c1)
@@ -2023,16 +2126,16 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
         } cleanup{s2, ready, done};
         BH_CHECK_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
         BH_CHECK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
-        factor_block<T>(ctx, s, W, A, lda, n, 0, std::min(NB, n), ipiv);
-        for (int k = 0; k < n; k += NB) {
-            const int kb = std::min(NB, n - k);
+        factor_block<T>(ctx, s, W, A, lda, n, 0, std::min(fp32_block_depth(n, 0), n), ipiv);
+        for (int k = 0, kb = 0; k < n; k += kb) {
+            kb = std::min(fp32_block_depth(n, k), n - k);
             // block k is factored; its interchanges reach the other columns (in
             // pieces of at most kMaxSwaps interchanges, applied in order)
             for (int q = k; q < k + kb; q += kMaxSwaps)
                 laswp<T>(ctx, s, A, lda, q, std::min(q + kMaxSwaps, k + kb), ipiv, 0, k, k + kb, n, W.plan.p);
             if (k + kb >= n) break;
             block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, k + kb, n, &PB);
-            const int nk = k + kb, nkb = std::min(NB, n - nk);
+            const int nk = k + kb, nkb = std::min(fp32_block_depth(n, nk), n - nk);
             trailing(s, k, kb, nk, nk + nkb);   // the next block column first
             BH_CHECK_HIP(hipEventRecord(ready, s));
             BH_CHECK_HIP(hipStreamWaitEvent(s2, ready, 0));
