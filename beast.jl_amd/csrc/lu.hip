@@ -821,7 +821,7 @@ __global__ __launch_bounds__(64) void k_trsm_unit(typename CT<T>::t *A, long lon
 // below (rows 32 (p + 1) + t / 16 + 16 m), reading the rows of L from L2.
 constexpr int kStripW = 16;
 constexpr int kNBInnerMax = 512;
-constexpr int kBlockRowOneLaunch = 12288;   // widest block row solved by k_blockrow_trsm
+constexpr int kBlockRowOneLaunch = 8192;   // widest block row solved by k_blockrow_trsm
 
 __global__ __launch_bounds__(256) void k_blockrow_trsm(float2 *A, long long lda, int k, int kb, int c0, int ncols)
 {
