@@ -73,6 +73,16 @@ def random_rhs(rng, count):
     return out
 
 
+def served_kind(shape, kind):
+    """The formulation a request is served with.  The MFIE here is tested with RT
+    functions (K + N/2, N = NCross on RT x RT); on flat faces K vanishes for
+    coplanar pairs and the RT x (n x RT) pairing left is nearly singular (condition
+    numbers of 1e11-1e12 on the cube against 4e4 on a sphere of the same size; the
+    reference's examples/mfie.jl tests with Buffa-Christiansen functions instead),
+    so flat-faced geometries are served with the EFIE."""
+    return "efie" if (kind == "mfie" and shape == "cube") else kind
+
+
 def batched_requests(seed=4, count=64, nmin=500, nmax=8000, kinds=("efie",)):
     """The batched workload: all requests at t = 0, N log-uniform, k ~ U[0.5, 8], 1-4 right-hand sides."""
     rng = np.random.default_rng(seed)
@@ -81,8 +91,8 @@ def batched_requests(seed=4, count=64, nmin=500, nmax=8000, kinds=("efie",)):
         shape = ["sphere", "ellipsoid", "cube"][rng.integers(3)]
         dofs = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
         k = float(rng.uniform(0.5, 8.0))
-        reqs.append(Request(i, 0.0, shape, dofs, kinds[rng.integers(len(kinds))], k,
-                            random_rhs(rng, int(rng.integers(1, 5)))))
+        kind = served_kind(shape, kinds[rng.integers(len(kinds))])
+        reqs.append(Request(i, 0.0, shape, dofs, kind, k, random_rhs(rng, int(rng.integers(1, 5)))))
     return reqs
 
 
@@ -97,7 +107,7 @@ def poisson_requests(seed=5, rate=20.0, count=200, nmin=500, nmax=30000, kinds=(
         else:
             shape = ["sphere", "ellipsoid", "cube"][rng.integers(3)]
             dofs = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
-            kind = kinds[rng.integers(len(kinds))]
+            kind = served_kind(shape, kinds[rng.integers(len(kinds))])
             k = float(rng.uniform(0.5, 8.0))
             seen.append((shape, dofs, kind, k))
         reqs.append(Request(i, t, shape, dofs, kind, k, random_rhs(rng, int(rng.integers(1, 5)))))

@@ -216,3 +216,25 @@ def test_ncross_oracle_closed_form(bj, orc, golden):
     assert np.abs(Z6 + Z6.T).max() < 1e-14 * np.abs(Zc).max()
     Zs = orc.ncross(X, X, degree=6, scale=0.5j)
     assert np.abs(Zs - 0.5j * Z6).max() < 1e-16
+
+
+def test_rt_tested_mfie_needs_smooth_geometry(bj, orc):
+    """K + N/2 tested with RT is nearly singular on flat faces (K vanishes on coplanar
+    pairs), well conditioned on a sphere: the serving workloads therefore serve cubes
+    with the EFIE and keep the MFIE for smooth shapes."""
+    from importlib import import_module
+    sv = import_module("beast_jl_amd.serve")
+
+    def cond(shape):
+        X = bj.raviartthomas(sv.geometry(shape, 300))
+        K = bj.Maxwell3D.doublelayer(wavenumber=1.0)
+        qs = bj.defaultquadstrat(K, X, X)
+        Z = orc.assemble(1, K.gamma, K.alpha, 0.0, qs.kind, qs.params, X, X) + 0.5 * orc.ncross(X, X, 6, 1.0)
+        s = np.linalg.svd(Z, compute_uv=False)
+        return s[0] / s[-1]
+
+    assert cond("cube") > 1e9
+    assert cond("sphere") < 1e6
+    for reqs in (sv.poisson_requests(seed=5, count=200), sv.batched_requests(seed=4, count=64, kinds=("efie", "mfie"))):
+        assert not any(r.shape == "cube" and r.kind == "mfie" for r in reqs)
+        assert any(r.kind == "mfie" for r in reqs)
