@@ -11,14 +11,18 @@
 //                   its rows in LDS; per column one grid barrier exchanges the
 //                   per-workgroup pivot candidates (value, row, row contents).
 //                   Pivot choice is LAPACK's: largest |Re|+|Im|, first row on ties.
-// FP32 factors (block depth 512) pivot each 32-column panel by tournament
+// FP32 factors (block depth 1024 while the trailing matrix is large, factored as
+// two 512-column halves, then 512) pivot each 32-column panel by tournament
 // (communication-avoiding LU, Grigori, Demmel & Xiang, SIAM J. Matrix Anal. Appl.
 // 32 (2011)):
-//   k_tslu_stage    stage 0: 256 rows per workgroup, one per thread in registers,
-//                   reduced to 32 candidates by partial pivoting; later stages merge
-//                   8 candidate sets; the last emits L11\U11, ipiv and a row plan.
-//   k_tslu_apply    moves the planned rows in the block's other columns and turns
-//                   the panel's rows into L21 = A21 U11^-1.
+//   k_tslu_panel    one launch per panel: 256 rows per workgroup, one per thread
+//                   in registers, reduced to 32 candidates by partial pivoting;
+//                   the last workgroup of each group of 8 to finish merges the
+//                   group one level up, and the last merge emits L11\U11, ipiv and
+//                   a row plan.  Further workgroups carry the previous panel's
+//                   update of the block.
+//   k_tslu_apply    moves the planned rows in the block's other columns, solves the
+//                   panel's block row and turns the panel rows into L21 = A21 U11^-1.
 //   The next block column is factored on a second stream while the trailing
 //   update of the current one runs (look-ahead by one block).
 // Shared by both:
@@ -30,8 +34,9 @@
 //   k_cgemm_bf16x3  FP32 trailing update from split-bf16 planes (hi*hi + hi*lo +
 //                   lo*hi on v_mfma_f32_16x16x32_bf16, FP32 accumulation).
 // Solves: permutation gather, then for FP64 factors blocked substitution
-// (k_trsv_step); for FP32 factors products with the inverted 128x128 diagonal
-// blocks (k_trinv, k_trsv_flag).  FP32 factors precondition flexible GMRES
+// (k_trsv_step); for FP32 factors one launch per triangle with the inverted
+// 128x128 diagonal blocks and their couplings (k_trinv, k_blk_coupling,
+// k_trsv_flag).  FP32 factors precondition flexible GMRES
 // inside iterative refinement in FP64 (GMRES-IR, Carson & Higham, SIAM J. Sci.
 // Comput. 40 (2018)), whose residuals use the resident FP64 matrix.
 #include <hip/hip_cooperative_groups.h>
@@ -239,10 +244,10 @@ __global__ __launch_bounds__(256) void k_panel(typename CT<T>::t *A, long long l
 // Grigori, Demmel & Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)), used by the
 // FP32 factorisations that GMRES-IR refines.  Partial pivoting needs one
 // grid-wide exchange per column; here the pw <= 32 pivot rows of a panel are chosen
-//   1. per workgroup of 1024 rows: partial pivoting selects <= pw candidates,
-//   2. by a reduction tree: a workgroup stacks up to 32 candidate sets (their
-//      original rows) and keeps the rows partial pivoting selects,
-// until one workgroup remains.  Partial pivoting on the final stack picks its
+//   1. per workgroup of kTsluT rows: partial pivoting selects <= pw candidates,
+//   2. by a reduction tree: a workgroup stacks kTsluFan candidate sets (their
+//      rows as they entered the panel) and keeps the rows partial pivoting selects,
+// until one set remains.  Partial pivoting on the final stack picks its
 // pivots among the winners in order, and eliminating with winners only leaves
 // the winners' rows exactly as partial pivoting on the winners alone would
 // (no loser is ever a pivot row), so the final workgroup also emits
