@@ -710,21 +710,19 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
     const int Qo = QO ? QO : A.Qo, Qi = QI ? QI : A.Qi;
     const double *tg = A.tgeo + (size_t)p * kGeo, *sg = A.sgeo + (size_t)q * kGeo;
     if (A.mode != kModeDoubleNum) {
-        int pk;
-        const int cls = classify_pair(tg, sg, sg[kGeoA], A.k2, A.mode, &pk);
-        if (cls != kFar) {
-            const int lo = A.nptr[p], hi = A.nptr[p + 1];
-            int hit = -1;
-            for (int e = lo; e < hi; ++e)
-                if (A.nent[e].y == q) { hit = e; break; }
-            if (hit < 0) {
+        // the classification pass listed p's non-far trial cells in ascending
+        // order with their blocks: a pair found there is not far
+        const int end = A.nptr[p + 1];
+        int lo = A.nptr[p], hi = end;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (A.nent[mid].y < q) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < end && A.nent[lo].y == q) {
+            const double *blk = A.nblk + (size_t)lo * 18;
 #pragma unroll
-                for (int k = 0; k < 9; ++k) z[k * stride] = {(R)0, (R)0};
-            } else {
-                const double *blk = A.nblk + (size_t)hit * 18;
-#pragma unroll
-                for (int k = 0; k < 9; ++k) z[k * stride] = {(R)blk[2 * k], (R)blk[2 * k + 1]};
-            }
+            for (int k = 0; k < 9; ++k) z[k * stride] = {(R)blk[2 * k], (R)blk[2 * k + 1]};
             return;
         }
     }
