@@ -708,6 +708,7 @@ constexpr int kMaxSwaps = 256;
 // block-column chain.  Smaller trailing matrices take 512-deep blocks.
 constexpr int kNBFP32 = 1024, kNBFP32Small = 512, kNBDeepAbove = 24576;
 constexpr int kNBInner = 512;
+constexpr int kAheadFirstBelow = 20480;   // trailing width below which the next block column goes first
 
 inline int fp32_block_depth(int n, int k)
 {
@@ -1974,7 +1975,7 @@ void gemm_split(Ctx &ctx, hipStream_t s, float2 *A, long long lda, int ar, int a
 // split-bf16 product through PB's planes).
 template <typename T>
 void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, int k, int kb,
-                   int c0, int c1, DevBuf<__bf16> *PB = nullptr);
+                   int c0, int c1, DevBuf<__bf16> *PB = nullptr, bool one_launch = true);
 
 // Block column [k, k + kb): its 32-column panels and the updates inside the block.
 template <typename T>
@@ -2017,24 +2018,26 @@ void factor_block(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, l
 // U12 = L11^-1 A12 for block row [k, k + kb) over the columns [c0, n).
 template <typename T>
 void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, int k, int kb,
-                   int c0, int c1, DevBuf<__bf16> *PB)
+                   int c0, int c1, DevBuf<__bf16> *PB, bool one_launch)
 {
     if (c1 <= c0) return;
     if constexpr (sizeof(T) == 4) {
         if (kb > kNBInner && PB) {
             // top half, its update of the bottom half (split-bf16 products), bottom half
             const int h = kNBInner;
-            block_row_u12<T>(ctx, s, W, A, lda, n, k, h, c0, c1);
+            block_row_u12<T>(ctx, s, W, A, lda, n, k, h, c0, c1, nullptr, one_launch);
             gemm_split(ctx, s, A, lda, k + h, k, k, c0, k + h, c0, kb - h, c1 - c0, h, W.pa_u12, *PB);
-            block_row_u12<T>(ctx, s, W, A, lda, n, k + h, kb - h, c0, c1);
+            block_row_u12<T>(ctx, s, W, A, lda, n, k + h, kb - h, c0, c1, nullptr, one_launch);
             return;
         }
     }
     if constexpr (sizeof(T) == 4) {
         // narrow block rows in one launch (latency-bound as separate 32-row steps);
-        // wide ones keep the MFMA updates below
+        // wide ones, and block rows that run beside the block-column chain (the
+        // one-launch kernel would take the whole device from it), keep the MFMA
+        // updates below
         const int ncols = c1 - c0;
-        if (kb <= kNBInnerMax && ncols <= kBlockRowOneLaunch) {
+        if (one_launch && kb <= kNBInnerMax && ncols <= kBlockRowOneLaunch) {
             ProfScope prof(ctx, "k_blockrow_trsm", 4.0 * kb * (double)kb * ncols, s);   // kb^2 / 2 complex MACs per column
             k_blockrow_trsm<<<(ncols + kStripW - 1) / kStripW, 256, 0, s>>>(A, lda, k, kb, c0, ncols);
             check("k_blockrow_trsm");
@@ -2132,21 +2135,42 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
         BH_CHECK_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
         BH_CHECK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
         factor_block<T>(ctx, s, W, A, lda, n, 0, std::min(fp32_block_depth(n, 0), n), ipiv);
+        // block k is factored; its interchanges reach the other columns (in pieces
+        // of at most kMaxSwaps interchanges, applied in order) over [c0, c1) and [c2, c3)
+        auto swaps = [&](int k, int kb, int c0, int c1, int c2, int c3) {
+            for (int q = k; q < k + kb; q += kMaxSwaps)
+                laswp<T>(ctx, s, A, lda, q, std::min(q + kMaxSwaps, k + kb), ipiv, c0, c1, c2, c3, W.plan.p);
+        };
         for (int k = 0, kb = 0; k < n; k += kb) {
             kb = std::min(fp32_block_depth(n, k), n - k);
-            // block k is factored; its interchanges reach the other columns (in
-            // pieces of at most kMaxSwaps interchanges, applied in order)
-            for (int q = k; q < k + kb; q += kMaxSwaps)
-                laswp<T>(ctx, s, A, lda, q, std::min(q + kMaxSwaps, k + kb), ipiv, 0, k, k + kb, n, W.plan.p);
-            if (k + kb >= n) break;
-            block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, k + kb, n, &PB);
+            if (k + kb >= n) {
+                swaps(k, kb, 0, k, 0, 0);
+                break;
+            }
             const int nk = k + kb, nkb = std::min(fp32_block_depth(n, nk), n - nk);
-            trailing(s, k, kb, nk, nk + nkb);   // the next block column first
+            // near the end the block-column chain is the critical path: there the
+            // next block column goes first on its own (interchanges, block row,
+            // update) and the rest is brought up to date while it is factored;
+            // earlier, one pass over all columns costs the context stream less
+            const bool ahead_first = n - nk <= kAheadFirstBelow;
+            if (ahead_first) {
+                swaps(k, kb, nk, nk + nkb, 0, 0);
+                block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, nk, nk + nkb, &PB);
+            } else {
+                swaps(k, kb, 0, k, nk, n);
+                block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, nk, n, &PB);
+            }
+            trailing(s, k, kb, nk, nk + nkb);
             BH_CHECK_HIP(hipEventRecord(ready, s));
             BH_CHECK_HIP(hipStreamWaitEvent(s2, ready, 0));
             factor_block<T>(ctx, s2, W, A, lda, n, nk, nkb, ipiv);
             BH_CHECK_HIP(hipEventRecord(done, s2));
-            trailing(s, k, kb, nk + nkb, n);    // the rest, concurrently
+            // the rest, concurrently
+            if (ahead_first) {
+                swaps(k, kb, 0, k, nk + nkb, n);
+                block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, nk + nkb, n, &PB, false);
+            }
+            trailing(s, k, kb, nk + nkb, n);
             BH_CHECK_HIP(hipStreamWaitEvent(s, done, 0));
         }
     }
