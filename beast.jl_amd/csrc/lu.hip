@@ -1691,17 +1691,26 @@ __global__ __launch_bounds__(256) void k_residual(const double2 *__restrict__ A,
     double2 acc[kMaxRhs];
 #pragma unroll
     for (int q = 0; q < kMaxRhs; ++q) acc[q] = {0, 0};
-    for (int c = lane; c < n; c += 64) {
-        const double2 av = a[c];
+    auto step = [&](const double2 av, int cc) {
 #pragma unroll
         for (int q = 0; q < kMaxRhs; ++q) {
             if (q < nrhs) {
-                const double2 xv = x[(size_t)c * nrhs + q];
+                const double2 xv = x[(size_t)cc * nrhs + q];
                 acc[q].x += av.x * xv.x - av.y * xv.y;
                 acc[q].y += av.x * xv.y + av.y * xv.x;
             }
         }
+    };
+    int c = lane;
+    // four row loads in flight per lane, summed in the same order as one at a time
+    for (; c + 192 < n; c += 256) {
+        double2 av[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) av[u] = a[c + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) step(av[u], c + 64 * u);
     }
+    for (; c < n; c += 64) step(a[c], c);
 #pragma unroll
     for (int q = 0; q < kMaxRhs; ++q) {
         if (q >= nrhs) break;
@@ -1726,7 +1735,21 @@ __global__ __launch_bounds__(256) void k_matvec(const double2 *__restrict__ A, i
     const int lane = threadIdx.x & 63;
     const double2 *a = A + (size_t)row * n;
     double sx = 0, sy = 0;
-    for (int c = lane; c < n; c += 64) {
+    int c = lane;
+    // four row loads in flight per lane, summed in the same order as one at a time
+    for (; c + 192 < n; c += 256) {
+        double2 av[4], xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) av[u] = a[c + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = x[c + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            sx += av[u].x * xv[u].x - av[u].y * xv[u].y;
+            sy += av[u].x * xv[u].y + av[u].y * xv[u].x;
+        }
+    }
+    for (; c < n; c += 64) {
         const double2 av = a[c], xv = x[c];
         sx += av.x * xv.x - av.y * xv.y;
         sy += av.x * xv.y + av.y * xv.x;
