@@ -40,13 +40,26 @@ const Orbit kD8[] = {{1, 0.144315607677787, {0, 0, 0}},
                      {3, 0.032458497623198, {0.898905543365938, 0.050547228317031, 0}},
                      {6, 0.027230314174435, {0.008394777409958, 0.263112829634638, 0.728492392955404}}};
 
-const Orbit *kTab[9] = {nullptr, kD1, kD2, kD3, kD4, kD5, kD6, kD7, kD8};
-const int kLen[9] = {0, 1, 1, 2, 2, 3, 3, 4, 5};
+const Orbit kD9[] = {{1, 0.097135796282799, {0, 0, 0}},
+                     {3, 0.031334700227139, {0.020634961602525, 0.489682519198738, 0}},
+                     {3, 0.077827541004774, {0.125820817014127, 0.437089591492937, 0}},
+                     {3, 0.079647738927210, {0.623592928761935, 0.188203535619033, 0}},
+                     {3, 0.025577675658698, {0.910540973211095, 0.044729513394453, 0}},
+                     {6, 0.043283539377289, {0.036838412054736, 0.221962989160766, 0.741198598784498}}};
+const Orbit kD10[] = {{1, 0.090817990382754, {0, 0, 0}},
+                      {3, 0.036725957756467, {0.028844733232685, 0.485577633383657, 0}},
+                      {3, 0.045321059435528, {0.781036849029926, 0.109481575485037, 0}},
+                      {6, 0.072757916845420, {0.141707219414880, 0.307939838764121, 0.550352941820999}},
+                      {6, 0.028327242531057, {0.025003534762686, 0.246672560639903, 0.728323904597411}},
+                      {6, 0.009421666963733, {0.009540815400299, 0.066803251012200, 0.923655933587500}}};
+
+const Orbit *kTab[11] = {nullptr, kD1, kD2, kD3, kD4, kD5, kD6, kD7, kD8, kD9, kD10};
+const int kLen[11] = {0, 1, 1, 2, 2, 3, 3, 4, 5, 6, 6};
 }  // namespace
 
 int triangle_rule(int degree, double *u, double *v, double *w)
 {
-    if (degree < 1 || degree > 8) return -1;
+    if (degree < 1 || degree > 10) return -1;
     int n = 0;
     for (int o = 0; o < kLen[degree]; ++o) {
         const Orbit &ob = kTab[degree][o];

@@ -41,6 +41,18 @@ _DUNAVANT = {
         (3, 0.103217370534718, (0.658861384496480, 0.170569307751760)),
         (3, 0.032458497623198, (0.898905543365938, 0.050547228317031)),
         (6, 0.027230314174435, (0.008394777409958, 0.263112829634638, 0.728492392955404))],
+    9: [(1, 0.097135796282799, ()),
+        (3, 0.031334700227139, (0.020634961602525, 0.489682519198738)),
+        (3, 0.077827541004774, (0.125820817014127, 0.437089591492937)),
+        (3, 0.079647738927210, (0.623592928761935, 0.188203535619033)),
+        (3, 0.025577675658698, (0.910540973211095, 0.044729513394453)),
+        (6, 0.043283539377289, (0.036838412054736, 0.221962989160766, 0.741198598784498))],
+    10: [(1, 0.090817990382754, ()),
+         (3, 0.036725957756467, (0.028844733232685, 0.485577633383657)),
+         (3, 0.045321059435528, (0.781036849029926, 0.109481575485037)),
+         (6, 0.072757916845420, (0.141707219414880, 0.307939838764121, 0.550352941820999)),
+         (6, 0.028327242531057, (0.025003534762686, 0.246672560639903, 0.728323904597411)),
+         (6, 0.009421666963733, (0.009540815400299, 0.066803251012200, 0.923655933587500))],
 }
 
 MAX_TRIANGLE_DEGREE = max(_DUNAVANT)

@@ -64,7 +64,7 @@ def _c2(z):
 
 
 def trgauss(degree):
-    u, v, w = np.zeros(16), np.zeros(16), np.zeros(16)
+    u, v, w = np.zeros(32), np.zeros(32), np.zeros(32)
     n = lib().or_trgauss(degree, _d(u), _d(v), _d(w))
     return np.stack([u[:n], v[:n]]), w[:n]
 

@@ -11,7 +11,7 @@ def _moment(a, b):
     return math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
 
 
-@pytest.mark.parametrize("deg", range(1, 9))
+@pytest.mark.parametrize("deg", range(1, 11))
 def test_dunavant_exactness(bj, orc, deg):
     uv, w = bj.trgauss(deg)
     assert abs(w.sum() - 0.5) < 1e-14
