@@ -11,9 +11,11 @@ from .operators import (Maxwell3D, MWSingleLayer3D, MWDoubleLayer3D, NCross, Loc
                         LinearCombinationOfOperators,
                         DoubleNumWiltonSauterQStrat, DoubleNumSauterQstrat, DoubleNumQStrat, SingleNumQStrat,
                         defaultquadstrat, gamma_wavenumber_handler, n, cross, x_hat, y_hat, z_hat,
-                        TangTraceMW, CrossTraceMW)
+                        TangTraceMW, CrossTraceMW, FarField)
 from .assembly import (Context, default_context, assemble, assemble_rhs, classify, local_blocks, element_to_dof,
                        DeviceMatrix, lu)
+from .postproc import (MWFarField3D, MWDoubleLayerFarField3D, MWDoubleLayerRotatedFarField3D,
+                       MWFarField3DDropConstant, potential)
 from ._native import BHError, UnsupportedOnGPU, SingularException, NoConvergence
 
 __all__ = [k for k in dir() if not k.startswith("_")]

@@ -103,7 +103,7 @@ struct DevBuf {
 // Dunavant triangle rules in chart parameters (u, v); weights sum to 1/2.
 int triangle_rule(int degree, double *u, double *v, double *w);  // returns npts or -1
 void legendre01(int n, double *x, double *w);                   // nodes ascending on [0,1]
-constexpr int kMaxTriPts = 16;
+constexpr int kMaxTriPts = 32;
 constexpr int kMaxGL = 16;
 
 // ---------------------------------------------------------------- objects

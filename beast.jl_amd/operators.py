@@ -1,7 +1,7 @@
 """Operators, excitations and quadrature strategies of the Maxwell/RT path.
 
 Mirrors the reference's constructors and parameter handling:
-``Maxwell3D.singlelayer/doublelayer/planewave`` (src/maxwell/maxwell.jl:16-95),
+``Maxwell3D.singlelayer/doublelayer/planewave/farfield`` (src/maxwell/maxwell.jl:16-107),
 ``MWSingleLayer3D``/``MWDoubleLayer3D`` (src/maxwell/mwops.jl:11-64),
 ``gamma_wavenumber_handler`` (mwops.jl:213-238), the quadrature strategies
 (src/quadrature/doublenumwiltonsauterqstrat.jl:1-10, doublenumsauterqstrat.jl:1-8,
@@ -70,7 +70,10 @@ class SingleNumQStrat(AbstractQuadStrat):
 
 def defaultquadstrat(op, X=None, Y=None):
     """defaultquadstrat for Maxwell operators on RT x RT (mwops.jl:55), for local
-    operators on RT x RT (identityop.jl:66-67) and for functionals (excitation.jl:6)."""
+    operators on RT x RT (identityop.jl:66-67), for functionals (excitation.jl:6)
+    and for far-field operators (postproc/farfield.jl:3)."""
+    if isinstance(op, FarField):
+        return SingleNumQStrat(10)
     if isinstance(op, Functional):
         return SingleNumQStrat(8)
     if isinstance(op, LocalOperator):
@@ -210,6 +213,12 @@ class _Maxwell3D:
     def planewave(direction, polarization, wavenumber, amplitude=1.0):
         return PlaneWaveMW(direction, polarization, 1j * wavenumber, amplitude)
 
+    @staticmethod
+    def farfield(wavenumber):
+        """Maxwell3D.farfield(; wavenumber) (maxwell.jl:105-107)."""
+        from .postproc import MWFarField3D
+        return MWFarField3D(wavenumber=wavenumber)
+
 
 Maxwell3D = _Maxwell3D()
 
@@ -240,6 +249,11 @@ class Functional:
     pass
 
 
+class FarField:
+    """Far-field post-processing operators (postproc/farfield.jl:1); the concrete
+    Maxwell ones are in the postproc module."""
+
+
 class CrossTraceMW(Functional):
     def __init__(self, field):
         self.field = field
@@ -254,6 +268,8 @@ class TangTraceMW(Functional):
 
 def cross(a, b):
     """``cross(n, E) -> CrossTraceMW``, ``cross(CrossTraceMW, n) -> TangTraceMW`` (excitation.jl:166-167)."""
+    if isinstance(a, NormalVector) and isinstance(b, FarField):
+        return b._ncross()
     if isinstance(a, NormalVector) and not isinstance(b, NormalVector):
         return CrossTraceMW(b)
     if isinstance(a, CrossTraceMW) and isinstance(b, NormalVector):

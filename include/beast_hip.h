@@ -150,6 +150,24 @@ int bh_assemble_planewave(bh_ctx *ctx, const bh_space *X, const double *directio
                           const double *pol_im, const double *gamma, const double *amplitude, int rule,
                           int accumulate, double *b);
 
+/* Far field of a surface current (potential(op, points, coeffs, X), src/postproc.jl:138-150,174-211,
+ * with the far-field operators of src/maxwell/farfield.jl:1-120): for each point y (a direction,
+ * points[3*p..3*p+2], used as given),
+ *     ff[p] = amplitude * sum over cells and rule points x of w I(y, K(y, x) J(x)),
+ * J(x) = sum_n coeffs[n] f_n(x) (coeffs: ndofs complex), K = exp(gamma y.x), and
+ *     BH_FF_SINGLELAYER          I = (y x KJ) x y           MWFarField3D (:61-63)
+ *     BH_FF_DOUBLELAYER          I =  y x KJ                MWDoubleLayerFarField3D (:65-67)
+ *     BH_FF_DOUBLELAYER_ROTATED  I =  y x (KJ x n)          MWDoubleLayerRotatedFarField3D (:98-100)
+ *     BH_FF_DROPCONSTANT         I = (y x KJ) x y, K = expm1(gamma y.x)   MWFarField3DDropConstant (:69-75)
+ * rule = triangle rule degree 1..10 (the default SingleNumQStrat(10), src/postproc/farfield.jl:3).
+ * ff: 3 complex per point (6 doubles), overwritten.  FP64 throughout. */
+#define BH_FF_SINGLELAYER 0
+#define BH_FF_DOUBLELAYER 1
+#define BH_FF_DOUBLELAYER_ROTATED 2
+#define BH_FF_DROPCONSTANT 3
+int bh_farfield(bh_ctx *ctx, int kind, const double *gamma, const double *amplitude, const bh_space *X,
+                const double *coeffs, int64_t npoints, const double *points, int rule, double *ff);
+
 /* ---------------------------------------------------------------- resident matrices
  * A square system matrix kept in HBM with its LU factors (no counterpart in the
  * reference: its Matrix{ComplexF64} lives on the host and is factored by

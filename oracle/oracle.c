@@ -28,6 +28,7 @@
  *   scatter             src/integralop.jl:146-184, src/operator.jl:168-184
  *   plane-wave RHS      src/excitation.jl:38-115,149-198, src/maxwell/mwexc.jl:36-42
  *   NCross              src/identityop.jl:14-22, src/localop.jl:114-146,362-386
+ *   far fields          src/postproc.jl:174-211,273-288, src/maxwell/farfield.jl:56-75,98-100
  *
  * Parity pins (see tests/test_oracle_*.py): Dunavant exactness, Wilton vs brute
  * force, DQ vs Wilton-SE on far pairs, SS vs Wilton-SE at the reference test
@@ -833,4 +834,102 @@ int or_planewave(const double *V, const int32_t *C, int nf, int nd, const int32_
             b[2 * m + 1] += cimag(v);
         }
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Far fields (postproc.jl:174-211 potential!, :273-288 farfieldlocal!, */
+/* with maxwell/farfield.jl:56-75,98-100): for each point y, for each   */
+/* cell, zlocal[r] = sum_x dx I(y, krn f_r(x)), krn = exp(gamma y.x)    */
+/* (expm1 for the drop-constant operator); then ff[y] += zlocal[r] b    */
+/* coeffs[n] for every (n, b) of the cell's shape r.  kind: 0 single    */
+/* layer (y x v) x y, 1 double layer y x v, 2 rotated double layer      */
+/* y x (v x n), 3 drop-constant (y x v) x y with expm1.                 */
+/* ------------------------------------------------------------------ */
+static cplx or_cexpm1(cplx z)
+{
+    /* e^(a+ib) - 1 = (e^a - 1) - 2 e^a sin^2(b/2) + i e^a sin b */
+    double a = creal(z), b = cimag(z), em = expm1(a), h = sin(0.5 * b);
+    return (em - 2.0 * (em + 1.0) * h * h) + I * ((em + 1.0) * sin(b));
+}
+
+int or_farfield(const double *V, const int32_t *C, int nf, int nd, const int32_t *ptr, const int32_t *cell,
+                const int32_t *ref, const double *coef, int kind, const double *gam, const double *amp, int npts,
+                const double *pts, const double *cre, const double *cim, int degree, double *ff)
+{
+    if (kind < 0 || kind > 3) return -2;
+    double u[OR_MAXQ], v[OR_MAXQ], w[OR_MAXQ];
+    if (or_trgauss(degree, u, v, w) < 0) return -1;
+    cplx gm = gam[0] + I * gam[1], am = amp[0] + I * amp[1];
+    /* element-to-dof lists: shapes of each (cell, r) in ascending dof order */
+    int ns = ptr[nd];
+    int *cnt = calloc((size_t)3 * nf + 1, sizeof(int)), *sh = malloc(sizeof(int) * (size_t)(ns + 1));
+    int *fill = calloc((size_t)3 * nf + 1, sizeof(int)), *dofof = malloc(sizeof(int) * (size_t)(ns + 1));
+    for (int k = 0; k < ns; ++k) cnt[3 * cell[k] + ref[k] + 1]++;
+    for (int s = 0; s < 3 * nf; ++s) cnt[s + 1] += cnt[s];
+    for (int m = 0; m < nd; ++m)
+        for (int k = ptr[m]; k < ptr[m + 1]; ++k) {
+            int s = 3 * cell[k] + ref[k];
+            sh[cnt[s] + fill[s]] = k;
+            dofof[cnt[s] + fill[s]] = m;
+            fill[s]++;
+        }
+    int rc = 0;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int p = 0; p < npts; ++p) {
+        const double *y = pts + 3 * p;
+        cplx acc[3] = {0, 0, 0};
+        for (int c = 0; c < nf; ++c) {
+            double tv[9];
+            for (int a = 0; a < 3; ++a)
+                for (int q = 0; q < 3; ++q) tv[3 * a + q] = V[3 * C[3 * c + a] + q];
+            double xp[3 * OR_MAXQ], wx[OR_MAXQ];
+            int np = tri_points(tv, degree, xp, wx);
+            double A = tri_area(tv), t1[3], t2[3], nrm[3];
+            sub3(t1, tv + 0, tv + 6);
+            sub3(t2, tv + 3, tv + 6);
+            cross3(nrm, t1, t2);
+            double nn = norm3(nrm);
+            for (int q = 0; q < 3; ++q) nrm[q] /= nn;
+            cplx z[3][3] = {{0}};
+            for (int k = 0; k < np; ++k) {
+                const double *x = xp + 3 * k;
+                cplx krn = kind == 3 ? or_cexpm1(gm * dot3(y, x)) : cexp(gm * dot3(y, x));
+                for (int r = 0; r < 3; ++r) {
+                    cplx f[3], a[3], b[3];
+                    for (int q = 0; q < 3; ++q) f[q] = krn * ((x[q] - tv[3 * r + q]) / (2.0 * A));
+                    if (kind == 2) { /* (krn f) x n */
+                        cplx g0 = f[1] * nrm[2] - f[2] * nrm[1], g1 = f[2] * nrm[0] - f[0] * nrm[2],
+                             g2 = f[0] * nrm[1] - f[1] * nrm[0];
+                        f[0] = g0; f[1] = g1; f[2] = g2;
+                    }
+                    /* a = y x f */
+                    a[0] = y[1] * f[2] - y[2] * f[1];
+                    a[1] = y[2] * f[0] - y[0] * f[2];
+                    a[2] = y[0] * f[1] - y[1] * f[0];
+                    if (kind == 0 || kind == 3) { /* (y x f) x y */
+                        b[0] = a[1] * y[2] - a[2] * y[1];
+                        b[1] = a[2] * y[0] - a[0] * y[2];
+                        b[2] = a[0] * y[1] - a[1] * y[0];
+                    } else {
+                        b[0] = a[0]; b[1] = a[1]; b[2] = a[2];
+                    }
+                    for (int q = 0; q < 3; ++q) z[r][q] += (am * b[q]) * wx[k];
+                }
+            }
+            for (int r = 0; r < 3; ++r) {
+                int s = 3 * c + r;
+                for (int e = cnt[s]; e < cnt[s + 1]; ++e) {
+                    int m = dofof[e];
+                    cplx cf = coef[sh[e]] * (cre[m] + I * cim[m]);
+                    for (int q = 0; q < 3; ++q) acc[q] += z[r][q] * cf;
+                }
+            }
+        }
+        for (int q = 0; q < 3; ++q) {
+            ff[6 * p + 2 * q] = creal(acc[q]);
+            ff[6 * p + 2 * q + 1] = cimag(acc[q]);
+        }
+    }
+    free(cnt); free(sh); free(fill); free(dofof);
+    return rc;
 }

@@ -50,6 +50,9 @@ def lib():
         L.or_ncross.argtypes = [_D, _I32, C.c_int, C.c_int, _I32, _I32, _I32, _D, C.c_int, _I32, _I32, _I32, _D,
                                 C.c_int, _D, _D]
         L.or_ncross.restype = C.c_int
+        L.or_farfield.argtypes = [_D, _I32, C.c_int, C.c_int, _I32, _I32, _I32, _D, C.c_int, _D, _D, C.c_int, _D,
+                                  _D, _D, C.c_int, _D]
+        L.or_farfield.restype = C.c_int
         _lib = L
     return _lib
 
@@ -175,6 +178,26 @@ def planewave(X, direction, polarization, gamma, amplitude=1.0, degree=8):
     if rc != 0:
         raise ValueError("bad rule degree")
     return b[0::2] + 1j * b[1::2]
+
+
+def farfield(kind, gamma, amplitude, points, coeffs, X, degree=10):
+    """potential(op, points, coeffs, X) for the far-field operators (postproc.jl:174-211,273-288,
+    maxwell/farfield.jl:56-75,98-100); kind 0 MWFarField3D, 1 MWDoubleLayerFarField3D,
+    2 MWDoubleLayerRotatedFarField3D, 3 MWFarField3DDropConstant.  Returns [npoints, 3] complex."""
+    m = X.geo
+    V, Cc = np.ascontiguousarray(m.vertices, float), np.ascontiguousarray(m.cells, np.int32)
+    n, p, c, r, co = _space_arrays(X)
+    pts = np.ascontiguousarray(np.asarray(points, float).reshape(-1, 3))
+    u = np.asarray(coeffs, dtype=np.complex128)
+    cre, cim = np.ascontiguousarray(u.real), np.ascontiguousarray(u.imag)
+    ff = np.zeros(6 * len(pts))
+    i32 = lambda t: t.ctypes.data_as(_I32)
+    rc = lib().or_farfield(_d(V), i32(Cc), m.numcells, n, i32(p), i32(c), i32(r), _d(co), int(kind),
+                           _d(_c2(gamma)), _d(_c2(amplitude)), len(pts), _d(pts), _d(cre), _d(cim), int(degree),
+                           _d(ff))
+    if rc != 0:
+        raise ValueError("bad rule degree or far-field kind")
+    return (ff[0::2] + 1j * ff[1::2]).reshape(-1, 3)
 
 
 def lu_solve(Z, B):

@@ -93,13 +93,16 @@ def wilton_brute(trgauss, p1, p2, p3, x, L=4):
     return s, v, g
 
 
-def farfield_magnitudes(bj, u, X, thetas, gamma=1.0j, omega=1.0, c=1.0, degree=8):
-    """|E_far| of the current sum_b u_b f_b at directions (sin t, 0, cos t), as test_farfield.jl:8-40:
-    MWFarField3D(gamma): integrand (y x (exp(gamma y.x) J)) x y, scaled by i omega / (4 pi c)."""
+def farfield(bj, u, X, Y, kind=0, gamma=1.0j, amplitude=1.0, degree=10):
+    """Far field of the current sum_b u_b f_b at directions Y, summed as S = sum w K J over all
+    rule points (numpy arrangement, independent of the oracle's per-cell loop):
+    kind 0 (y x S) x y, 1 y x S, 2 y x S_n with S_n = sum w K (J x n), 3 as 0 with K = expm1."""
     mesh = X.geo
     uv, w = bj.trgauss(degree)
     V = mesh.vertices[mesh.cells]
     A = mesh.areas()
+    nrm = np.cross(V[:, 0] - V[:, 2], V[:, 1] - V[:, 2])
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
     P = (uv[0][None, :, None] * V[:, 0:1, :] + uv[1][None, :, None] * V[:, 1:2, :]
          + (1 - uv[0] - uv[1])[None, :, None] * V[:, 2:3, :])
     W = w[None, :] * 2 * A[:, None]
@@ -107,12 +110,24 @@ def farfield_magnitudes(bj, u, X, thetas, gamma=1.0j, omega=1.0, c=1.0, degree=8
     for b in range(X.numfunctions()):
         for (cell, r, cf) in X.basisfunction(b):
             J[cell] += cf * u[b] * (P[cell] - V[cell, r][None, :]) / (2 * A[cell])
+    if kind == 2:
+        J = np.cross(J, nrm[:, None, :])
     out = []
-    for th in thetas:
-        y = np.array([np.sin(th), 0.0, np.cos(th)])
-        S = np.einsum("fq,fq,fqk->k", W, np.exp(gamma * (P @ y)), J)
-        out.append(np.linalg.norm(np.cross(np.cross(y, S), y) * (1j * omega / (4 * np.pi * c))))
+    for y in np.asarray(Y, float):
+        ph = gamma * (P @ y)
+        K = np.expm1(ph) if kind == 3 else np.exp(ph)
+        S = np.einsum("fq,fq,fqk->k", W, K, J)
+        v = np.cross(y, S)
+        out.append(amplitude * (np.cross(v, y) if kind in (0, 3) else v))
     return np.array(out)
+
+
+def farfield_magnitudes(bj, u, X, thetas, gamma=1.0j, omega=1.0, c=1.0, degree=8):
+    """|E_far| of the current sum_b u_b f_b at directions (sin t, 0, cos t), as test_farfield.jl:8-40:
+    MWFarField3D(gamma): integrand (y x (exp(gamma y.x) J)) x y, scaled by i omega / (4 pi c)."""
+    Y = np.stack([np.sin(thetas), 0 * np.asarray(thetas), np.cos(thetas)], 1)
+    ff = farfield(bj, u, X, Y, 0, gamma, 1.0, degree) * (1j * omega / (4 * np.pi * c))
+    return np.linalg.norm(ff, axis=1)
 
 
 # test/test_farfield.jl:34-40 (sphere824.in = sphere2.in, EFIE solution at kappa = 1)
