@@ -17,7 +17,10 @@
 // (k_ff_current) and S is a dense sum over (direction, point) pairs
 // (k_ff_partial: directions on lanes, points staged through LDS and broadcast to
 // every lane, chunks of points over the grid's second dimension), reduced over
-// the chunks in a fixed order (k_ff_finish).  Everything is FP64.
+// the chunks in a fixed order (k_ff_finish).  For lossless kernels the phase is
+// factored per cell (k_ff_partial_cells): one sincos per (direction, cell) and
+// Taylor polynomials of the small in-cell phase.  Everything is FP64.
+#include <algorithm>
 #include <cmath>
 
 #include "bh_device.h"
@@ -36,9 +39,10 @@ constexpr int kFFBlocks = 4096;    // target workgroups of k_ff_partial
 // element-to-dof slots (dof, c) of (cell, r) of c coeffs[dof] (the RT shape
 // values of src/bases/local/rtlocal.jl and the assemblydata scatter of
 // postproc.jl:200-205).  rotated: w J x n (the rotated double layer's J x n).
+// relative: the position is stored as x - centroid (for k_ff_partial_cells).
 __global__ void k_ff_current(const double *__restrict__ geo, const double *__restrict__ qp, int Q, int64_t nf,
                              const int32_t *__restrict__ e2d, const double *__restrict__ e2c, int kmax,
-                             const double *__restrict__ coeffs, int rotated, double *__restrict__ pts)
+                             const double *__restrict__ coeffs, int rotated, int relative, double *__restrict__ pts)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nf * Q) return;
@@ -71,7 +75,7 @@ __global__ void k_ff_current(const double *__restrict__ geo, const double *__res
     }
     double *o = pts + i * kFFPoint;
     for (int d = 0; d < 3; ++d) {
-        o[d] = x[d];
+        o[d] = relative ? q[d] : x[d];
         o[3 + d] = jr[d];
         o[6 + d] = ji[d];
     }
@@ -125,6 +129,86 @@ __global__ __launch_bounds__(kFFThreads) void k_ff_partial(const double *__restr
             for (int d = 0; d < 3; ++d) {
                 sr[d] += kr * s[3 + d] - ki * s[6 + d];
                 si[d] += kr * s[6 + d] + ki * s[3 + d];
+            }
+        }
+    }
+    if (!live) return;
+    double *o = part + ((int64_t)blockIdx.y * P + p) * 6;
+    for (int d = 0; d < 3; ++d) {
+        o[d] = sr[d];
+        o[3 + d] = si[d];
+    }
+}
+
+// Taylor coefficients of sin and cos in u = t^2: sin t = t sum_k s_k u^k,
+// cos t = sum_k c_k u^k, s_k = (-1)^k / (2k+1)!, c_k = (-1)^k / (2k)!.
+template <int K> struct TrigSeries {
+    double s[K + 1], c[K + 1];
+    constexpr TrigSeries() : s(), c() {
+        double f = 1.0;   // (2k)!
+        for (int k = 0; k <= K; ++k) {
+            if (k > 0) f *= (2.0 * k - 1.0) * (2.0 * k);
+            c[k] = (k % 2 ? -1.0 : 1.0) / f;
+            s[k] = (k % 2 ? -1.0 : 1.0) / (f * (2.0 * k + 1.0));
+        }
+    }
+};
+
+// Partial sums as k_ff_partial for a lossless kernel exp(i gi y.x) (gi real), factored
+// per cell: exp(i gi y.x) = exp(i gi y.c) exp(i gi y.(x - c)) with c the cell centroid.
+// The first factor is one sincos per (direction, cell); the second has a phase of at
+// most theta_max = |gi| max|y| max|x - c|, and its cos and sin are the Taylor
+// polynomials of degree 2K and 2K+1, whose remainders theta_max^(2K+2)/(2K+2)! and
+// theta_max^(2K+3)/(2K+3)! the host keeps below 1e-17 when it picks K.  Per cell:
+// A = sum_q exp(i gi y.(x_q - c)) w J(x_q), then S += exp(i gi y.c) A.
+// Tiles hold whole cells (CT = floor(kFFThreads / Q) cells), and chunks whole tiles.
+template <int K>
+__global__ __launch_bounds__(kFFThreads) void k_ff_partial_cells(const double *__restrict__ pts,
+                                                                 const double *__restrict__ geo, int64_t nf, int Q,
+                                                                 int64_t per_cells, const double *__restrict__ dirs,
+                                                                 int64_t P, double gi, double *__restrict__ part)
+{
+    constexpr TrigSeries<K> ts{};
+    __shared__ double tile[kFFThreads * kFFPoint];
+    __shared__ double cen[kFFThreads * 3];
+    const int CT = kFFThreads / Q;
+    const int64_t p = (int64_t)blockIdx.x * kFFThreads + threadIdx.x;
+    const bool live = p < P;
+    const double y0 = live ? dirs[3 * p] : 0.0, y1 = live ? dirs[3 * p + 1] : 0.0, y2 = live ? dirs[3 * p + 2] : 0.0;
+    const double g0 = gi * y0, g1 = gi * y1, g2 = gi * y2;
+    const int64_t c0 = (int64_t)blockIdx.y * per_cells;
+    const int64_t c1 = c0 + per_cells < nf ? c0 + per_cells : nf;
+    double sr[3] = {0.0, 0.0, 0.0}, si[3] = {0.0, 0.0, 0.0};
+    for (int64_t t0 = c0; t0 < c1; t0 += CT) {
+        const int nc = (int)(c1 - t0 < CT ? c1 - t0 : CT);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nc * Q * kFFPoint; e += kFFThreads) tile[e] = pts[t0 * Q * kFFPoint + e];
+        for (int e = threadIdx.x; e < nc * 3; e += kFFThreads) cen[e] = geo[(t0 + e / 3) * kGeo + kGeoC + e % 3];
+        __syncthreads();
+        if (!live) continue;
+        for (int j = 0; j < nc; ++j) {
+            double ar[3] = {0.0, 0.0, 0.0}, ai[3] = {0.0, 0.0, 0.0};
+            for (int q = 0; q < Q; ++q) {
+                const double *s = tile + (j * Q + q) * kFFPoint;
+                const double t = g0 * s[0] + g1 * s[1] + g2 * s[2];
+                const double u = t * t;
+                double ps = ts.s[K], pc = ts.c[K];
+#pragma unroll
+                for (int k = K - 1; k >= 0; --k) {
+                    ps = ps * u + ts.s[k];
+                    pc = pc * u + ts.c[k];
+                }
+                const double sn = t * ps, cs = pc;
+                for (int d = 0; d < 3; ++d) {
+                    ar[d] += cs * s[3 + d] - sn * s[6 + d];
+                    ai[d] += cs * s[6 + d] + sn * s[3 + d];
+                }
+            }
+            double en, ec;
+            sincos(g0 * cen[3 * j] + g1 * cen[3 * j + 1] + g2 * cen[3 * j + 2], &en, &ec);
+            for (int d = 0; d < 3; ++d) {
+                sr[d] += ec * ar[d] - en * ai[d];
+                si[d] += ec * ai[d] + en * ar[d];
             }
         }
     }
@@ -202,8 +286,33 @@ extern "C" int bh_farfield(bh_ctx *ctx, int kind, const double *gamma, const dou
     const double *qp = (const double *)quad_points(m, rule, &Q, false, s);
     const int64_t M = m.nf * Q;
     const int64_t P = npoints;
+    // Lossless kernels (gamma purely imaginary, not the drop-constant one) take the
+    // cell-factored path when the in-cell phase bound admits a Taylor degree in {4, 6, 9}.
+    int K = 0;
+    if (gamma[0] == 0.0 && kind != BH_FF_DROPCONSTANT && m.nf > 0 && Q <= kFFThreads) {
+        double rmax = 0.0, ymax = 0.0;
+        for (int64_t c = 0; c < m.nf; ++c) {
+            const double *g = &m.geo[(size_t)c * kGeo];
+            for (int a = 0; a < 3; ++a) {
+                const double d0 = g[3 * a] - g[kGeoC], d1 = g[3 * a + 1] - g[kGeoC + 1], d2 = g[3 * a + 2] - g[kGeoC + 2];
+                rmax = std::max(rmax, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+            }
+        }
+        for (int64_t p = 0; p < P; ++p) {
+            const double *y = points + 3 * p;
+            ymax = std::max(ymax, std::sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]));
+        }
+        const double th = std::fabs(gamma[1]) * ymax * rmax * (1.0 + 1e-12);
+        for (int k : {4, 6, 9}) {
+            // remainder of the cos series, theta^(2k+2) / (2k+2)!, bounds the sin one's too
+            double r = 1.0;
+            for (int j = 1; j <= 2 * k + 2; ++j) r *= th / j;
+            if (std::isfinite(th) && r <= 1e-17) { K = k; break; }
+        }
+    }
     const int64_t nblk = (P + kFFThreads - 1) / kFFThreads;
-    const int64_t tiles = (M + kFFThreads - 1) / kFFThreads;
+    const int CT = K ? kFFThreads / Q : 1;                            // cells per tile
+    const int64_t tiles = K ? (m.nf + CT - 1) / CT : (M + kFFThreads - 1) / kFFThreads;
     int64_t nchunk = (kFFBlocks + nblk - 1) / nblk;
     if (nchunk > tiles) nchunk = tiles;
     if (nchunk < 1) nchunk = 1;
@@ -211,7 +320,7 @@ extern "C" int bh_farfield(bh_ctx *ctx, int kind, const double *gamma, const dou
     const int64_t cap = ((int64_t)1 << 30) / (48 * P);
     if (nchunk > cap) nchunk = cap < 1 ? 1 : cap;
     BH_REQUIRE(nblk <= 0x7fffffff && nchunk <= 65535, "too many far-field directions");
-    const int64_t per = (tiles + nchunk - 1) / nchunk * kFFThreads;   // whole tiles per chunk
+    const int64_t per_tiles = (tiles + nchunk - 1) / nchunk;          // whole tiles per chunk
     DevBuf<double> dcoef(2 * sp->ndofs + 2), dpts(M * kFFPoint + 1), ddir(3 * P), dpart(nchunk * P * 6), dff(6 * P);
     if (sp->ndofs) dcoef.upload(coeffs, 2 * sp->ndofs, s);
     ddir.upload(points, 3 * P, s);
@@ -219,13 +328,27 @@ extern "C" int bh_farfield(bh_ctx *ctx, int kind, const double *gamma, const dou
         ProfScope prof(*ctx, "k_ff_current", (double)M);
         k_ff_current<<<(unsigned)((M + 255) / 256), 256, 0, s>>>(m.d_geo.p, qp, Q, m.nf, sp->d_e2d.p, sp->d_e2c.p,
                                                                sp->kmax, dcoef.p,
-                                                               kind == BH_FF_DOUBLELAYER_ROTATED, dpts.p);
+                                                               kind == BH_FF_DOUBLELAYER_ROTATED, K != 0, dpts.p);
         check_ff_launch("k_ff_current");
     }
-    {
+    if (K) {
+        ProfScope prof(*ctx, "k_ff_partial_cells", (double)M * (double)P);
+        const dim3 grid((unsigned)nblk, (unsigned)nchunk);
+        const int64_t per_cells = per_tiles * CT;
+        if (K == 4)
+            k_ff_partial_cells<4><<<grid, kFFThreads, 0, s>>>(dpts.p, m.d_geo.p, m.nf, Q, per_cells, ddir.p, P,
+                                                              gamma[1], dpart.p);
+        else if (K == 6)
+            k_ff_partial_cells<6><<<grid, kFFThreads, 0, s>>>(dpts.p, m.d_geo.p, m.nf, Q, per_cells, ddir.p, P,
+                                                              gamma[1], dpart.p);
+        else
+            k_ff_partial_cells<9><<<grid, kFFThreads, 0, s>>>(dpts.p, m.d_geo.p, m.nf, Q, per_cells, ddir.p, P,
+                                                              gamma[1], dpart.p);
+        check_ff_launch("k_ff_partial_cells");
+    } else {
         ProfScope prof(*ctx, "k_ff_partial", (double)M * (double)P);
         k_ff_partial<<<dim3((unsigned)nblk, (unsigned)nchunk), kFFThreads, 0, s>>>(
-            dpts.p, M, per, ddir.p, P, gamma[0], gamma[1], kind == BH_FF_DROPCONSTANT, dpart.p);
+            dpts.p, M, per_tiles * kFFThreads, ddir.p, P, gamma[0], gamma[1], kind == BH_FF_DROPCONSTANT, dpart.p);
         check_ff_launch("k_ff_partial");
     }
     {

@@ -84,6 +84,42 @@ def cpu_baseline(bj, mesh, X, t, k, ncells=128, nlu=6000):
                        "CPU restatement of the reference algorithm, not BEAST.jl")}
 
 
+def farfield_sample(bj, ctx, X, x, k, cpu=True, nth=181, nph=361):
+    """Far field of the solved current on a full (theta, phi) grid of directions
+    (potential(MWFarField3D(wavenumber=k), points, x, X), default SingleNumQStrat(10)),
+    outside the timed region.  Work unit: one (direction, rule point) term."""
+    th, ph = np.meshgrid(np.linspace(0, np.pi, nth), np.linspace(0, 2 * np.pi, nph), indexing="ij")
+    Y = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1).reshape(-1, 3)
+    op = bj.MWFarField3D(wavenumber=k)
+    bj.potential(op, Y[:256], x, X)
+    ctx.profile(True)
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ff = bj.potential(op, Y, x, X)
+        walls.append(time.perf_counter() - t0)
+    kern = {}
+    for name in ("k_ff_current", "k_ff_partial_cells", "k_ff_partial", "k_ff_finish"):
+        n_, s_, w_ = ctx.kernel_stats(name)
+        if n_:
+            kern[name] = {"launches": n_, "avg_ms": round(1e3 * s_ / n_, 3)}
+    ctx.profile(False)
+    terms = float(len(Y)) * X.geo.numcells * len(bj.trgauss(10)[1])
+    wall = float(np.median(walls))
+    out = {"workload": f"MWFarField3D(wavenumber={k}) of the solution at {nth}x{nph} directions, "
+                       f"{X.geo.numcells} cells x 25 rule points", "directions": len(Y), "terms": terms,
+           "wall_s": wall, "terms_per_s": terms / wall, "kernels": kern}
+    if cpu:
+        from oracle import oracle
+        sub = Y[:: max(1, len(Y) // 48)][:48]
+        t0 = time.perf_counter()
+        oracle.farfield(0, op.gamma, op.amplitude, sub, x, X)
+        tc = time.perf_counter() - t0
+        out["cpu_oracle"] = {"terms_per_s": len(sub) * X.geo.numcells * 25 / tc,
+                             "sample": f"{len(sub)} directions", "cores": int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +200,7 @@ def main():
             breakdown[name] = {"launches": n_, "ms": round(1e3 * s_, 3)}
     f_launch, f_sec, f_work = ctx.kernel_stats("k_far_tiles")
     ctx.profile(False)
+    farfield = farfield_sample(bj, ctx, X, x, k, cpu=not args.no_cpu_baseline)
     err = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
     value = t_all / args.steps
     achieved = g_work / g_sec if g_sec > 0 else 0.0
@@ -199,6 +236,7 @@ def main():
                      "launches": g_launch // max(1, args.steps), "avg_launch_ms": 1e3 * g_sec / max(1, g_launch)},
         "kernels": dict(breakdown, k_far_tiles={"launches": f_launch, "ms": round(1e3 * f_sec, 3),
                                                 "gflops_ref_formulation": f_work / max(f_sec, 1e-30) / 1e9}),
+        "farfield": farfield,
     }
     if args.serve_requests > 0 or args.batch_requests > 0:
         from importlib import import_module

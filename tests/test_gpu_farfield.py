@@ -38,6 +38,25 @@ def test_farfield_matches_oracle(bj, orc, golden, gamma, rule):
         assert _relmax(got, want) < 1e-12, (op, _relmax(got, want))
 
 
+@pytest.mark.parametrize("k", [0.2, 1.0, 4.0, 25.0])
+def test_farfield_cell_factored_paths(bj, orc, golden, k):
+    """Lossless kernels factor the phase per cell and take Taylor polynomials of degree
+    8/12/18 for the in-cell phase (bound |k| max|y| max|x - c|: 0.03, 0.16, 0.64 on
+    sphere2.in with unit directions), or the direct sincos path beyond that (k = 25):
+    every path against the oracle's complex exponential at 1e-12."""
+    m = bj.readmesh(os.path.join(golden, "sphere2.in"))
+    X = bj.raviartthomas(m)
+    rng = np.random.default_rng(int(k * 10))
+    u = rng.standard_normal(X.numfunctions()) + 1j * rng.standard_normal(X.numfunctions())
+    Y = rng.standard_normal((77, 3))
+    Y /= np.linalg.norm(Y, axis=1)[:, None]
+    for op in (bj.MWFarField3D(wavenumber=k), bj.MWDoubleLayerRotatedFarField3D(wavenumber=k)):
+        print(f"k {k} {type(op).__name__}", flush=True)
+        got = bj.potential(op, Y, u, X)
+        want = orc.farfield(op.kind, op.gamma, op.amplitude, Y, u, X)
+        assert _relmax(got, want) < 1e-12, (k, _relmax(got, want))
+
+
 def test_farfield_shapes_and_edges(bj, orc, golden):
     """One direction, an empty point set, a grid of points keeps its shape, a zero current
     gives zeros, and two calls give the same bits (fixed-order reduction)."""
