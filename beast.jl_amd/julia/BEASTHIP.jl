@@ -222,4 +222,39 @@ end
 
 Base.:\(M::DeviceMatrix, b::AbstractVecOrMat) = ldiv!(similar(b, ComplexF64), M, ComplexF64.(b))
 
+# Far fields: potential(op, points, coeffs, X) (postproc.jl:138-150,174-211) for the
+# Maxwell far-field operators (maxwell/farfield.jl:1-103) on a raviartthomas space.
+const FFKINDS = Dict{Any,Int32}(BEAST.MWFarField3D => 0, BEAST.MWDoubleLayerFarField3D => 1,
+                                BEAST.MWDoubleLayerRotatedFarField3D => 2, BEAST.MWFarField3DDropConstant => 3)
+
+"""
+    gpu_potential(op, points, coeffs, X; quadstrat=defaultquadstrat(op, X))
+
+Far field of the current `sum_n coeffs[n] f_n` at `points`, as `potential` returns it
+(an array of `SVector{3,ComplexF64}` of the points' shape).
+"""
+function gpu_potential(op::BEAST.MWFarField, points, coeffs, X::BEAST.RTBasis;
+        quadstrat=BEAST.defaultquadstrat(op, X), ctx=context(), type=BEAST.SVector{3,ComplexF64})
+    type == BEAST.SVector{3,ComplexF64} || throw(Unsupported("far fields are computed as SVector{3,ComplexF64}"))
+    kind = get(FFKINDS, typeof(op).name.wrapper, nothing)
+    kind === nothing && throw(Unsupported("far-field operator $(typeof(op))"))
+    amp = ComplexF64(op isa BEAST.MWFarField3DDropConstant ? op.coeff : op.amplitude)
+    g = ComplexF64(op.gamma)
+    @assert length(coeffs) == numfunctions(X)
+    P = Float64[p[k] for k in 1:3, p in vec(points)]
+    u = Vector{ComplexF64}(coeffs)
+    ff = Matrix{ComplexF64}(undef, 3, length(points))
+    check(ccall((:bh_farfield, libbeast_hip), Cint,
+        (Ptr{Cvoid}, Cint, Ref{ComplexF64}, Ref{ComplexF64}, Ptr{Cvoid}, Ptr{ComplexF64}, Int64, Ptr{Float64}, Cint,
+         Ptr{ComplexF64}),
+        ctx.ptr, kind, g, amp, spacehandle(X, ctx), u, length(points), P, quadstrat.quad_rule, ff))
+    return reshape([BEAST.SVector{3,ComplexF64}(ff[1, i], ff[2, i], ff[3, i]) for i in 1:length(points)], size(points))
+end
+
+function BEAST.potential(op::BEAST.MWFarField, points, coeffs, X::BEAST.RTBasis;
+        threading=BEAST.Threading{:single}, kwargs...)
+    threading == :gpu && return gpu_potential(op, points, coeffs, X; kwargs...)
+    return invoke(BEAST.potential, Tuple{Any,Any,Any,Any}, op, points, coeffs, X; threading, kwargs...)
+end
+
 end # module
