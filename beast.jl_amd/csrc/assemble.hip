@@ -704,12 +704,13 @@ struct FarArgs {
 
 // Far-field 3x3 block of (test p, trial q) by double Gauss, written to z[k*stride].
 // QO, QI: the point counts when fixed at compile time (0: read from A).
+// maybe_near = false: the caller knows the pair is not in p's non-far list.
 template <typename R, int OPK, int QO = 0, int QI = 0>
-__device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> *z, int stride)
+__device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> *z, int stride, bool maybe_near = true)
 {
     const int Qo = QO ? QO : A.Qo, Qi = QI ? QI : A.Qi;
     const double *tg = A.tgeo + (size_t)p * kGeo, *sg = A.sgeo + (size_t)q * kGeo;
-    if (A.mode != kModeDoubleNum) {
+    if (A.mode != kModeDoubleNum && maybe_near) {
         // the classification pass listed p's non-far trial cells in ascending
         // order with their blocks: a pair found there is not far
         const int end = A.nptr[p + 1];
@@ -862,6 +863,8 @@ struct TileArgs {
     TileSide row, col;
     void *Z;              // output, complex R
     long long ld;
+    void *Z32;            // optional row-major complex-float copy (FP64 row-major launches)
+    long long ld32;
     int rowmajor;
     int accumulate;
     double scale_r, scale_i;
@@ -874,6 +877,7 @@ __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
     __shared__ int s_rslot[kTile][2], s_cslot[kTile][2];
     __shared__ R s_rcoef[kTile][2], s_ccoef[kTile][2];
     __shared__ cx<R> s_blk[kPairChunk * 9];
+    __shared__ unsigned s_near[kTileCells * kTileCells / 32];   // bit rc * kTileCells + cc: non-far pair
     const int tr = blockIdx.y, tc = blockIdx.x;
     const int tid = threadIdx.x;
     const int r0 = A.row.ptr[tr], nr = A.row.ptr[tr + 1] - r0;
@@ -889,12 +893,37 @@ __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
         s_cslot[l][s] = n < A.col.ndofs ? A.col.slot[2 * n + s] : -1;
         s_ccoef[l][s] = n < A.col.ndofs ? (R)A.col.coef[2 * n + s] : (R)0;
     }
+    for (int k = tid; k < kTileCells * kTileCells / 32; k += 256) s_near[k] = 0u;
     // ownership: ROWMAJOR -> one column, 16 rows; else one row, 16 columns
     const int own = tid & 63, grp = tid >> 6;
     cx<R> acc[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = {(R)0, (R)0};
     __syncthreads();
+    // mark the tile's non-far pairs: each row cell's non-far list (ascending trial
+    // cells) looked up in the sorted column cells, so that far pairs skip the
+    // search of far_block
+    const bool lists = A.far.mode != kModeDoubleNum;
+    if (lists) {
+        for (int rc = tid; rc < nr; rc += 256) {
+            const int p = s_rc[rc];
+            const int e1 = A.far.nptr[p + 1];
+            for (int e = A.far.nptr[p]; e < e1; ++e) {
+                const int q = A.far.nent[e].y;
+                int lo = 0, hi = nc;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_cc[mid] < q) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < nc && s_cc[lo] == q) {
+                    const int bit = rc * kTileCells + lo;
+                    atomicOr(&s_near[bit >> 5], 1u << (bit & 31));
+                }
+            }
+        }
+        __syncthreads();
+    }
 
     const int CC = nr > 0 ? max(1, kPairChunk / nr) : 1;
     for (int cb = 0; cb < nc; cb += CC) {
@@ -903,7 +932,9 @@ __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
 #pragma unroll 1
         for (int t = tid; t < nr * cn; t += 256) {
             const int rc = t % nr, cc = t / nr;
-            far_block<R, OPK, QO, QI>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kPairChunk);
+            const int bit = rc * kTileCells + cb + cc;
+            const bool near = lists && ((s_near[bit >> 5] >> (bit & 31)) & 1u);
+            far_block<R, OPK, QO, QI>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kPairChunk, near);
         }
         __syncthreads();
         // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184
@@ -946,6 +977,9 @@ __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
         cx<R> v{sr * acc[e].r - si * acc[e].i, sr * acc[e].i + si * acc[e].r};
         if (A.accumulate) { v.r += Z[idx].r; v.i += Z[idx].i; }
         Z[idx] = v;
+        if constexpr (ROWMAJOR && sizeof(R) == 8) {
+            if (A.Z32) ((float2 *)A.Z32)[(size_t)m * A.ld32 + n] = float2{(float)v.r, (float)v.i};
+        }
     }
 }
 
@@ -1284,10 +1318,12 @@ void make_plan(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, Mesh &tm
 
 template <typename R>
 void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long long ld, bool rowmajor, bool accumulate,
-                  double sr, double si)
+                  double sr, double si, void *Z32 = nullptr, long long ld32 = 0)
 {
     TileArgs A{};
     A.far = P.far;
+    A.Z32 = Z32;
+    A.ld32 = ld32;
     A.row = {X.t_ptr.p, X.t_cells.p, X.t_slot.p, X.t_coef.p, (int)X.ndofs};
     A.col = {Y.t_ptr.p, Y.t_cells.p, Y.t_slot.p, Y.t_coef.p, (int)Y.ndofs};
     A.Z = Z;
@@ -1405,23 +1441,25 @@ void assemble_ncross(Ctx &ctx, const bh_quadstrat *qs, bool fp32, Space &X, Spac
     BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
 }
 
-void assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ,
-                     long long ld, bool rowmajor, bool accumulate, double sr, double si)
+bool assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ,
+                     long long ld, bool rowmajor, bool accumulate, double sr, double si, void *dZ32, long long ld32)
 {
     BH_REQUIRE(op != nullptr, "operator is NULL");
     if (op->kind == BH_OP_NCROSS) {
-        if (X.ndofs == 0 || Y.ndofs == 0) return;
+        if (X.ndofs == 0 || Y.ndofs == 0) return false;
         assemble_ncross(ctx, qs, fp32, X, Y, dZ, ld, rowmajor, accumulate, sr, si);
-        return;
+        return false;
     }
     Plan P;
     make_plan(ctx, op, qs, *X.mesh, *Y.mesh, fp32, P);
     build_tiles(X, ctx.stream);
     build_tiles(Y, ctx.stream);
-    if (X.ndofs == 0 || Y.ndofs == 0) return;
+    if (X.ndofs == 0 || Y.ndofs == 0) return false;
+    const bool copy32 = dZ32 != nullptr && !fp32 && rowmajor;
     if (fp32) launch_tiles<float>(ctx, P, X, Y, dZ, ld, rowmajor, accumulate, sr, si);
-    else launch_tiles<double>(ctx, P, X, Y, dZ, ld, rowmajor, accumulate, sr, si);
+    else launch_tiles<double>(ctx, P, X, Y, dZ, ld, rowmajor, accumulate, sr, si, copy32 ? dZ32 : nullptr, ld32);
     BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
+    return copy32;
 }
 
 }  // namespace bh

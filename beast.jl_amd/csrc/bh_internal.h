@@ -112,8 +112,11 @@ struct Ctx;
 struct Mesh;
 // quadrature points of a triangle rule on all cells of a mesh (cached): [nf][Q][4]
 const void *quad_points(Mesh &m, int degree, int *Q, bool fp32, hipStream_t s);
-void assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ,
-                     long long ld, bool rowmajor, bool accumulate, double sr, double si);
+// dZ32 (optional, FP64 row-major assembly only): a row-major complex-float copy
+// with leading dimension ld32 written by the same launch.  Returns whether it was.
+bool assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ,
+                     long long ld, bool rowmajor, bool accumulate, double sr, double si, void *dZ32 = nullptr,
+                     long long ld32 = 0);
 
 // Per-kernel timing with HIP events on the context stream (measurement only).
 struct KernelStat {

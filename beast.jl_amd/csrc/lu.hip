@@ -2304,6 +2304,7 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
     BH_CHECK_HIP(hipEventRecord(e0, ctx.stream));
     if (precision == BH_FP64) {
         M->lu32.release();
+        M->lu32_fresh = false;
         M->dinv32.release();
         M->lu64.alloc((size_t)lda * n * 2);
         k_to_work<double><<<cgrid, 256, 0, ctx.stream>>>((const double2 *)M->a64.p, (int)n, (double2 *)M->lu64.p, lda);
@@ -2311,9 +2312,16 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
         lu_factor<double>(ctx, (double2 *)M->lu64.p, lda, (int)n, M->ipiv.p, &info);
     } else {
         M->lu64.release();
-        M->lu32.alloc((size_t)lda * n * 2);
-        k_to_work<float><<<cgrid, 256, 0, ctx.stream>>>((const double2 *)M->a64.p, (int)n, (float2 *)M->lu32.p, lda);
-        check("k_to_work");
+        if (M->lu32.n != (size_t)lda * n * 2) {
+            M->lu32.alloc((size_t)lda * n * 2);
+            M->lu32_fresh = false;
+        }
+        if (!M->lu32_fresh) {   // else the assembly wrote the FP32 copy
+            k_to_work<float><<<cgrid, 256, 0, ctx.stream>>>((const double2 *)M->a64.p, (int)n, (float2 *)M->lu32.p,
+                                                            lda);
+            check("k_to_work");
+        }
+        M->lu32_fresh = false;   // it now holds the factors
         lu_factor<float>(ctx, (float2 *)M->lu32.p, lda, (int)n, M->ipiv.p, &info, precision == BH_FP32_SPLIT);
         if (info == 0) {
             const size_t nb = (size_t)(n + kIB - 1) / kIB;

@@ -56,7 +56,12 @@ extern "C" int bh_matrix_assemble(bh_ctx *ctx, const bh_operator *op, const bh_q
     M->n = X->ndofs;
     M->a64.alloc((size_t)M->n * M->n * 2);
     bh_space *sp = const_cast<bh_space *>(X);
-    assemble_device(*ctx, op, qs, false, *sp, *sp, M->a64.p, M->n, true, false, 1.0, 0.0);
+    // the tile kernel also writes the FP32 copy that an FP32 factorisation starts from
+    const long long lda = padded_ld(M->n);
+    if (M->n > 0) M->lu32.alloc((size_t)lda * M->n * 2);
+    M->lu32_fresh = assemble_device(*ctx, op, qs, false, *sp, *sp, M->a64.p, M->n, true, false, 1.0, 0.0,
+                                    M->n > 0 ? M->lu32.p : nullptr, lda);
+    if (!M->lu32_fresh) M->lu32.release();
     ctx_retain(ctx);
     *out = M.release();
     BH_API_END
@@ -75,6 +80,7 @@ extern "C" int bh_matrix_accumulate(bh_matrix *M, const bh_operator *op, const b
     const double sr = scale ? scale[0] : 1.0, si = scale ? scale[1] : 0.0;
     assemble_device(*M->ctx, op, qs, false, *sp, *sp, M->a64.p, M->n, true, true, sr, si);
     M->factored = -1;   // earlier factors describe the old matrix
+    M->lu32_fresh = false;
     BH_API_END
 }
 
