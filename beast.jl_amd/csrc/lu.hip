@@ -1855,6 +1855,7 @@ struct LUWork {
     DevBuf<int> plan_side;                 // row-interchange plan of the block-column stream
     DevBuf<__bf16> pa_side, pb_side;       // split planes of the block-column stream
     DevBuf<__bf16> pa_u12;                 // split planes of L for the block row
+    DevBuf<float2> eblk;                   // I - inv(L_bb) of the 128-row diagonal blocks of a block row
 };
 
 template <typename T, int TM, int TN, bool M3, int WPE>
@@ -2038,6 +2039,31 @@ void factor_block(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, l
     }
 }
 
+// E = I - S for kb/kIB blocks S = inv(L_bb) as k_trinv<true> leaves them (unit
+// diagonal, zero above it): zero diagonal, negated strictly lower part.
+__global__ void k_inv_to_e(float2 *E, int nblk)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nblk * kIB * kIB) return;
+    const int i = (int)(idx / kIB % kIB), j = (int)(idx % kIB);
+    const float2 v = E[idx];
+    E[idx] = i == j ? float2{0, 0} : float2{-v.x, -v.y};
+}
+
+// Rows [r, r + kIB) of a block row, columns [c0, c1): C <- inv(L_bb) C, formed as
+// C - E C with E = I - inv(L_bb).  The product has one tile row (M = kIB = BM), so
+// each workgroup reads, as B, exactly the columns of C it writes, and reads all of
+// them (main loop, then the epilogue's reads of C) before its first write.
+void blockrow_apply_inverse(Ctx &ctx, hipStream_t s, const float2 *E, float2 *A, long long lda, int r, int c0, int c1)
+{
+    const int N = c1 - c0;
+    static_assert(32 * 4 == kIB, "one tile row of the FP32 product must cover the block");
+    ProfScope prof(ctx, "k_cgemm", 8.0 * kIB * (double)N * kIB, s);
+    float2 *C = A + (long long)r * lda + c0;
+    k_cgemm<float, 4, 2, true, 2><<<(unsigned)((N + 63) / 64), 256, 0, s>>>(E, kIB, C, lda, C, lda, kIB, N, kIB);
+    check("k_cgemm");
+}
+
 // U12 = L11^-1 A12 for block row [k, k + kb) over the columns [c0, n).
 template <typename T>
 void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, long long lda, int n, int k, int kb,
@@ -2064,6 +2090,23 @@ void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, 
             ProfScope prof(ctx, "k_blockrow_trsm", 4.0 * kb * (double)kb * ncols, s);   // kb^2 / 2 complex MACs per column
             k_blockrow_trsm<<<(ncols + kStripW - 1) / kStripW, 256, 0, s>>>(A, lda, k, kb, c0, ncols);
             check("k_blockrow_trsm");
+            return;
+        }
+    }
+    if constexpr (sizeof(T) == 4) {
+        // wide FP32 block rows by 128-row blocks: the inverses of L11's diagonal blocks,
+        // then per block one product with I - inv(L_bb) and one K = 128 update below it
+        if (kb % kIB == 0 && kb <= kNBInnerMax) {
+            const int nb = kb / kIB;
+            k_trinv<true><<<nb, kIB, 0, s>>>(A + (long long)k * lda + k, lda, kb, W.eblk.p);
+            check("k_trinv");
+            k_inv_to_e<<<(unsigned)(((long long)nb * kIB * kIB + 255) / 256), 256, 0, s>>>(W.eblk.p, nb);
+            check("k_inv_to_e");
+            for (int b = 0; b < nb; ++b) {
+                const int r = k + b * kIB;
+                blockrow_apply_inverse(ctx, s, W.eblk.p + (size_t)b * kIB * kIB, A, lda, r, c0, c1);
+                if (b + 1 < nb) gemm<T>(ctx, s, A, lda, r + kIB, r, r, c0, r + kIB, c0, k + kb - (r + kIB), c1 - c0, kIB);
+            }
             return;
         }
     }
@@ -2143,6 +2186,7 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
         W.top.alloc((size_t)kPW * kPW);
         W.oldtop.alloc((size_t)kPW * kPW);
         W.tplan.alloc(1 + 2 * kPlanMax);
+        W.eblk.alloc((size_t)(kNBInnerMax / kIB) * kIB * kIB);
         hipStream_t s2 = nullptr;
         hipEvent_t ready = nullptr, done = nullptr;
         BH_CHECK_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
