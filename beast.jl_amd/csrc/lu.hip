@@ -1855,7 +1855,8 @@ struct LUWork {
     DevBuf<int> plan_side;                 // row-interchange plan of the block-column stream
     DevBuf<__bf16> pa_side, pb_side;       // split planes of the block-column stream
     DevBuf<__bf16> pa_u12;                 // split planes of L for the block row
-    DevBuf<float2> eblk;                   // I - inv(L_bb) of the 128-row diagonal blocks of a block row
+    DevBuf<float2> eblk, eblk_side;        // I - inv(L_bb) of the 128-row diagonal blocks of a block row
+                                           // (context stream, block-column stream)
 };
 
 template <typename T, int TM, int TN, bool M3, int WPE>
@@ -2081,32 +2082,32 @@ void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, 
         }
     }
     if constexpr (sizeof(T) == 4) {
-        // narrow block rows in one launch (latency-bound as separate 32-row steps);
-        // wide ones, and block rows that run beside the block-column chain (the
-        // one-launch kernel would take the whole device from it), keep the MFMA
-        // updates below
+        // FP32 block rows by 128-row blocks: the inverses of L11's diagonal blocks, then
+        // per block one product with I - inv(L_bb) and one K = 128 update below it
+        // (measured faster than the one-launch VALU solve at every width)
+        if (kb % kIB == 0 && kb <= kNBInnerMax) {
+            const int nb = kb / kIB;
+            float2 *E = (s == ctx.stream ? W.eblk : W.eblk_side).p;
+            k_trinv<true><<<nb, kIB, 0, s>>>(A + (long long)k * lda + k, lda, kb, E);
+            check("k_trinv");
+            k_inv_to_e<<<(unsigned)(((long long)nb * kIB * kIB + 255) / 256), 256, 0, s>>>(E, nb);
+            check("k_inv_to_e");
+            for (int b = 0; b < nb; ++b) {
+                const int r = k + b * kIB;
+                blockrow_apply_inverse(ctx, s, E + (size_t)b * kIB * kIB, A, lda, r, c0, c1);
+                if (b + 1 < nb) gemm<T>(ctx, s, A, lda, r + kIB, r, r, c0, r + kIB, c0, k + kb - (r + kIB), c1 - c0, kIB);
+            }
+            return;
+        }
+        // other depths: narrow block rows in one launch (latency-bound as separate
+        // 32-row steps); wide ones, and block rows that run beside the block-column
+        // chain (the one-launch kernel would take the whole device from it), keep the
+        // MFMA updates below
         const int ncols = c1 - c0;
         if (one_launch && kb <= kNBInnerMax && ncols <= kBlockRowOneLaunch) {
             ProfScope prof(ctx, "k_blockrow_trsm", 4.0 * kb * (double)kb * ncols, s);   // kb^2 / 2 complex MACs per column
             k_blockrow_trsm<<<(ncols + kStripW - 1) / kStripW, 256, 0, s>>>(A, lda, k, kb, c0, ncols);
             check("k_blockrow_trsm");
-            return;
-        }
-    }
-    if constexpr (sizeof(T) == 4) {
-        // wide FP32 block rows by 128-row blocks: the inverses of L11's diagonal blocks,
-        // then per block one product with I - inv(L_bb) and one K = 128 update below it
-        if (kb % kIB == 0 && kb <= kNBInnerMax) {
-            const int nb = kb / kIB;
-            k_trinv<true><<<nb, kIB, 0, s>>>(A + (long long)k * lda + k, lda, kb, W.eblk.p);
-            check("k_trinv");
-            k_inv_to_e<<<(unsigned)(((long long)nb * kIB * kIB + 255) / 256), 256, 0, s>>>(W.eblk.p, nb);
-            check("k_inv_to_e");
-            for (int b = 0; b < nb; ++b) {
-                const int r = k + b * kIB;
-                blockrow_apply_inverse(ctx, s, W.eblk.p + (size_t)b * kIB * kIB, A, lda, r, c0, c1);
-                if (b + 1 < nb) gemm<T>(ctx, s, A, lda, r + kIB, r, r, c0, r + kIB, c0, k + kb - (r + kIB), c1 - c0, kIB);
-            }
             return;
         }
     }
@@ -2187,6 +2188,7 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
         W.oldtop.alloc((size_t)kPW * kPW);
         W.tplan.alloc(1 + 2 * kPlanMax);
         W.eblk.alloc((size_t)(kNBInnerMax / kIB) * kIB * kIB);
+        W.eblk_side.alloc((size_t)(kNBInnerMax / kIB) * kIB * kIB);
         hipStream_t s2 = nullptr;
         hipEvent_t ready = nullptr, done = nullptr;
         BH_CHECK_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
