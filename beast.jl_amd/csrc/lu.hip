@@ -1427,18 +1427,20 @@ __global__ __launch_bounds__(256) void k_trsv_step(const typename CT<T>::t *A, l
 // factorisation): each block step is then two small matrix-vector products with
 // no serial sweep, one launch per block.
 constexpr int kIB = 128;
+constexpr int kTrinvParts = 4;   // lanes per column in k_trinv
 
 // In-place inverse of diagonal block blockIdx.x of the unit-lower (LOWER) or
 // upper factor, row by row in LDS: X[i][t] = ([i == t] - sum_k T[i][k] X[k][t]) / T[i][i]
 // over the rows k already inverted (entries outside the triangle are zero).
-// One thread per column; rows past the end of the matrix become identity rows.
+// kTrinvParts threads per column share each row's dot product; rows past the end
+// of the matrix become identity rows.
 template <bool LOWER>
-__global__ __launch_bounds__(kIB) void k_trinv(const float2 *A, long long lda, int n, float2 *Dinv)
+__global__ __launch_bounds__(kIB * kTrinvParts) void k_trinv(const float2 *A, long long lda, int n, float2 *Dinv)
 {
     __shared__ float2 S[kIB * kIB];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, col = t / kTrinvParts, part = t % kTrinvParts;
     const int b0 = blockIdx.x * kIB, bs = min(kIB, n - b0);
-    for (int idx = t; idx < kIB * kIB; idx += kIB) {
+    for (int idx = t; idx < kIB * kIB; idx += kIB * kTrinvParts) {
         const int i = idx / kIB, j = idx % kIB;
         float2 v{0, 0};
         if (i < bs && j < bs && (LOWER ? j < i : j >= i)) v = A[(long long)(b0 + i) * lda + b0 + j];
@@ -1448,34 +1450,43 @@ __global__ __launch_bounds__(kIB) void k_trinv(const float2 *A, long long lda, i
     __syncthreads();
     for (int s = LOWER ? 1 : 0; s < kIB; ++s) {
         const int i = LOWER ? s : kIB - 1 - s;
+        // the kTrinvParts lanes of a column (consecutive, one wave) each sum every
+        // kTrinvParts-th term of the row's dot product
         float2 acc{0, 0};
         if (LOWER) {
-            for (int k = 0; k < i; ++k) {
-                const float2 a = S[i * kIB + k], x = S[k * kIB + t];
+            for (int k = part; k < i; k += kTrinvParts) {
+                const float2 a = S[i * kIB + k], x = S[k * kIB + col];
                 acc.x += a.x * x.x - a.y * x.y;
                 acc.y += a.x * x.y + a.y * x.x;
             }
         } else {
-            for (int k = i + 1; k < kIB; ++k) {
-                const float2 a = S[i * kIB + k], x = S[k * kIB + t];
+            for (int k = i + 1 + part; k < kIB; k += kTrinvParts) {
+                const float2 a = S[i * kIB + k], x = S[k * kIB + col];
                 acc.x += a.x * x.x - a.y * x.y;
                 acc.y += a.x * x.y + a.y * x.x;
             }
         }
+#pragma unroll
+        for (int o = 1; o < kTrinvParts; o <<= 1) {
+            acc.x += __shfl_xor(acc.x, o, 64);
+            acc.y += __shfl_xor(acc.y, o, 64);
+        }
         const float2 d = S[i * kIB + i];
         __syncthreads();
-        if (LOWER) {
-            if (t < i) S[i * kIB + t] = {-acc.x, -acc.y};
-        } else {
-            const float den = d.x * d.x + d.y * d.y;
-            const float2 inv{d.x / den, -d.y / den};
-            if (t > i) S[i * kIB + t] = {-(acc.x * inv.x - acc.y * inv.y), -(acc.x * inv.y + acc.y * inv.x)};
-            else if (t == i) S[i * kIB + i] = inv;
+        if (part == 0) {
+            if (LOWER) {
+                if (col < i) S[i * kIB + col] = {-acc.x, -acc.y};
+            } else {
+                const float den = d.x * d.x + d.y * d.y;
+                const float2 inv{d.x / den, -d.y / den};
+                if (col > i) S[i * kIB + col] = {-(acc.x * inv.x - acc.y * inv.y), -(acc.x * inv.y + acc.y * inv.x)};
+                else if (col == i) S[i * kIB + i] = inv;
+            }
         }
         __syncthreads();
     }
     float2 *out = Dinv + (size_t)blockIdx.x * kIB * kIB;
-    for (int idx = t; idx < kIB * kIB; idx += kIB) out[idx] = S[idx];
+    for (int idx = t; idx < kIB * kIB; idx += kIB * kTrinvParts) out[idx] = S[idx];
 }
 
 // out[i] = (SUB ? out[i] : 0) - / + sum_c Mx[i][c] xin[c] for rows i < rs, columns
@@ -2088,7 +2099,7 @@ void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, 
         if (kb % kIB == 0 && kb <= kNBInnerMax) {
             const int nb = kb / kIB;
             float2 *E = (s == ctx.stream ? W.eblk : W.eblk_side).p;
-            k_trinv<true><<<nb, kIB, 0, s>>>(A + (long long)k * lda + k, lda, kb, E);
+            k_trinv<true><<<nb, kIB * kTrinvParts, 0, s>>>(A + (long long)k * lda + k, lda, kb, E);
             check("k_trinv");
             k_inv_to_e<<<(unsigned)(((long long)nb * kIB * kIB + 255) / 256), 256, 0, s>>>(E, nb);
             check("k_inv_to_e");
@@ -2271,9 +2282,9 @@ void invert_diagonal_blocks(hipStream_t s, const float2 *LU, long long lda, int 
 {
     const int nb = (n + kIB - 1) / kIB;
     const size_t bsz = (size_t)nb * kIB * kIB;
-    k_trinv<true><<<nb, kIB, 0, s>>>(LU, lda, n, Dinv);
+    k_trinv<true><<<nb, kIB * kTrinvParts, 0, s>>>(LU, lda, n, Dinv);
     check("k_trinv");
-    k_trinv<false><<<nb, kIB, 0, s>>>(LU, lda, n, Dinv + bsz);
+    k_trinv<false><<<nb, kIB * kTrinvParts, 0, s>>>(LU, lda, n, Dinv + bsz);
     check("k_trinv");
     k_blk_coupling<true><<<dim3(nb, kIB / 16), 256, 0, s>>>(LU, lda, n, Dinv, Dinv + 2 * bsz);
     check("k_blk_coupling");
