@@ -1868,6 +1868,7 @@ struct LUWork {
     DevBuf<__bf16> pa_u12;                 // split planes of L for the block row
     DevBuf<float2> eblk, eblk_side;        // I - inv(L_bb) of the 128-row diagonal blocks of a block row
                                            // (context stream, block-column stream)
+    int eblk_of = -1, eblk_side_of = -1;   // first row of the block row whose inverses they hold
 };
 
 template <typename T, int TM, int TN, bool M3, int WPE>
@@ -2098,11 +2099,16 @@ void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, 
         // (measured faster than the one-launch VALU solve at every width)
         if (kb % kIB == 0 && kb <= kNBInnerMax) {
             const int nb = kb / kIB;
-            float2 *E = (s == ctx.stream ? W.eblk : W.eblk_side).p;
-            k_trinv<true><<<nb, kIB * kTrinvParts, 0, s>>>(A + (long long)k * lda + k, lda, kb, E);
-            check("k_trinv");
-            k_inv_to_e<<<(unsigned)(((long long)nb * kIB * kIB + 255) / 256), 256, 0, s>>>(E, nb);
-            check("k_inv_to_e");
+            const bool side = s != ctx.stream;
+            float2 *E = (side ? W.eblk_side : W.eblk).p;
+            int &of = side ? W.eblk_side_of : W.eblk_of;
+            if (of != k) {   // a block row taken in two column ranges reuses its inverses
+                k_trinv<true><<<nb, kIB * kTrinvParts, 0, s>>>(A + (long long)k * lda + k, lda, kb, E);
+                check("k_trinv");
+                k_inv_to_e<<<(unsigned)(((long long)nb * kIB * kIB + 255) / 256), 256, 0, s>>>(E, nb);
+                check("k_inv_to_e");
+                of = k;
+            }
             for (int b = 0; b < nb; ++b) {
                 const int r = k + b * kIB;
                 blockrow_apply_inverse(ctx, s, E + (size_t)b * kIB * kIB, A, lda, r, c0, c1);
@@ -2136,9 +2142,14 @@ void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, 
 // block column is updated first and factored on a second stream while the rest
 // of the trailing matrix is updated on the context stream; the two touch
 // disjoint columns.
+void invert_diagonal_blocks(hipStream_t s, const float2 *LU, long long lda, int n, float2 *Dinv);
+
+// dinv (FP32 only, may be NULL): the diagonal-block inverses and couplings of the
+// factors are formed on the context stream before the workspace is released, so
+// the device does not idle while the host frees it.
 template <typename T>
 void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *ipiv, int *info_host,
-               bool split = false)
+               bool split = false, float2 *dinv = nullptr)
 {
     hipStream_t s = ctx.stream;
     LUWork<T> W;
@@ -2253,6 +2264,9 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
             trailing(s, k, kb, nk + nkb, n);
             BH_CHECK_HIP(hipStreamWaitEvent(s, done, 0));
         }
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (dinv) invert_diagonal_blocks(s, (const float2 *)A, lda, n, dinv);
     }
     BH_CHECK_HIP(hipMemcpyAsync(info_host, W.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
     BH_CHECK_HIP(hipStreamSynchronize(s));
@@ -2379,12 +2393,11 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
             check("k_to_work");
         }
         M->lu32_fresh = false;   // it now holds the factors
-        lu_factor<float>(ctx, (float2 *)M->lu32.p, lda, (int)n, M->ipiv.p, &info, precision == BH_FP32_SPLIT);
-        if (info == 0) {
-            const size_t nb = (size_t)(n + kIB - 1) / kIB;
+        const size_t nb = (size_t)(n + kIB - 1) / kIB;
+        if (M->dinv32.n != 4 * nb * kIB * kIB * 2)
             M->dinv32.alloc(4 * nb * kIB * kIB * 2);   // L and U diagonal inverses, then their couplings
-            invert_diagonal_blocks(ctx.stream, (const float2 *)M->lu32.p, lda, (int)n, (float2 *)M->dinv32.p);
-        }
+        lu_factor<float>(ctx, (float2 *)M->lu32.p, lda, (int)n, M->ipiv.p, &info, precision == BH_FP32_SPLIT,
+                         (float2 *)M->dinv32.p);
     }
     BH_CHECK_HIP(hipEventRecord(e1, ctx.stream));
     BH_CHECK_HIP(hipEventSynchronize(e1));
