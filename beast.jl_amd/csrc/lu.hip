@@ -1432,13 +1432,16 @@ constexpr int kTrinvParts = 4;   // lanes per column in k_trinv
 // In-place inverse of diagonal block blockIdx.x of the unit-lower (LOWER) or
 // upper factor, row by row in LDS: X[i][t] = ([i == t] - sum_k T[i][k] X[k][t]) / T[i][i]
 // over the rows k already inverted (entries outside the triangle are zero).
-// kTrinvParts threads per column share each row's dot product; rows past the end
-// of the matrix become identity rows.
+// kTrinvParts threads per column share each row's dot product (partial sums
+// combined through LDS); rows past the end of the matrix become identity rows.
 template <bool LOWER>
 __global__ __launch_bounds__(kIB * kTrinvParts) void k_trinv(const float2 *A, long long lda, int n, float2 *Dinv)
 {
     __shared__ float2 S[kIB * kIB];
-    const int t = threadIdx.x, col = t / kTrinvParts, part = t % kTrinvParts;
+    __shared__ float2 red[kTrinvParts][kIB];
+    // part = t / kIB: the waves of a part read whole rows of S (consecutive columns,
+    // no bank conflicts); part p sums the terms k = p mod kTrinvParts of each dot product
+    const int t = threadIdx.x, col = t % kIB, part = t / kIB;
     const int b0 = blockIdx.x * kIB, bs = min(kIB, n - b0);
     for (int idx = t; idx < kIB * kIB; idx += kIB * kTrinvParts) {
         const int i = idx / kIB, j = idx % kIB;
@@ -1450,30 +1453,29 @@ __global__ __launch_bounds__(kIB * kTrinvParts) void k_trinv(const float2 *A, lo
     __syncthreads();
     for (int s = LOWER ? 1 : 0; s < kIB; ++s) {
         const int i = LOWER ? s : kIB - 1 - s;
-        // the kTrinvParts lanes of a column (consecutive, one wave) each sum every
-        // kTrinvParts-th term of the row's dot product
-        float2 acc{0, 0};
-        if (LOWER) {
-            for (int k = part; k < i; k += kTrinvParts) {
-                const float2 a = S[i * kIB + k], x = S[k * kIB + col];
-                acc.x += a.x * x.x - a.y * x.y;
-                acc.y += a.x * x.y + a.y * x.x;
-            }
-        } else {
-            for (int k = i + 1 + part; k < kIB; k += kTrinvParts) {
-                const float2 a = S[i * kIB + k], x = S[k * kIB + col];
-                acc.x += a.x * x.x - a.y * x.y;
-                acc.y += a.x * x.y + a.y * x.x;
-            }
+        const int k0 = LOWER ? part : i + 1 + part, k1 = LOWER ? i : kIB;
+        float2 u{0, 0}, w{0, 0};   // two chains, so consecutive terms overlap
+        int k = k0;
+        for (; k + kTrinvParts < k1; k += 2 * kTrinvParts) {
+            const float2 a = S[i * kIB + k], x = S[k * kIB + col];
+            const float2 b = S[i * kIB + k + kTrinvParts], y = S[(k + kTrinvParts) * kIB + col];
+            u.x += a.x * x.x - a.y * x.y;
+            u.y += a.x * x.y + a.y * x.x;
+            w.x += b.x * y.x - b.y * y.y;
+            w.y += b.x * y.y + b.y * y.x;
         }
-#pragma unroll
-        for (int o = 1; o < kTrinvParts; o <<= 1) {
-            acc.x += __shfl_xor(acc.x, o, 64);
-            acc.y += __shfl_xor(acc.y, o, 64);
+        if (k < k1) {
+            const float2 a = S[i * kIB + k], x = S[k * kIB + col];
+            u.x += a.x * x.x - a.y * x.y;
+            u.y += a.x * x.y + a.y * x.x;
         }
+        red[part][col] = {u.x + w.x, u.y + w.y};
         const float2 d = S[i * kIB + i];
         __syncthreads();
         if (part == 0) {
+            float2 acc = red[0][col];
+#pragma unroll
+            for (int p = 1; p < kTrinvParts; ++p) { acc.x += red[p][col].x; acc.y += red[p][col].y; }
             if (LOWER) {
                 if (col < i) S[i * kIB + col] = {-acc.x, -acc.y};
             } else {
