@@ -2105,6 +2105,7 @@ void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, 
             float2 *E = (side ? W.eblk_side : W.eblk).p;
             int &of = side ? W.eblk_side_of : W.eblk_of;
             if (of != k) {   // a block row taken in two column ranges reuses its inverses
+                ProfScope prof(ctx, "k_trinv", 4.0 * kIB * (double)kIB * kIB * nb / 3.0, s);   // kIB^3/6 complex MACs per block
                 k_trinv<true><<<nb, kIB * kTrinvParts, 0, s>>>(A + (long long)k * lda + k, lda, kb, E);
                 check("k_trinv");
                 k_inv_to_e<<<(unsigned)(((long long)nb * kIB * kIB + 255) / 256), 256, 0, s>>>(E, nb);

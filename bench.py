@@ -193,8 +193,9 @@ def main():
     step()
     ctx.synchronize()
     breakdown = {}
-    for name in ("k_far_tiles", "k_classify_count", "k_sauter_schwab", "k_wilton", dom, "k_split", "k_tslu_panel", "k_blockrow_trsm",
-                 "k_tslu_apply", "k_laswp", "k_cgemm", "k_trsv_inv", "k_matvec", "k_residual"):
+    for name in ("k_far_tiles", "k_classify_count", "k_sauter_schwab", "k_wilton", dom, "k_split", "k_tslu_panel",
+                 "k_blockrow_trsm", "k_trinv", "k_tslu_apply", "k_laswp", "k_cgemm", "k_trsv_inv", "k_matvec",
+                 "k_residual"):
         n_, s_, w_ = ctx.kernel_stats(name)
         if n_:
             breakdown[name] = {"launches": n_, "ms": round(1e3 * s_, 3)}
