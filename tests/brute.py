@@ -133,3 +133,33 @@ def farfield_magnitudes(bj, u, X, thetas, gamma=1.0j, omega=1.0, c=1.0, degree=8
 # test/test_farfield.jl:34-40 (sphere824.in = sphere2.in, EFIE solution at kappa = 1)
 FARFIELD_GOLDEN = [0.6426645162880508, 0.5222644345134998, 0.28287430186444407, 0.3873439919314161,
                    0.6788129041674158, 0.8800967217249486, 0.9487895695366841]
+
+
+def mie_pec_sphere(ka, thetas):
+    """Scattering amplitudes S1, S2 of a perfectly conducting sphere (Bohren & Huffman,
+    "Absorption and Scattering of Light by Small Particles", 1983, Sec. 4.4, with the
+    conductor limit of the coefficients): a_n = psi_n'(x)/xi_n'(x), b_n = psi_n(x)/xi_n(x),
+    psi_n = x j_n(x), xi_n = x h_n(x), x = ka, and the angular functions pi_n, tau_n by
+    their upward recurrences.  For an x-polarised plane wave along z the far field
+    r |E_s| is |S2(theta)|/k in the plane phi = 0 and |S1(theta)|/k in phi = 90 degrees."""
+    from scipy.special import spherical_jn, spherical_yn
+    x = float(ka)
+    nmax = int(x + 4 * x ** (1 / 3) + 12)
+    n = np.arange(1, nmax + 1)
+    jn, jnp = spherical_jn(n, x), spherical_jn(n, x, derivative=True)
+    yn, ynp = spherical_yn(n, x), spherical_yn(n, x, derivative=True)
+    hn, hnp = jn + 1j * yn, jnp + 1j * ynp
+    a = (jn + x * jnp) / (hn + x * hnp)
+    b = (x * jn) / (x * hn)
+    c = (2 * n + 1) / (n * (n + 1))
+    S1, S2 = [], []
+    for th in thetas:
+        mu = np.cos(th)
+        pi = np.zeros(nmax + 1)
+        pi[1] = 1.0
+        for k in range(2, nmax + 1):
+            pi[k] = (2 * k - 1) / (k - 1) * mu * pi[k - 1] - k / (k - 1) * pi[k - 2]
+        tau = n * mu * pi[1:] - (n + 1) * pi[:-1]
+        S1.append(np.sum(c * (a * pi[1:] + b * tau)))
+        S2.append(np.sum(c * (a * tau + b * pi[1:])))
+    return np.array(S1), np.array(S2)

@@ -141,6 +141,44 @@ def test_oracle_farfield_golden(bj, orc, golden):
     assert np.abs(a - nb).max() < 1e-12
 
 
+def test_mie_series_pins():
+    """The Mie comparator itself: the Rayleigh limit of a small conducting sphere
+    (backscatter |S|/k -> (3/2) k^2 a^3 ... with a_1 -> -2i x^3/3, b_1 -> i x^3/3), and
+    the reference's golden magnitudes (test_farfield.jl:34-40, its own 824-triangle
+    solution) within the 2% of that mesh."""
+    x = 1e-3
+    S1, S2 = brute.mie_pec_sphere(x, [np.pi])
+    # theta = pi: pi_1 = 1, tau_1 = -1: S = (3/2)(a_1 - b_1) -> (3/2)(-i x^3) for both
+    assert abs(abs(S1[0]) - 1.5 * x ** 3) < 1e-6 * x ** 3
+    assert abs(abs(S2[0]) - 1.5 * x ** 3) < 1e-6 * x ** 3
+    th = np.linspace(0, np.pi, 7)
+    _, S2 = brute.mie_pec_sphere(1.0, th)
+    assert np.abs(np.abs(S2) - np.array(brute.FARFIELD_GOLDEN)).max() < 0.02 * np.abs(S2).max()
+
+
+@pytest.mark.parametrize("nv", [400, 1200])
+def test_oracle_farfield_mie(bj, orc, nv):
+    """EFIE on a unit sphere at k = 1 (oracle assembly, LAPACK solve, oracle far field)
+    against the PEC Mie series in the E and H planes (r|E_s| = |S|/k, k = 1 here); the
+    agreement improves with the mesh (the far field converges to the exact scattered
+    field)."""
+    m = bj.meshsphere(1.0, nv=nv)
+    X = bj.raviartthomas(m)
+    t = bj.Maxwell3D.singlelayer(wavenumber=1.0)
+    Z = orc.assemble(0, t.gamma, t.alpha, t.beta, 0, (2, 3, 6, 7, 5, 5, 4, 3), X, X)
+    b = orc.planewave(X, [0, 0, 1.0], [1.0, 0, 0], 1.0j)
+    u = orc.lu_solve(Z, b)
+    th = np.linspace(0.05, np.pi - 0.05, 9)
+    S1, S2 = brute.mie_pec_sphere(1.0, th)
+    E = np.stack([np.sin(th), 0 * th, np.cos(th)], 1)
+    H = np.stack([0 * th, np.sin(th), np.cos(th)], 1)
+    fe = np.linalg.norm(orc.farfield(0, 1.0j, 1.0, E, u, X), axis=1) / (4 * np.pi)
+    fh = np.linalg.norm(orc.farfield(0, 1.0j, 1.0, H, u, X), axis=1) / (4 * np.pi)
+    err = max(np.abs(fe - np.abs(S2)).max(), np.abs(fh - np.abs(S1)).max()) / np.abs(S2).max()
+    print(f"nv={nv} cells={m.numcells}: max Mie error {err:.2e}")
+    assert err < (0.015 if nv == 400 else 0.005)
+
+
 def test_farfield_constructors(bj):
     """farfield.jl:22-53,82-103 and test_handlers.jl:72-74."""
     for cls in (bj.MWFarField3D, bj.MWDoubleLayerFarField3D, bj.MWDoubleLayerRotatedFarField3D):

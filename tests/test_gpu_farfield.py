@@ -120,3 +120,28 @@ def test_farfield_golden_through_gpu(bj, golden):
     ff = bj.potential(bj.MWFarField3D(1.0j), pts, u, X) * (1j / (4 * np.pi))
     a = np.linalg.norm(ff, axis=1)
     assert np.abs(a - np.array(brute.FARFIELD_GOLDEN)).max() < 1e-3
+
+
+def test_farfield_mie_through_gpu(bj):
+    """EFIE on the 26664-triangle unit sphere at k = 2 fully on the GPU (assembly,
+    right-hand side, mixed solve, far field) against the PEC Mie series, r|E_s| =
+    |S2|/k in the E plane and |S1|/k in the H plane (i omega / (4 pi c) scaling of
+    test_farfield.jl:26 with omega = k, c = 1)."""
+    k = 2.0
+    m = bj.meshsphere(1.0, nv=13334)
+    X = bj.raviartthomas(m)
+    t = bj.Maxwell3D.singlelayer(wavenumber=k)
+    E = bj.Maxwell3D.planewave(direction=bj.z_hat, polarization=bj.x_hat, wavenumber=k)
+    b = bj.assemble(bj.cross(bj.cross(bj.n, E), bj.n), X)
+    u = bj.DeviceMatrix.assemble(t, X).factor("fp32split").solve(b, tol=1e-11)
+    th = np.linspace(0.05, np.pi - 0.05, 13)
+    S1, S2 = brute.mie_pec_sphere(k, th)
+    Ye = np.stack([np.sin(th), 0 * th, np.cos(th)], 1)
+    Yh = np.stack([0 * th, np.sin(th), np.cos(th)], 1)
+    ff = bj.MWFarField3D(wavenumber=k)
+    fe = np.linalg.norm(bj.potential(ff, Ye, u, X), axis=1) * k / (4 * np.pi)
+    fh = np.linalg.norm(bj.potential(ff, Yh, u, X), axis=1) * k / (4 * np.pi)
+    ref = np.abs(S2).max() / k
+    err = max(np.abs(fe - np.abs(S2) / k).max(), np.abs(fh - np.abs(S1) / k).max()) / ref
+    print(f"26664 cells, k = 2: max Mie error {err:.2e}")
+    assert err < 1e-3
