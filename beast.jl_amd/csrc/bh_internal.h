@@ -87,6 +87,9 @@ struct DevBuf {
         }
         n = count;
     }
+    void ensure(size_t count) {   // at least count elements; contents unspecified
+        if (n < count) alloc(count);
+    }
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -135,6 +138,10 @@ struct Ctx {
     std::map<std::string, KernelStat> kstats;
     std::vector<hipEvent_t> event_pool;   // timing events reused across launches
     std::string profile_only;             // when set, only this kernel name is timed
+    // workspace of the FP32 factorisation kept between calls (split planes,
+    // tournament buffers, the block-column stream), so that repeated
+    // factorisations allocate and free nothing
+    std::shared_ptr<void> lu_cache;
     // objects (meshes, matrices) created on this context; destruction of the
     // context is deferred until the last of them is gone
     std::atomic<int> refs{0};

@@ -48,6 +48,7 @@
 #include <climits>
 #include <cmath>
 #include <complex>
+#include <memory>
 #include <vector>
 
 #include "bh_internal.h"
@@ -1985,8 +1986,8 @@ void gemm_split(Ctx &ctx, hipStream_t s, float2 *A, long long lda, int ar, int a
     if (M <= 0 || N <= 0 || K <= 0) return;
     BH_REQUIRE(K % kSplitBK == 0, "split gemm K=%d not a multiple of %d", K, kSplitBK);
     BH_REQUIRE(!reuse_a || PA.n >= (size_t)4 * M * K, "split gemm: no planes to reuse");
-    if (PA.n < (size_t)4 * M * K) PA.alloc((size_t)4 * M * K);
-    if (PB.n < (size_t)4 * N * K) PB.alloc((size_t)4 * N * K);
+    PA.ensure((size_t)4 * M * K);
+    PB.ensure((size_t)4 * N * K);
     {
         ProfScope prof(ctx, "k_split", 8.0 * ((reuse_a ? 0.0 : (double)M) + N) * K * 2.0, s);
         const long long na = (long long)M * K;
@@ -2147,6 +2148,33 @@ void block_row_u12(Ctx &ctx, hipStream_t s, LUWork<T> &W, typename CT<T>::t *A, 
 // disjoint columns.
 void invert_diagonal_blocks(hipStream_t s, const float2 *LU, long long lda, int n, float2 *Dinv);
 
+template <typename T>
+struct LUBuffers {
+    LUWork<T> W;
+    DevBuf<__bf16> PA, PB;   // split planes of the trailing updates
+};
+
+// The FP32 factorisation's workspace and block-column stream, kept in the context.
+struct LUCache {
+    std::mutex mu;
+    LUBuffers<float> b;
+    hipStream_t s2 = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+    ~LUCache() {
+        if (s2) { (void)hipStreamSynchronize(s2); (void)hipStreamDestroy(s2); }
+        if (ready) (void)hipEventDestroy(ready);
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+
+LUCache &lu_cache(Ctx &ctx)
+{
+    static std::mutex create;
+    std::lock_guard<std::mutex> g(create);
+    if (!ctx.lu_cache) ctx.lu_cache = std::make_shared<LUCache>();
+    return *static_cast<LUCache *>(ctx.lu_cache.get());
+}
+
 // dinv (FP32 only, may be NULL): the diagonal-block inverses and couplings of the
 // factors are formed on the context stream before the workspace is released, so
 // the device does not idle while the host frees it.
@@ -2155,8 +2183,28 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
                bool split = false, float2 *dinv = nullptr)
 {
     hipStream_t s = ctx.stream;
-    LUWork<T> W;
-    DevBuf<__bf16> PA, PB;
+    // FP32: the context's cached workspace unless another factorisation holds it
+    LUCache *cache = nullptr;
+    std::unique_lock<std::mutex> held;
+    std::unique_ptr<LUBuffers<T>> own;
+    LUBuffers<T> *buf = nullptr;
+    if constexpr (sizeof(T) == 4) {
+        cache = &lu_cache(ctx);
+        held = std::unique_lock<std::mutex>(cache->mu, std::try_to_lock);
+        if (held.owns_lock()) {
+            buf = &cache->b;
+            if (cache->s2) BH_CHECK_HIP(hipStreamSynchronize(cache->s2));   // after a failed call
+        } else {
+            cache = nullptr;
+        }
+    }
+    if (!buf) {
+        own = std::make_unique<LUBuffers<T>>();
+        buf = own.get();
+    }
+    LUWork<T> &W = buf->W;
+    DevBuf<__bf16> &PA = buf->PA, &PB = buf->PB;
+    W.eblk_of = W.eblk_side_of = -1;
     // rows per workgroup kept in LDS: 64 KiB of panel slice
     W.rpb_max = (int)(65536 / (kPW * sizeof(typename CT<T>::t)));
     int per_cu = 0;
@@ -2164,14 +2212,14 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
                                                               (size_t)W.rpb_max * kPW * sizeof(typename CT<T>::t)));
     BH_REQUIRE(per_cu >= 1, "panel kernel cannot be resident");
     W.maxG = per_cu * ctx.num_cus;
-    W.slot_val.alloc(2 * (size_t)W.maxG);
-    W.slot_row.alloc(2 * (size_t)W.maxG);
-    W.cand.alloc(2 * (size_t)W.maxG * kPW);
-    W.diag.alloc(2 * kPW);
-    W.bar.alloc(kBarWords);
-    W.info.alloc(1);
-    W.plan.alloc(4 * kMaxSwaps);
-    W.plan_side.alloc(4 * kMaxSwaps);
+    W.slot_val.ensure(2 * (size_t)W.maxG);
+    W.slot_row.ensure(2 * (size_t)W.maxG);
+    W.cand.ensure(2 * (size_t)W.maxG * kPW);
+    W.diag.ensure(2 * kPW);
+    W.bar.ensure(kBarWords);
+    W.info.ensure(1);
+    W.plan.ensure(4 * kMaxSwaps);
+    W.plan_side.ensure(4 * kMaxSwaps);
     BH_CHECK_HIP(hipMemsetAsync(W.bar.p, 0, kBarWords * sizeof(unsigned), s));
     BH_CHECK_HIP(hipMemsetAsync(W.info.p, 0, sizeof(int), s));
     // block column of the right-looking LU (trailing-update depth); the FP32
@@ -2205,29 +2253,40 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
         // FP32 factors only precondition GMRES-IR: their panels use tournament pivoting
         // candidate sets of all levels: n_0 + n_1 + ... < 2 n_0 + levels
         const size_t sets = 2 * ((size_t)(n + kTsluT - 1) / kTsluT) + 8;
-        W.tcnt.alloc(sets);
-        W.tidx.alloc(sets * kPW);
-        W.tval.alloc(sets * kPW * kPW);
+        W.tcnt.ensure(sets);
+        W.tidx.ensure(sets * kPW);
+        W.tval.ensure(sets * kPW * kPW);
         BH_CHECK_HIP(hipMemsetAsync(W.tcnt.p, 0, sets * sizeof(int), s));
-        W.top.alloc((size_t)kPW * kPW);
-        W.oldtop.alloc((size_t)kPW * kPW);
-        W.tplan.alloc(1 + 2 * kPlanMax);
-        W.eblk.alloc((size_t)(kNBInnerMax / kIB) * kIB * kIB);
-        W.eblk_side.alloc((size_t)(kNBInnerMax / kIB) * kIB * kIB);
+        W.top.ensure((size_t)kPW * kPW);
+        W.oldtop.ensure((size_t)kPW * kPW);
+        W.tplan.ensure(1 + 2 * kPlanMax);
+        W.eblk.ensure((size_t)(kNBInnerMax / kIB) * kIB * kIB);
+        W.eblk_side.ensure((size_t)(kNBInnerMax / kIB) * kIB * kIB);
         hipStream_t s2 = nullptr;
         hipEvent_t ready = nullptr, done = nullptr;
-        BH_CHECK_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-        struct Cleanup {
+        struct Cleanup {   // a stream and events of this call only (no cache)
             hipStream_t &s2;
             hipEvent_t &a, &b;
+            bool own;
             ~Cleanup() {
+                if (!own) return;
                 if (s2) { (void)hipStreamSynchronize(s2); (void)hipStreamDestroy(s2); }
                 if (a) (void)hipEventDestroy(a);
                 if (b) (void)hipEventDestroy(b);
             }
-        } cleanup{s2, ready, done};
-        BH_CHECK_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-        BH_CHECK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        } cleanup{s2, ready, done, cache == nullptr};
+        if (cache) {
+            if (!cache->s2) BH_CHECK_HIP(hipStreamCreateWithFlags(&cache->s2, hipStreamNonBlocking));
+            if (!cache->ready) BH_CHECK_HIP(hipEventCreateWithFlags(&cache->ready, hipEventDisableTiming));
+            if (!cache->done) BH_CHECK_HIP(hipEventCreateWithFlags(&cache->done, hipEventDisableTiming));
+            s2 = cache->s2;
+            ready = cache->ready;
+            done = cache->done;
+        } else {
+            BH_CHECK_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            BH_CHECK_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+            BH_CHECK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        }
         factor_block<T>(ctx, s, W, A, lda, n, 0, std::min(fp32_block_depth(n, 0), n), ipiv);
         // block k is factored; its interchanges reach the other columns (in pieces
         // of at most kMaxSwaps interchanges, applied in order) over [c0, c1) and [c2, c3)
