@@ -56,8 +56,10 @@ def build_problem(bj, name):
     return mesh, X, t, e, k
 
 
-def cpu_baseline(bj, mesh, X, t, k, ncells=128, nlu=6000):
-    """Oracle (C/OpenMP port of the reference algorithm) + LAPACK, on a bounded sample."""
+def cpu_baseline(bj, mesh, X, t, k, ncells=128, nlu=6000, farfield=None):
+    """Oracle (C/OpenMP port of the reference algorithm) + LAPACK, on a bounded sample.
+    farfield = (directions, coefficients): also the oracle's far-field rate on a sample of
+    48 of those directions, reported under "farfield"."""
     from oracle import oracle
     import scipy.linalg as sla
     F, N = mesh.numcells, X.numfunctions()
@@ -78,18 +80,31 @@ def cpu_baseline(bj, mesh, X, t, k, ncells=128, nlu=6000):
     del lu, A
     est = ta * F / S + tl * (N / nlu) ** 3
     threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-    return {"value": est, "unit": "s", "cores": threads, "kind": "port",
-            "sample": (f"oracle assembly of {S} test cells x {F} trial cells ({ta:.2f} s) scaled by {F}/{S}; "
-                       f"LAPACK getrf of a {nlu}^2 complex matrix ({tl:.2f} s) scaled by ({N}/{nlu})^3; "
-                       "CPU restatement of the reference algorithm, not BEAST.jl")}
+    out = {"value": est, "unit": "s", "cores": threads, "kind": "port",
+           "sample": (f"oracle assembly of {S} test cells x {F} trial cells ({ta:.2f} s) scaled by {F}/{S}; "
+                      f"LAPACK getrf of a {nlu}^2 complex matrix ({tl:.2f} s) scaled by ({N}/{nlu})^3; "
+                      "CPU restatement of the reference algorithm, not BEAST.jl")}
+    if farfield is not None:
+        Y, x = farfield
+        sub = Y[:: max(1, len(Y) // 48)][:48]
+        t0 = time.perf_counter()
+        oracle.farfield(0, 1j * k, 1.0, sub, x, X)
+        tf = time.perf_counter() - t0
+        out["farfield"] = {"terms_per_s": len(sub) * F * len(bj.trgauss(10)[1]) / tf,
+                           "sample": f"{len(sub)} directions", "cores": threads}
+    return out
 
 
-def farfield_sample(bj, ctx, X, x, k, cpu=True, nth=181, nph=361):
+def farfield_directions(nth=181, nph=361):
+    th, ph = np.meshgrid(np.linspace(0, np.pi, nth), np.linspace(0, 2 * np.pi, nph), indexing="ij")
+    return np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1).reshape(-1, 3)
+
+
+def farfield_sample(bj, ctx, X, x, k, nth=181, nph=361):
     """Far field of the solved current on a full (theta, phi) grid of directions
     (potential(MWFarField3D(wavenumber=k), points, x, X), default SingleNumQStrat(10)),
     outside the timed region.  Work unit: one (direction, rule point) term."""
-    th, ph = np.meshgrid(np.linspace(0, np.pi, nth), np.linspace(0, 2 * np.pi, nph), indexing="ij")
-    Y = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1).reshape(-1, 3)
+    Y = farfield_directions(nth, nph)
     op = bj.MWFarField3D(wavenumber=k)
     bj.potential(op, Y[:256], x, X)
     ctx.profile(True)
@@ -109,14 +124,6 @@ def farfield_sample(bj, ctx, X, x, k, cpu=True, nth=181, nph=361):
     out = {"workload": f"MWFarField3D(wavenumber={k}) of the solution at {nth}x{nph} directions, "
                        f"{X.geo.numcells} cells x 25 rule points", "directions": len(Y), "terms": terms,
            "wall_s": wall, "terms_per_s": terms / wall, "kernels": kern}
-    if cpu:
-        from oracle import oracle
-        sub = Y[:: max(1, len(Y) // 48)][:48]
-        t0 = time.perf_counter()
-        oracle.farfield(0, op.gamma, op.amplitude, sub, x, X)
-        tc = time.perf_counter() - t0
-        out["cpu_oracle"] = {"terms_per_s": len(sub) * X.geo.numcells * 25 / tc,
-                             "sample": f"{len(sub)} directions", "cores": int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))}
     return out
 
 
@@ -201,7 +208,7 @@ def main():
             breakdown[name] = {"launches": n_, "ms": round(1e3 * s_, 3)}
     f_launch, f_sec, f_work = ctx.kernel_stats("k_far_tiles")
     ctx.profile(False)
-    farfield = farfield_sample(bj, ctx, X, x, k, cpu=not args.no_cpu_baseline)
+    farfield = farfield_sample(bj, ctx, X, x, k)
     err = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
     value = t_all / args.steps
     achieved = g_work / g_sec if g_sec > 0 else 0.0
@@ -261,7 +268,8 @@ def main():
             log(f"bench: serving {st}")
     if not args.no_cpu_baseline:
         log("bench: CPU baseline sample")
-        out["cpu_baseline"] = cpu_baseline(bj, mesh, X, t, k)
+        out["cpu_baseline"] = cpu_baseline(bj, mesh, X, t, k, farfield=(farfield_directions(), x))
+        out["farfield"]["cpu_oracle"] = out["cpu_baseline"]["farfield"]
     print(json.dumps(out), flush=True)
 
 
