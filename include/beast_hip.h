@@ -180,7 +180,9 @@ typedef struct {
     double seconds_solve;   /* device time of the solve */
 } bh_solve_stats;
 
-/* Assemble op on (X, X) into a resident FP64 matrix (row-major in HBM). */
+/* Assemble op on (X, X) into a resident FP64 matrix (row-major in HBM).  The same
+ * launch also writes the matrix rounded to FP32 (8 N^2 further bytes, held until the
+ * matrix is factored: an FP32 factorisation starts from it, an FP64 one frees it). */
 int bh_matrix_assemble(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, const bh_space *X,
                        bh_matrix **out);
 /* Z += scale * op(X, X) on a resident FP64 matrix (before it is factored); the
@@ -196,7 +198,9 @@ int bh_matrix_download(bh_matrix *M, double *A, int64_t ld);
  * lower L).  BH_FP64 factors the FP64 matrix with LAPACK's partial pivoting (the
  * same ipiv as zgetrf).  BH_FP32 / BH_FP32_SPLIT factor a rounded FP32 copy; their
  * panels choose pivot rows by tournament pivoting, so ipiv may differ from
- * zgetrf's.  Calls on one matrix are serialised with its solves. */
+ * zgetrf's.  Calls on one matrix are serialised with its solves.  The FP32
+ * factorisation's workspace (about 1 GB at N = 40k) stays allocated in the context
+ * for the next factorisation until the context is destroyed. */
 int bh_matrix_factor(bh_matrix *M, int precision);
 /* Solve A X = B for nrhs column-major right-hand sides (ld = n).  With FP64
  * factors: getrs.  With FP32 factors: FP64-residual iterative refinement with
