@@ -696,6 +696,7 @@ struct FarArgs {
     int Qo, Qi;
     int mode;
     double k2;
+    int shifted;                  // FP64, lossless: phases taken relative to the centroid distance
     const int *nptr;              // CSR of non-far pairs per test cell
     const int4 *nent;
     const double *nblk;           // 18 doubles per entry
@@ -705,7 +706,7 @@ struct FarArgs {
 // Far-field 3x3 block of (test p, trial q) by double Gauss, written to z[k*stride].
 // QO, QI: the point counts when fixed at compile time (0: read from A).
 // maybe_near = false: the caller knows the pair is not in p's non-far list.
-template <typename R, int OPK, int QO = 0, int QI = 0>
+template <typename R, int OPK, int QO = 0, int QI = 0, bool SHIFT = false>
 __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> *z, int stride, bool maybe_near = true)
 {
     const int Qo = QO ? QO : A.Qo, Qi = QI ? QI : A.Qi;
@@ -740,6 +741,38 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         }
     };
     const R gr = (R)A.op.gr, gi = (R)A.op.gi;
+    // shifted: exp(-i gi R) = exp(-i gi Rc) exp(-i gi (R - Rc)), Rc the centroid distance,
+    // the second factor by its Taylor series (|gi (R - Rc)| <= |gi| (r_p + r_q), bounded on
+    // the host so that the series' remainder stays below 1e-17)
+    R cc = (R)1, sc = (R)0, rc = (R)0;
+    constexpr bool shifted = SHIFT && sizeof(R) == 8;
+    if constexpr (shifted) {
+        rc = sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]);
+        double s_, c_;
+        sincos_f64((double)(gi * rc), &s_, &c_);
+        sc = (R)s_;
+        cc = (R)c_;
+    }
+    auto kernel_g = [&](R dist, R idist) -> cx<R> {
+        if constexpr (!shifted) return green<R>(gr, gi, dist, idist);
+        const R t = gi * (dist - rc), u = t * t;
+        // cos t = sum (-1)^k u^k / (2k)!, sin t = t sum (-1)^k u^k / (2k+1)!, k <= 6
+        R pc = (R)(1.0 / 479001600.0), ps = (R)(1.0 / 6227020800.0);
+        pc = pc * u - (R)(1.0 / 3628800.0);
+        ps = ps * u - (R)(1.0 / 39916800.0);
+        pc = pc * u + (R)(1.0 / 40320.0);
+        ps = ps * u + (R)(1.0 / 362880.0);
+        pc = pc * u - (R)(1.0 / 720.0);
+        ps = ps * u - (R)(1.0 / 5040.0);
+        pc = pc * u + (R)(1.0 / 24.0);
+        ps = ps * u + (R)(1.0 / 120.0);
+        pc = pc * u - (R)0.5;
+        ps = ps * u - (R)(1.0 / 6.0);
+        const R ct = pc * u + (R)1, st = t * (ps * u + (R)1);
+        const R amp = idist * (R)kInv4Pi;
+        // exp(-i gi R) = (cc - i sc)(ct - i st)
+        return {(cc * ct - sc * st) * amp, -(sc * ct + cc * st) * amp};
+    };
     const R *xq = (const R *)A.tq + (size_t)p * Qo * 4;
     const R *yq = (const R *)A.sq + (size_t)q * Qi * 4;
     if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
@@ -759,7 +792,7 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
                 R dist, idist;
                 if constexpr (sizeof(R) == 8) { idist = rsqrt_f64(R2); dist = R2 * idist; }
                 else { idist = rsqrtf(R2); dist = R2 * idist; }
-                const cx<R> G = green<R>(gr, gi, dist, idist);
+                const cx<R> G = kernel_g(dist, idist);
                 const cx<R> wg{wq * G.r, wq * G.i};
                 S0.r += wg.r; S0.i += wg.i;
 #pragma unroll
@@ -798,7 +831,7 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
                 R dist, idist;
                 if constexpr (sizeof(R) == 8) { idist = rsqrt_f64(R2); dist = R2 * idist; }
                 else { idist = rsqrtf(R2); dist = R2 * idist; }
-                const cx<R> G = green<R>(gr, gi, dist, idist);
+                const cx<R> G = kernel_g(dist, idist);
                 cx<R> h = cmul(cx<R>{gr + idist, gi}, G);
                 const R f = -idist * wq;
                 h.r *= f; h.i *= f;
@@ -870,7 +903,7 @@ struct TileArgs {
     double scale_r, scale_i;
 };
 
-template <typename R, int OPK, bool ROWMAJOR, int QO, int QI>
+template <typename R, int OPK, bool ROWMAJOR, int QO, int QI, bool SHIFT = false>
 __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
 {
     __shared__ int s_rc[kTileCells], s_cc[kTileCells];
@@ -934,7 +967,7 @@ __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
             const int rc = t % nr, cc = t / nr;
             const int bit = rc * kTileCells + cb + cc;
             const bool near = lists && ((s_near[bit >> 5] >> (bit & 31)) & 1u);
-            far_block<R, OPK, QO, QI>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kPairChunk, near);
+            far_block<R, OPK, QO, QI, SHIFT>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kPairChunk, near);
         }
         __syncthreads();
         // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184
@@ -1310,6 +1343,25 @@ void make_plan(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, Mesh &tm
     F.sq = quad_points(sm, P.st.far_i, &F.Qi, fp32, ctx.stream);
     F.mode = P.st.mode;
     F.k2 = P.op.gr * P.op.gr + P.op.gi * P.op.gi;
+    // lossless FP64: phases relative to the centroid distance when |gi| (r_p + r_q) stays
+    // below 0.369, where the degree-12/13 Taylor remainders of cos and sin are below 1e-17
+    F.shifted = 0;
+    if (!fp32 && P.op.gr == 0.0) {
+        auto radius = [](const Mesh &m) {
+            double r = 0.0;
+            for (int64_t c = 0; c < m.nf; ++c) {
+                const double *g = &m.geo[(size_t)c * kGeo];
+                for (int a = 0; a < 3; ++a) {
+                    const double d0 = g[3 * a] - g[kGeoC], d1 = g[3 * a + 1] - g[kGeoC + 1],
+                                 d2 = g[3 * a + 2] - g[kGeoC + 2];
+                    r = std::max(r, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+                }
+            }
+            return r;
+        };
+        const double th = std::fabs(P.op.gi) * (radius(tm) + radius(sm)) * (1.0 + 1e-12);
+        F.shifted = th <= 0.369 ? 1 : 0;
+    }
     F.nptr = P.nd.nptr.p;
     F.nent = P.nd.entries.p;
     F.nblk = P.nd.blocks.p;
@@ -1342,10 +1394,13 @@ void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long lon
     // the default far rules (Dunavant degrees 2 and 3: 3 x 4 points) get fully
     // unrolled point loops
     const bool dflt = P.far.Qo == 3 && P.far.Qi == 4;
-#define BH_TILES(OPK, RM)                                                               \
-    do {                                                                                \
-        if (dflt) k_far_tiles<R, OPK, RM, 3, 4><<<grid, 256, 0, ctx.stream>>>(A);      \
-        else k_far_tiles<R, OPK, RM, 0, 0><<<grid, 256, 0, ctx.stream>>>(A);           \
+    // the centroid-shifted phases (FP64, lossless, small cells) for the default rules
+    const bool shift = sizeof(R) == 8 && dflt && P.far.shifted;
+#define BH_TILES(OPK, RM)                                                                    \
+    do {                                                                                     \
+        if (shift) k_far_tiles<R, OPK, RM, 3, 4, true><<<grid, 256, 0, ctx.stream>>>(A);    \
+        else if (dflt) k_far_tiles<R, OPK, RM, 3, 4><<<grid, 256, 0, ctx.stream>>>(A);      \
+        else k_far_tiles<R, OPK, RM, 0, 0><<<grid, 256, 0, ctx.stream>>>(A);                \
     } while (0)
     if (rowmajor) {
         if (sl) BH_TILES(BH_OP_MW_SINGLELAYER, true);
