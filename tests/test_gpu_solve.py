@@ -143,3 +143,32 @@ def test_fp32split_deep_blocks_match_fp64(bj):
     M = bj.DeviceMatrix.assemble(t, X).factor("fp64")
     x64 = M.solve(b)
     assert relfro(x32, x64) < 1e-8
+
+
+def test_refactor_sequences(bj, golden):
+    """The FP32 copy written by the resident assembly is consumed by the first FP32
+    factorisation; factoring again (FP32 twice, then FP64, then FP32) converts from the
+    FP64 matrix each time and gives the same solution; the context's cached FP32
+    workspace serves matrices of different sizes in turn."""
+    m = bj.readmesh(os.path.join(golden, "sphere2.in"))
+    X = bj.raviartthomas(m)
+    t = bj.Maxwell3D.singlelayer(wavenumber=1.0)
+    E = bj.Maxwell3D.planewave(direction=bj.z_hat, polarization=bj.x_hat, wavenumber=1.0)
+    b = bj.assemble(bj.cross(bj.cross(bj.n, E), bj.n), X)
+    M = bj.DeviceMatrix.assemble(t, X)
+    sols = []
+    for prec in ("fp32split", "fp32split", "fp64", "fp32"):
+        M.factor(prec)
+        sols.append(M.solve(b, tol=1e-13 if prec != "fp64" else 0.0))
+    for u in sols[1:]:
+        assert relfro(u, sols[0]) < 1e-10
+    # a larger and then a smaller system on the same context
+    m2 = bj.meshsphere(1.0, nv=900)
+    X2 = bj.raviartthomas(m2)
+    M2 = bj.DeviceMatrix.assemble(t, X2).factor("fp32split")
+    b2 = bj.assemble(bj.cross(bj.cross(bj.n, E), bj.n), X2)
+    u2 = M2.solve(b2, tol=1e-12)
+    assert M2.stats["residual"] <= 1e-12
+    M.factor("fp32split")
+    assert relfro(M.solve(b, tol=1e-13), sols[0]) < 1e-10
+    del u2
