@@ -140,7 +140,22 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the Poisson serving sample (0 skips serving)")
     ap.add_argument("--batch-requests", type=int, default=64, help="requests of the batched sample (0 skips)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three plain 256 MiB device copies (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 64 << 20   # float32 elements: 256 MiB
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n,
+                                               "copies": 3, "launch_order": "one fill of src, then the copies"}}),
+              flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: the assembly and the LU "
               "are measured on one GPU's share of the work", file=sys.stderr)
