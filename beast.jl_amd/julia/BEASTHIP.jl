@@ -222,6 +222,29 @@ end
 
 Base.:\(M::DeviceMatrix, b::AbstractVecOrMat) = ldiv!(similar(b, ComplexF64), M, ComplexF64.(b))
 
+# Plane-wave right-hand side: assemble((n x E) x n, X) (excitation.jl:17-115,149-198,
+# mwexc.jl:1-52) for E a PlaneWaveMW, on the device; rule = the SingleNumQStrat degree.
+function gpu_assemble_rhs(e::BEAST.TangTraceMW, X::BEAST.RTBasis; quadstrat=BEAST.defaultquadstrat(e, X),
+        ctx=context())
+    E = e.field
+    E isa BEAST.PlaneWaveMW || throw(Unsupported("right-hand side of $(typeof(E))"))
+    d = Float64[E.direction[k] for k in 1:3]
+    pr = Float64[real(E.polarisation[k]) for k in 1:3]
+    pi_ = Float64[imag(E.polarisation[k]) for k in 1:3]
+    g = ComplexF64(E.gamma); a = ComplexF64(E.amplitude)
+    b = zeros(ComplexF64, numfunctions(X))
+    check(ccall((:bh_assemble_planewave, libbeast_hip), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{ComplexF64}, Ref{ComplexF64}, Cint,
+         Cint, Ptr{ComplexF64}),
+        ctx.ptr, spacehandle(X, ctx), d, pr, pi_, g, a, quadstrat.quad_rule, 1, b))
+    return b
+end
+
+function BEAST.assemble(e::BEAST.TangTraceMW, X::BEAST.RTBasis; threading=nothing, kwargs...)
+    threading == :gpu && return gpu_assemble_rhs(e, X; kwargs...)
+    return invoke(BEAST.assemble, Tuple{BEAST.Functional,Any}, e, X; kwargs...)
+end
+
 # Far fields: potential(op, points, coeffs, X) (postproc.jl:138-150,174-211) for the
 # Maxwell far-field operators (maxwell/farfield.jl:1-103) on a raviartthomas space.
 const FFKINDS = Dict{Any,Int32}(BEAST.MWFarField3D => 0, BEAST.MWDoubleLayerFarField3D => 1,
