@@ -70,6 +70,7 @@ _SIGS = [
     ("bh_ctx_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("bh_ctx_destroy", C.c_int, [_P]),
     ("bh_ctx_synchronize", C.c_int, [_P]),
+    ("bh_ctx_pool", C.c_int, [_P, C.c_int64, _I64]),
     ("bh_ctx_profile", C.c_int, [_P, C.c_int]),
     ("bh_ctx_kernel_stats", C.c_int, [_P, C.c_char_p, _I64, _D, _D]),
     ("bh_ctx_profile_only", C.c_int, [_P, C.c_char_p]),

@@ -35,6 +35,13 @@ class Context:
         nat.check(self._lib.bh_ctx_profile_only(self.handle, (only or "").encode()))
         nat.check(self._lib.bh_ctx_profile(self.handle, int(bool(enable))))
 
+    def pool(self, limit_bytes=-1):
+        """Set the most bytes of freed matrix buffers the context keeps for reuse (-1 leaves
+        it); returns the bytes it holds."""
+        held = C.c_int64()
+        nat.check(self._lib.bh_ctx_pool(self.handle, int(limit_bytes), C.byref(held)))
+        return held.value
+
     def kernel_stats(self, name):
         """(launches, device seconds, algorithmic work) of one kernel since profiling was enabled."""
         n = C.c_int64()
@@ -162,10 +169,15 @@ def assemble(op, X, Y=None, quadstrat=defaultquadstrat, precision="fp64", ctx=No
     if out is None:
         out = np.zeros((M, N), dtype=dt, order="F")
         row_offset = col_offset = 0
-    if out.dtype != dt or not out.flags.f_contiguous:
-        raise ValueError("out must be a Fortran-ordered array of the assembly precision")
+    if out.dtype != dt or not out.flags.f_contiguous or out.ndim != 2:
+        raise ValueError("out must be a 2-d Fortran-ordered array of the assembly precision")
     if M == 0 or N == 0:
         return out
+    # the block must lie inside out (the reference's store raises BoundsError otherwise)
+    row_offset, col_offset = int(row_offset), int(col_offset)
+    if row_offset < 0 or col_offset < 0 or row_offset + M > out.shape[0] or col_offset + N > out.shape[1]:
+        raise IndexError(f"a {M} x {N} block at offsets ({row_offset}, {col_offset}) does not fit out of shape "
+                         f"{out.shape}")
     lib = nat.load()
     xh, yh = space_handle(X, ctx), space_handle(Y, ctx)
     for coef, term in _terms(op):

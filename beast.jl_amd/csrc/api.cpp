@@ -61,6 +61,8 @@ static void ctx_free(Ctx *c)
     if (!c->freed.compare_exchange_strong(expected, true)) return;   // exactly once
     (void)hipSetDevice(c->device);
     resolve_stats(*c);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    pool_trim(*c, 0);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     c->event_pool.clear();
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -70,6 +72,76 @@ static void ctx_free(Ctx *c)
 }
 
 void ctx_retain(Ctx *c) { c->refs.fetch_add(1); }
+
+// ---------------------------------------------------------------- buffer pool
+namespace {
+constexpr size_t kPoolMinBytes = size_t(1) << 20;   // smaller buffers are freed at once
+}
+
+void *pool_alloc(Ctx &ctx, size_t bytes)
+{
+    {
+        std::lock_guard<std::mutex> g(ctx.pool_mu);
+        int best = -1;
+        for (int i = 0; i < (int)ctx.pool.size(); ++i) {
+            const size_t b = ctx.pool[i].second;
+            if (b >= bytes && b - bytes <= bytes / 4 && (best < 0 || b < ctx.pool[best].second)) best = i;
+        }
+        if (best >= 0) {
+            void *p = ctx.pool[best].first;
+            ctx.pool_bytes -= ctx.pool[best].second;
+            ctx.pool.erase(ctx.pool.begin() + best);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {   // pooled memory may be what is missing
+        (void)hipGetLastError();
+        pool_trim(ctx, 0);
+        e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess) {
+        set_error("hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        throw Error{BH_ERR_OOM};
+    }
+    return p;
+}
+
+void pool_free(Ctx &ctx, void *p, size_t bytes)
+{
+    if (!p) return;
+    std::vector<void *> drop;
+    {
+        std::lock_guard<std::mutex> g(ctx.pool_mu);
+        if (bytes < kPoolMinBytes || bytes > ctx.pool_limit) {
+            drop.push_back(p);
+        } else {
+            ctx.pool.emplace_back(p, bytes);
+            ctx.pool_bytes += bytes;
+            while (ctx.pool_bytes > ctx.pool_limit) {   // oldest first
+                drop.push_back(ctx.pool.front().first);
+                ctx.pool_bytes -= ctx.pool.front().second;
+                ctx.pool.erase(ctx.pool.begin());
+            }
+        }
+    }
+    for (void *q : drop) (void)hipFree(q);
+}
+
+void pool_trim(Ctx &ctx, size_t keep_bytes)
+{
+    std::vector<void *> drop;
+    {
+        std::lock_guard<std::mutex> g(ctx.pool_mu);
+        while (ctx.pool_bytes > keep_bytes && !ctx.pool.empty()) {
+            drop.push_back(ctx.pool.front().first);
+            ctx.pool_bytes -= ctx.pool.front().second;
+            ctx.pool.erase(ctx.pool.begin());
+        }
+    }
+    for (void *q : drop) (void)hipFree(q);
+}
 
 void ctx_release(Ctx *c)
 {
@@ -84,6 +156,25 @@ extern "C" int bh_ctx_destroy(bh_ctx *ctx)
     ctx->closing.store(true);
     if (ctx->refs.load() > 0) return BH_OK;   // freed with the last mesh or matrix created on it
     bh::ctx_free(ctx);
+    BH_API_END
+}
+
+extern "C" int bh_ctx_pool(bh_ctx *ctx, int64_t limit_bytes, int64_t *pooled_bytes)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctx != nullptr, "ctx is NULL");
+    if (limit_bytes >= 0) {
+        BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        {
+            std::lock_guard<std::mutex> g(ctx->pool_mu);
+            ctx->pool_limit = (size_t)limit_bytes;
+        }
+        pool_trim(*ctx, (size_t)limit_bytes);
+    }
+    if (pooled_bytes) {
+        std::lock_guard<std::mutex> g(ctx->pool_mu);
+        *pooled_bytes = (int64_t)ctx->pool_bytes;
+    }
     BH_API_END
 }
 

@@ -100,6 +100,9 @@ struct DevBuf {
         if (count) BH_CHECK_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
     }
     size_t bytes() const { return n * sizeof(T); }
+    // ownership transfer to and from the context's buffer pool
+    void adopt(T *q, size_t count) { release(); p = q; n = count; }
+    T *detach() { T *q = p; p = nullptr; n = 0; return q; }
 };
 
 // ---------------------------------------------------------------- quadrature rules
@@ -147,9 +150,39 @@ struct Ctx {
     std::atomic<int> refs{0};
     std::atomic<bool> closing{false};
     std::atomic<bool> freed{false};
+    // device buffers of destroyed resident matrices, kept for the next matrix of
+    // a similar size (a repeated assemble-factor-solve allocates nothing); at most
+    // pool_limit bytes are kept, the oldest buffers are freed first
+    std::mutex pool_mu;
+    std::vector<std::pair<void *, size_t>> pool;
+    size_t pool_bytes = 0;
+    size_t pool_limit = size_t(64) << 30;
 };
 void ctx_retain(Ctx *c);
 void ctx_release(Ctx *c);
+
+// Pool of large device buffers (see Ctx::pool).  pool_alloc returns a pooled
+// buffer of at least `bytes` and at most 5/4 of it, else a fresh allocation
+// (after freeing the pool if the device is out of memory).  pool_free returns a
+// buffer to the pool (or frees it when it is small or the pool is disabled).
+// The caller orders the work: a buffer goes to the pool only once the kernels
+// that use it are complete or ordered on the context stream.
+void *pool_alloc(Ctx &ctx, size_t bytes);
+void pool_free(Ctx &ctx, void *p, size_t bytes);
+void pool_trim(Ctx &ctx, size_t keep_bytes);
+template <typename T>
+void pool_release(Ctx &ctx, DevBuf<T> &b)
+{
+    const size_t bytes = b.n * sizeof(T);
+    if (b.p) pool_free(ctx, b.detach(), bytes);
+}
+template <typename T>
+void pool_alloc(Ctx &ctx, DevBuf<T> &b, size_t count)
+{
+    if (b.n == count) return;
+    pool_release(ctx, b);
+    if (count) b.adopt((T *)pool_alloc(ctx, count * sizeof(T)), count);
+}
 
 // Brackets one launch on ctx.stream when profiling is on.
 struct ProfScope {

@@ -27,4 +27,12 @@ struct bh_matrix {
     int factored = -1;             // -1 none, BH_FP64 or BH_FP32
     double seconds_factor = 0.0;
     std::mutex mu;                 // factor and solve calls on one matrix run one at a time
+    // the large buffers go back to the context's pool (the caller has synchronised)
+    ~bh_matrix() {
+        if (!ctx) return;
+        bh::pool_release(*ctx, a64);
+        bh::pool_release(*ctx, lu64);
+        bh::pool_release(*ctx, lu32);
+        bh::pool_release(*ctx, dinv32);
+    }
 };

@@ -2436,17 +2436,17 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
     const unsigned cgrid = (unsigned)(((long long)n * n + 255) / 256);
     BH_CHECK_HIP(hipEventRecord(e0, ctx.stream));
     if (precision == BH_FP64) {
-        M->lu32.release();
+        pool_release(ctx, M->lu32);
         M->lu32_fresh = false;
-        M->dinv32.release();
-        M->lu64.alloc((size_t)lda * n * 2);
+        pool_release(ctx, M->dinv32);
+        pool_alloc(ctx, M->lu64, (size_t)lda * n * 2);
         k_to_work<double><<<cgrid, 256, 0, ctx.stream>>>((const double2 *)M->a64.p, (int)n, (double2 *)M->lu64.p, lda);
         check("k_to_work");
         lu_factor<double>(ctx, (double2 *)M->lu64.p, lda, (int)n, M->ipiv.p, &info);
     } else {
-        M->lu64.release();
+        pool_release(ctx, M->lu64);
         if (M->lu32.n != (size_t)lda * n * 2) {
-            M->lu32.alloc((size_t)lda * n * 2);
+            pool_alloc(ctx, M->lu32, (size_t)lda * n * 2);
             M->lu32_fresh = false;
         }
         if (!M->lu32_fresh) {   // else the assembly wrote the FP32 copy
@@ -2456,8 +2456,7 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
         }
         M->lu32_fresh = false;   // it now holds the factors
         const size_t nb = (size_t)(n + kIB - 1) / kIB;
-        if (M->dinv32.n != 4 * nb * kIB * kIB * 2)
-            M->dinv32.alloc(4 * nb * kIB * kIB * 2);   // L and U diagonal inverses, then their couplings
+        pool_alloc(ctx, M->dinv32, 4 * nb * kIB * kIB * 2);   // L and U diagonal inverses, then their couplings
         lu_factor<float>(ctx, (float2 *)M->lu32.p, lda, (int)n, M->ipiv.p, &info, precision == BH_FP32_SPLIT,
                          (float2 *)M->dinv32.p);
     }
@@ -2685,6 +2684,22 @@ extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, doub
     double res = 0;
     std::vector<double> bn(q);
     for (int r = 0; r < q; ++r) bn[r] = host_norm(hb, n, q, r);
+    // the FP32 triangular solves report a block wait that gave up in tsync[1]; it is
+    // read (and cleared) after every synchronisation that ends a use of the solves,
+    // before anything derived from their results is judged
+    auto check_trsv = [&]() {
+        if (M->factored != BH_FP32 || !M->tsync.p) return;
+        int trsv_err = 0;
+        BH_CHECK_HIP(hipMemcpyAsync(&trsv_err, M->tsync.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        BH_CHECK_HIP(hipStreamSynchronize(s));
+        if (!trsv_err) return;
+        // results are void; the counters restart
+        BH_CHECK_HIP(hipMemsetAsync(M->tsync.p, 0, M->tsync.n * sizeof(int), s));
+        BH_CHECK_HIP(hipStreamSynchronize(s));
+        M->tsync_tickets = 0;
+        M->tsync_epoch = 0;
+        BH_FAIL(BH_ERR_HIP, "FP32 triangular solve: a block wait exceeded its limit");
+    };
     const bool refine = M->factored == BH_FP32 || tol > 0;
     if (refine) {
         const int maxit_ = maxit > 0 ? maxit : 20;
@@ -2698,6 +2713,7 @@ extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, doub
             std::vector<double2> hr(cnt);
             BH_CHECK_HIP(hipMemcpyAsync(hr.data(), dr.p, sizeof(double2) * cnt, hipMemcpyDeviceToHost, s));
             BH_CHECK_HIP(hipStreamSynchronize(s));
+            check_trsv();
             res = 0;
             std::vector<double> rel(q);
             for (int r = 0; r < q; ++r) {
@@ -2728,17 +2744,8 @@ extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, doub
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     BH_CHECK_HIP(hipMemcpyAsync(hx.data(), dx.p, sizeof(double2) * cnt, hipMemcpyDeviceToHost, s));
-    int trsv_err = 0;
-    if (M->factored == BH_FP32 && M->tsync.p)
-        BH_CHECK_HIP(hipMemcpyAsync(&trsv_err, M->tsync.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     BH_CHECK_HIP(hipStreamSynchronize(s));
-    if (trsv_err) {   // a triangular-solve wait gave up: results are void, the counters restart
-        BH_CHECK_HIP(hipMemsetAsync(M->tsync.p, 0, M->tsync.n * sizeof(int), s));
-        BH_CHECK_HIP(hipStreamSynchronize(s));
-        M->tsync_tickets = 0;
-        M->tsync_epoch = 0;
-        BH_FAIL(BH_ERR_HIP, "FP32 triangular solve: a block wait exceeded its limit");
-    }
+    check_trsv();
     for (int r = 0; r < q; ++r)
         for (int i = 0; i < n; ++i) {
             X[2 * ((size_t)r * n + i)] = hx[(size_t)i * q + r].x;

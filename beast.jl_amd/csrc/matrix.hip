@@ -54,14 +54,14 @@ extern "C" int bh_matrix_assemble(bh_ctx *ctx, const bh_operator *op, const bh_q
     std::unique_ptr<bh_matrix> M(new bh_matrix());
     M->ctx = ctx;
     M->n = X->ndofs;
-    M->a64.alloc((size_t)M->n * M->n * 2);
+    pool_alloc(*ctx, M->a64, (size_t)M->n * M->n * 2);
     bh_space *sp = const_cast<bh_space *>(X);
     // the tile kernel also writes the FP32 copy that an FP32 factorisation starts from
     const long long lda = padded_ld(M->n);
-    if (M->n > 0) M->lu32.alloc((size_t)lda * M->n * 2);
+    if (M->n > 0) pool_alloc(*ctx, M->lu32, (size_t)lda * M->n * 2);
     M->lu32_fresh = assemble_device(*ctx, op, qs, false, *sp, *sp, M->a64.p, M->n, true, false, 1.0, 0.0,
                                     M->n > 0 ? M->lu32.p : nullptr, lda);
-    if (!M->lu32_fresh) M->lu32.release();
+    if (!M->lu32_fresh) pool_release(*ctx, M->lu32);
     ctx_retain(ctx);
     *out = M.release();
     BH_API_END
@@ -93,11 +93,13 @@ extern "C" int bh_matrix_from_host(bh_ctx *ctx, int64_t n, const double *A, int6
     std::unique_ptr<bh_matrix> M(new bh_matrix());
     M->ctx = ctx;
     M->n = n;
-    M->a64.alloc((size_t)n * n * 2);
-    DevBuf<double> tmp((size_t)ld * n * 2);
-    BH_CHECK_HIP(hipMemcpyAsync(tmp.p, A, sizeof(double) * 2 * ld * n, hipMemcpyHostToDevice, ctx->stream));
+    pool_alloc(*ctx, M->a64, (size_t)n * n * 2);
+    // only the n x n block is read: rows [n, ld) of each caller column are padding
+    DevBuf<double> tmp((size_t)n * n * 2);
+    BH_CHECK_HIP(hipMemcpy2DAsync(tmp.p, sizeof(double) * 2 * n, A, sizeof(double) * 2 * ld, sizeof(double) * 2 * n,
+                                  n, hipMemcpyHostToDevice, ctx->stream));
     dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32)), block(32, 8);
-    k_colmajor_to_rowmajor<<<grid, block, 0, ctx->stream>>>((const double2 *)tmp.p, ld, (double2 *)M->a64.p, n);
+    k_colmajor_to_rowmajor<<<grid, block, 0, ctx->stream>>>((const double2 *)tmp.p, n, (double2 *)M->a64.p, n);
     BH_CHECK_HIP(hipGetLastError());
     BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     ctx_retain(ctx);
@@ -131,11 +133,13 @@ extern "C" int bh_matrix_download(bh_matrix *M, double *A, int64_t ld)
     BH_REQUIRE(M && A, "NULL argument");
     BH_REQUIRE(ld >= M->n, "ld < n");
     const int64_t n = M->n;
-    DevBuf<double> tmp((size_t)ld * n * 2);
+    // only the n x n block is written: rows [n, ld) of each caller column stay untouched
+    DevBuf<double> tmp((size_t)n * n * 2);
     dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32)), block(32, 8);
-    k_rowmajor_to_colmajor<<<grid, block, 0, M->ctx->stream>>>((const double2 *)M->a64.p, (double2 *)tmp.p, ld, n);
+    k_rowmajor_to_colmajor<<<grid, block, 0, M->ctx->stream>>>((const double2 *)M->a64.p, (double2 *)tmp.p, n, n);
     BH_CHECK_HIP(hipGetLastError());
-    BH_CHECK_HIP(hipMemcpyAsync(A, tmp.p, sizeof(double) * 2 * ld * n, hipMemcpyDeviceToHost, M->ctx->stream));
+    BH_CHECK_HIP(hipMemcpy2DAsync(A, sizeof(double) * 2 * ld, tmp.p, sizeof(double) * 2 * n, sizeof(double) * 2 * n,
+                                  n, hipMemcpyDeviceToHost, M->ctx->stream));
     BH_CHECK_HIP(hipStreamSynchronize(M->ctx->stream));
     BH_API_END
 }

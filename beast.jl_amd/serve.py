@@ -127,6 +127,8 @@ class Server:
         self._busy = {}                    # key -> refcount of requests using it
         self._reserved = 0.0
         self._contexts = [asm.Context(0) for _ in range(workers)]
+        for c in self._contexts:   # evicted matrices free their memory (the budget counts only residents)
+            c.pool(0)
 
     @staticmethod
     def footprint(n):
@@ -190,8 +192,15 @@ class Server:
         try:
             if M is None:
                 op = self._operator(req.kind, req.wavenumber)
-                M = asm.DeviceMatrix.assemble(op, X, ctx=ctx)
-                M.factor(self.precision)
+                try:
+                    M = asm.DeviceMatrix.assemble(op, X, ctx=ctx)
+                    M.factor(self.precision)
+                except BaseException:
+                    # give the reservation back, or later admissions wait for it forever
+                    with self._lock:
+                        self._reserved -= nbytes
+                        self._lock.notify_all()
+                    raise
                 with self._lock:
                     self._reserved -= nbytes
                     self._resident[key] = (M, nbytes)

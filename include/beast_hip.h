@@ -92,6 +92,14 @@ const char *bh_last_error(void);
 int bh_ctx_create(int device, bh_ctx **out);
 int bh_ctx_destroy(bh_ctx *ctx);
 int bh_ctx_synchronize(bh_ctx *ctx);
+/* Buffer pool (no counterpart in the reference): the device buffers of destroyed
+ * resident matrices stay with the context for the next matrix of a similar size
+ * (at least the size asked for, at most 5/4 of it), so that a repeated
+ * assemble-factor-solve allocates nothing.  limit_bytes >= 0 sets the most the pool
+ * keeps (default 64 GiB; 0 disables it and frees what it holds), a negative value
+ * leaves it; *pooled_bytes (may be NULL) receives what the pool holds now.  A
+ * device allocation that fails frees the pool and tries once more. */
+int bh_ctx_pool(bh_ctx *ctx, int64_t limit_bytes, int64_t *pooled_bytes);
 /* Measurement (no counterpart in the reference): while profiling is on, every
  * launch of the library's kernels is bracketed by HIP events on the context
  * stream; bh_ctx_kernel_stats returns, per kernel name (e.g. "k_cgemm"), the

@@ -172,3 +172,23 @@ def test_refactor_sequences(bj, golden):
     M.factor("fp32split")
     assert relfro(M.solve(b, tol=1e-13), sols[0]) < 1e-10
     del u2
+
+
+def test_transfers_with_padded_leading_dimension(bj):
+    """bh_matrix_from_host / bh_matrix_download with ld > n read and write the n x n block
+    only: the padding rows of the caller's columns are neither read nor overwritten."""
+    import ctypes as C
+    nat = bj._native
+    n, ld = 37, 41
+    A = _rand(n, 5)
+    H = np.full((ld, n), np.nan + 1j * np.nan, dtype=np.complex128, order="F")
+    H[:n] = A
+    ctx = bj.default_context()
+    h = C.c_void_p()
+    nat.check(nat.load().bh_matrix_from_host(ctx.handle, n, H.ctypes.data_as(nat._D), ld, C.byref(h)))
+    M = bj.DeviceMatrix(h, n, ctx)
+    assert np.array_equal(M.download(), A)
+    out = np.full((ld, n), 7.0 - 3.0j, dtype=np.complex128, order="F")
+    nat.check(nat.load().bh_matrix_download(M.h, out.ctypes.data_as(nat._D), ld))
+    assert np.array_equal(out[:n], A)
+    assert np.all(out[n:] == 7.0 - 3.0j)
