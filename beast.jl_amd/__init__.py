@@ -7,7 +7,8 @@ API for that path; all arithmetic runs in the native library.
 from .mesh import Mesh, readmesh, writemesh, meshsphere, meshellipsoid, meshcube, meshrectangle
 from .rt import RTBasis, raviartthomas, assemblydata, mesh_edges
 from .quadrules import trgauss, legendre01
-from .operators import (Maxwell3D, MWSingleLayer3D, MWDoubleLayer3D, NCross, LocalOperator, PlaneWaveMW,
+from .operators import (Maxwell3D, MWSingleLayer3D, MWDoubleLayer3D, DoubleLayerRotatedMW3D, NCross, Identity,
+                        LocalOperator, PlaneWaveMW, curl,
                         LinearCombinationOfOperators,
                         DoubleNumWiltonSauterQStrat, DoubleNumSauterQstrat, DoubleNumQStrat, SingleNumQStrat,
                         defaultquadstrat, gamma_wavenumber_handler, n, cross, x_hat, y_hat, z_hat,

@@ -20,7 +20,8 @@ BH_ERR_UNSUPPORTED = -5
 BH_ERR_SINGULAR = -6
 BH_ERR_NOCONV = -7
 
-OP_SINGLELAYER, OP_DOUBLELAYER, OP_NCROSS = 0, 1, 2
+OP_SINGLELAYER, OP_DOUBLELAYER, OP_NCROSS, OP_DOUBLELAYER_ROTATED, OP_IDENTITY = 0, 1, 2, 3, 4
+TRACE_TANGENTIAL, TRACE_CROSS = 0, 1
 QS_WILTON_SAUTER, QS_SAUTER, QS_DOUBLENUM, QS_SINGLENUM = 0, 1, 2, 3
 FP64, FP32, FP32_SPLIT = 0, 1, 2
 
@@ -86,6 +87,7 @@ _SIGS = [
     ("bh_local_blocks", C.c_int, [_P, C.POINTER(bh_operator), C.POINTER(bh_quadstrat), _P, _P, C.c_int64,
                                   _I32, _I32, _D]),
     ("bh_assemble_planewave", C.c_int, [_P, _P, _D, _D, _D, _D, _D, C.c_int, C.c_int, _D]),
+    ("bh_assemble_excitation", C.c_int, [_P, _P, C.c_int, _D, _D, _D, _D, _D, C.c_int, C.c_int, _D]),
     ("bh_farfield", C.c_int, [_P, C.c_int, _D, _D, _P, _D, C.c_int64, _D, C.c_int, _D]),
     ("bh_matrix_assemble", C.c_int, [_P, C.POINTER(bh_operator), C.POINTER(bh_quadstrat), _P, C.POINTER(_P)]),
     ("bh_matrix_accumulate", C.c_int, [_P, C.POINTER(bh_operator), C.POINTER(bh_quadstrat), _P, _D]),

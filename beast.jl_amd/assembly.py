@@ -10,9 +10,10 @@ import ctypes as C
 import numpy as np
 
 from . import _native as nat
-from .operators import (DoubleNumQStrat, DoubleNumSauterQstrat, DoubleNumWiltonSauterQStrat,
-                        LinearCombinationOfOperators, MWDoubleLayer3D, MWSingleLayer3D, NCross, PlaneWaveMW,
-                        SingleNumQStrat, TangTraceMW, defaultquadstrat, resolve_quadstrat)
+from .operators import (CrossTraceMW, DoubleLayerRotatedMW3D, DoubleNumQStrat, DoubleNumSauterQstrat,
+                        DoubleNumWiltonSauterQStrat, Identity, LinearCombinationOfOperators, MWDoubleLayer3D,
+                        MWSingleLayer3D, NCross, PlaneWaveMW, SingleNumQStrat, TangTraceMW, defaultquadstrat,
+                        resolve_quadstrat)
 from .rt import RTBasis
 
 
@@ -123,8 +124,15 @@ def op_struct(op):
         s.kind = nat.OP_DOUBLELAYER
         s.alpha[:] = [op.alpha.real, op.alpha.imag]
         s.beta[:] = [0.0, 0.0]
+    elif isinstance(op, DoubleLayerRotatedMW3D):
+        s.kind = nat.OP_DOUBLELAYER_ROTATED
+        s.alpha[:] = [op.alpha.real, op.alpha.imag]
+        s.beta[:] = [0.0, 0.0]
     elif isinstance(op, NCross):
         s.kind = nat.OP_NCROSS
+        return s
+    elif isinstance(op, Identity):
+        s.kind = nat.OP_IDENTITY
         return s
     else:
         raise nat.UnsupportedOnGPU(nat.BH_ERR_UNSUPPORTED, f"operator {type(op).__name__} has no GPU path")
@@ -189,10 +197,13 @@ def assemble(op, X, Y=None, quadstrat=defaultquadstrat, precision="fp64", ctx=No
 
 
 def assemble_rhs(field, X, quadstrat=defaultquadstrat, ctx=None):
-    """``assemble((n x E) x n, X)`` for a plane wave E (excitation.jl:17-115)."""
+    """``assemble((n x E) x n, X)`` or ``assemble(n x E, X)`` for a plane wave E
+    (excitation.jl:17-115,145-180)."""
     ctx = ctx or default_context()
-    if not isinstance(field, TangTraceMW) or not isinstance(field.field, PlaneWaveMW):
-        raise nat.UnsupportedOnGPU(nat.BH_ERR_UNSUPPORTED, "GPU right-hand side supports (n x E) x n of a plane wave")
+    if not isinstance(field, (TangTraceMW, CrossTraceMW)) or not isinstance(field.field, PlaneWaveMW):
+        raise nat.UnsupportedOnGPU(nat.BH_ERR_UNSUPPORTED,
+                                   "GPU right-hand sides support (n x E) x n and n x E of a plane wave")
+    trace = nat.TRACE_CROSS if isinstance(field, CrossTraceMW) else nat.TRACE_TANGENTIAL
     qs = resolve_quadstrat(quadstrat, field, X, None)
     if not isinstance(qs, SingleNumQStrat):
         raise ValueError("right-hand sides take a SingleNumQStrat")
@@ -204,9 +215,9 @@ def assemble_rhs(field, X, quadstrat=defaultquadstrat, ctx=None):
     pi = nat.as_f64(E.polarisation.imag)
     g = np.array([E.gamma.real, E.gamma.imag])
     a = np.array([E.amplitude.real, E.amplitude.imag])
-    nat.check(lib.bh_assemble_planewave(ctx.handle, space_handle(X, ctx), nat.dptr(d), nat.dptr(pr), nat.dptr(pi),
-                                        nat.dptr(g), nat.dptr(a), int(qs.quad_rule), 1,
-                                        b.ctypes.data_as(nat._D)))
+    nat.check(lib.bh_assemble_excitation(ctx.handle, space_handle(X, ctx), trace, nat.dptr(d), nat.dptr(pr),
+                                         nat.dptr(pi), nat.dptr(g), nat.dptr(a), int(qs.quad_rule), 1,
+                                         b.ctypes.data_as(nat._D)))
     return b
 
 

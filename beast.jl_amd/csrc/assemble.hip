@@ -409,10 +409,12 @@ __global__ __launch_bounds__(256) void k_sauter_schwab(const double *__restrict_
     }
     const int n = gl.n, n2 = n * n, n4 = n2 * n2;
     const int nreg = ss_nregions(cls);
-    // moments: EFIE (M0, Mx, My, Mxy) / MFIE (Kc, Kr)
-    double m[16];
+    // moments: EFIE (M0, Mx, My, Mxy) / MFIE-K (Kc, Kr) / n x K (m0..m7, see below)
+    constexpr int NM = OPK == BH_OP_MW_DOUBLELAYER_ROTATED ? 32 : 16;
+    const double nrm[3] = {tg[kGeoN], tg[kGeoN + 1], tg[kGeoN + 2]};
+    double m[NM];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) m[k] = 0.0;
+    for (int k = 0; k < NM; ++k) m[k] = 0.0;
     for (int r = 0; r < nreg; ++r) {
         for (int idx = lane; idx < n4; idx += 64) {
             const int i0 = idx % n, i1 = (idx / n) % n, i2 = (idx / n2) % n, i3 = idx / (n2 * n);
@@ -440,6 +442,26 @@ __global__ __launch_bounds__(256) void k_sauter_schwab(const double *__restrict_
                     m[8 + 2 * k] += wg.r * y[k]; m[9 + 2 * k] += wg.i * y[k];
                 }
                 m[14] += wg.r * xy; m[15] += wg.i * xy;
+            } else if constexpr (OPK == BH_OP_MW_DOUBLELAYER_ROTATED) {
+                // (x - a).r (n.(y - b)) - (x - a).(y - b) (n.r), h = -(gamma + 1/R) G / R:
+                // m0 = h (x.r)(n.y), m1 = h x.r, m2 = h r (n.y), m3 = h r,
+                // m4 = h (x.y)(n.r), m5 = h y (n.r), m6 = h x (n.r), m7 = h n.r
+                const cx<double> gp{op.gr + iR, op.gi};
+                cx<double> h = cmul(gp, G);
+                h.r *= -iR * wj;
+                h.i *= -iR * wj;
+                const double ny = dot3(nrm, y), nr = dot3(nrm, rr), xr = dot3(x, rr), xy = dot3(x, y);
+                m[0] += h.r * xr * ny; m[1] += h.i * xr * ny;
+                m[2] += h.r * xr; m[3] += h.i * xr;
+                m[16] += h.r * xy * nr; m[17] += h.i * xy * nr;
+                m[30] += h.r * nr; m[31] += h.i * nr;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    m[4 + 2 * k] += h.r * rr[k] * ny; m[5 + 2 * k] += h.i * rr[k] * ny;
+                    m[10 + 2 * k] += h.r * rr[k]; m[11 + 2 * k] += h.i * rr[k];
+                    m[18 + 2 * k] += h.r * y[k] * nr; m[19 + 2 * k] += h.i * y[k] * nr;
+                    m[24 + 2 * k] += h.r * x[k] * nr; m[25 + 2 * k] += h.i * x[k] * nr;
+                }
             } else {
                 // h = -(gamma + 1/R) G / R
                 const cx<double> gp{op.gr + iR, op.gi};
@@ -457,10 +479,28 @@ __global__ __launch_bounds__(256) void k_sauter_schwab(const double *__restrict_
         }
     }
 #pragma unroll
-    for (int k = 0; k < 16; ++k) m[k] = wave_sum(m[k]);
+    for (int k = 0; k < NM; ++k) m[k] = wave_sum(m[k]);
     if (lane != 0) return;
     cx<double> z[9];
-    if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
+    if constexpr (OPK == BH_OP_MW_DOUBLELAYER_ROTATED) {
+        // z_ij = m0 - nb m1 - a.m2 + nb a.m3 - m4 + a.m5 + b.m6 - (a.b) m7, nb = n.b_j
+        const cx<double> m0{m[0], m[1]}, m1{m[2], m[3]}, m4{m[16], m[17]}, m7{m[30], m[31]};
+        const cx<double> m2[3] = {{m[4], m[5]}, {m[6], m[7]}, {m[8], m[9]}};
+        const cx<double> m3[3] = {{m[10], m[11]}, {m[12], m[13]}, {m[14], m[15]}};
+        const cx<double> m5[3] = {{m[18], m[19]}, {m[20], m[21]}, {m[22], m[23]}};
+        const cx<double> m6[3] = {{m[24], m[25]}, {m[26], m[27]}, {m[28], m[29]}};
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const double *ai = a + 3 * i, *bj = b + 3 * j;
+                const double nb = dot3(nrm, bj), ab = dot3(ai, bj);
+                z[i + 3 * j] = {m0.r - nb * m1.r - dot3c_r(ai, m2) + nb * dot3c_r(ai, m3) - m4.r + dot3c_r(ai, m5) +
+                                    dot3c_r(bj, m6) - ab * m7.r,
+                                m0.i - nb * m1.i - dot3c_i(ai, m2) + nb * dot3c_i(ai, m3) - m4.i + dot3c_i(ai, m5) +
+                                    dot3c_i(bj, m6) - ab * m7.i};
+            }
+    } else if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
         const cx<double> M0{m[0], m[1]};
         const cx<double> Mx[3] = {{m[2], m[3]}, {m[4], m[5]}, {m[6], m[7]}};
         const cx<double> My[3] = {{m[8], m[9]}, {m[10], m[11]}, {m[12], m[13]}};
@@ -811,6 +851,82 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         load_verts(a, b);
         const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
         efie_block<R>(A.op, s1, a, b, T0, Tx, Ty, T1, z, stride);
+    } else if constexpr (OPK == BH_OP_MW_DOUBLELAYER_ROTATED) {
+        // f_i . (n x (K x g_j)) = (f_i . K)(n . g_j) - (f_i . g_j)(n . K), K = h r, r = xd - y,
+        // n the test normal, f_i = (x - a_i)/(2 A_t), g_j = (y - b_j)/(2 A_s).  Per test point,
+        // with Q = sum w h, P = sum w h (n.y), V = sum w h y and s = n.xd, the y (n.y) terms
+        // cancel and 4 A_t A_s z_ij sums (x - a).xd (P - nb Q) + (nb - s)(x - a).V
+        // + ((x - a).b)(s Q - P), nb = n.b_j; expanded in a and b into the moments below.
+        const R nrm[3] = {(R)tg[kGeoN], (R)tg[kGeoN + 1], (R)tg[kGeoN + 2]};
+        cx<R> Ma{0, 0}, Mb{0, 0}, Me{0, 0}, Mg{0, 0}, Mk{0, 0}, Ml{0, 0};
+        cx<R> Mc[3] = {{0, 0}, {0, 0}, {0, 0}}, Md[3] = {{0, 0}, {0, 0}, {0, 0}}, Mf[3] = {{0, 0}, {0, 0}, {0, 0}},
+              Mh[3] = {{0, 0}, {0, 0}, {0, 0}}, Mi[3] = {{0, 0}, {0, 0}, {0, 0}}, Mj[3] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll 1
+        for (int ip = 0; ip < Qo; ++ip) {
+            const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
+            const R wp = xq[4 * ip + 3];
+            const R xd[3] = {x[0] + dc[0], x[1] + dc[1], x[2] + dc[2]};
+            cx<R> Qs{0, 0}, Ps{0, 0}, V[3] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+            for (int iq = 0; iq < Qi; ++iq) {
+                const R y[3] = {yq[4 * iq], yq[4 * iq + 1], yq[4 * iq + 2]};
+                const R wq = yq[4 * iq + 3];
+                const R r[3] = {xd[0] - y[0], xd[1] - y[1], xd[2] - y[2]};
+                const R R2 = dot3(r, r);
+                R dist, idist;
+                if constexpr (sizeof(R) == 8) { idist = rsqrt_f64(R2); dist = R2 * idist; }
+                else { idist = rsqrtf(R2); dist = R2 * idist; }
+                const cx<R> G = kernel_g(dist, idist);
+                cx<R> h = cmul(cx<R>{gr + idist, gi}, G);
+                const R f = -idist * wq;
+                h.r *= f; h.i *= f;
+                const R ny = dot3(nrm, y);
+                Qs.r += h.r; Qs.i += h.i;
+                Ps.r += h.r * ny; Ps.i += h.i * ny;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) { V[k].r += h.r * y[k]; V[k].i += h.i * y[k]; }
+            }
+            const R s = dot3(nrm, xd), xxd = dot3(x, xd);
+            const cx<R> xV{x[0] * V[0].r + x[1] * V[1].r + x[2] * V[2].r, x[0] * V[0].i + x[1] * V[1].i + x[2] * V[2].i};
+            Ma.r += wp * xxd * Ps.r; Ma.i += wp * xxd * Ps.i;
+            Mb.r += wp * xxd * Qs.r; Mb.i += wp * xxd * Qs.i;
+            Me.r += wp * xV.r; Me.i += wp * xV.i;
+            Mg.r += wp * s * xV.r; Mg.i += wp * s * xV.i;
+            Mk.r += wp * s * Qs.r; Mk.i += wp * s * Qs.i;
+            Ml.r += wp * Ps.r; Ml.i += wp * Ps.i;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Mc[k].r += wp * xd[k] * Ps.r; Mc[k].i += wp * xd[k] * Ps.i;
+                Md[k].r += wp * xd[k] * Qs.r; Md[k].i += wp * xd[k] * Qs.i;
+                Mf[k].r += wp * V[k].r; Mf[k].i += wp * V[k].i;
+                Mh[k].r += wp * s * V[k].r; Mh[k].i += wp * s * V[k].i;
+                Mi[k].r += wp * s * x[k] * Qs.r; Mi[k].i += wp * s * x[k] * Qs.i;
+                Mj[k].r += wp * x[k] * Ps.r; Mj[k].i += wp * x[k] * Ps.i;
+            }
+        }
+        R a[9], b[9];
+        load_verts(a, b);
+        const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const R *bj = b + 3 * j;
+            const R nb = dot3(nrm, bj);
+            // the j-only part: Ma - nb Mb + nb Me - Mg + b.Mi - b.Mj
+            const cx<R> tj{Ma.r - nb * Mb.r + nb * Me.r - Mg.r + dot3c_r(bj, Mi) - dot3c_r(bj, Mj),
+                           Ma.i - nb * Mb.i + nb * Me.i - Mg.i + dot3c_i(bj, Mi) - dot3c_i(bj, Mj)};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const R *ai = a + 3 * i;
+                const R ab = dot3(ai, bj);
+                cx<R> t;
+                // - a.Mc + nb a.Md - nb a.Mf + a.Mh - (a.b) Mk + (a.b) Ml
+                t.r = tj.r - dot3c_r(ai, Mc) + nb * dot3c_r(ai, Md) - nb * dot3c_r(ai, Mf) + dot3c_r(ai, Mh) -
+                      ab * Mk.r + ab * Ml.r;
+                t.i = tj.i - dot3c_i(ai, Mc) + nb * dot3c_i(ai, Md) - nb * dot3c_i(ai, Mf) + dot3c_i(ai, Mh) -
+                      ab * Mk.i + ab * Ml.i;
+                z[(i + 3 * j) * stride] = {s1 * t.r, s1 * t.i};
+            }
+        }
     } else {
         // M1 = sum w x.U, MU = sum w U, MX = sum w x x Rv, MR = sum w Rv with
         // U = xd x H1, Rv = xd H0 - H1, H0 = sum w h, H1 = sum w h y, h = -(gamma + 1/R) G / R
@@ -1068,7 +1184,8 @@ Strategy parse_qs(const bh_quadstrat *qs)
 OpParams parse_op(const bh_operator *op)
 {
     BH_REQUIRE(op != nullptr, "operator is NULL");
-    BH_REQUIRE(op->kind == BH_OP_MW_SINGLELAYER || op->kind == BH_OP_MW_DOUBLELAYER || op->kind == BH_OP_NCROSS,
+    BH_REQUIRE(op->kind == BH_OP_MW_SINGLELAYER || op->kind == BH_OP_MW_DOUBLELAYER || op->kind == BH_OP_NCROSS ||
+                   op->kind == BH_OP_MW_DOUBLELAYER_ROTATED || op->kind == BH_OP_IDENTITY,
                "operator kind %d is not supported", op->kind);
     OpParams p{op->kind, op->gamma[0], op->gamma[1], op->alpha[0], op->alpha[1], op->beta[0], op->beta[1]};
     for (double v : {p.gr, p.gi, p.ar, p.ai, p.br, p.bi})
@@ -1253,6 +1370,9 @@ void build_near(Ctx &ctx, Mesh &tm, Mesh &sm, const Strategy &st, const OpParams
         if (op.kind == BH_OP_MW_SINGLELAYER)
             k_sauter_schwab<BH_OP_MW_SINGLELAYER><<<grid, 256, 0, s>>>(tm.d_geo.p, sm.d_geo.p, nd.entries.p, lp, cnt,
                                                                        cls, gl, op, nd.blocks.p);
+        else if (op.kind == BH_OP_MW_DOUBLELAYER_ROTATED)
+            k_sauter_schwab<BH_OP_MW_DOUBLELAYER_ROTATED><<<grid, 256, 0, s>>>(tm.d_geo.p, sm.d_geo.p, nd.entries.p,
+                                                                               lp, cnt, cls, gl, op, nd.blocks.p);
         else
             k_sauter_schwab<BH_OP_MW_DOUBLELAYER><<<grid, 256, 0, s>>>(tm.d_geo.p, sm.d_geo.p, nd.entries.p, lp, cnt,
                                                                        cls, gl, op, nd.blocks.p);
@@ -1333,6 +1453,11 @@ void make_plan(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, Mesh &tm
 {
     P.st = parse_qs(qs);
     P.op = parse_op(op);
+    if (P.op.kind == BH_OP_MW_DOUBLELAYER_ROTATED && P.st.mode == kModeWiltonSauter)
+        BH_FAIL(BH_ERR_UNSUPPORTED, "DoubleLayerRotatedMW3D has no singularity extraction: "
+                                    "DoubleNumWiltonSauterQStrat does not apply (use DoubleNumQStrat or "
+                                    "DoubleNumSauterQstrat)");
+    BH_REQUIRE(P.op.kind != BH_OP_NCROSS && P.op.kind != BH_OP_IDENTITY, "local operators have no pair plan");
     P.nft = tm.nf;
     P.nfs = sm.nf;
     build_near(ctx, tm, sm, P.st, P.op, P.nd);
@@ -1402,7 +1527,11 @@ void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long lon
         else if (dflt) k_far_tiles<R, OPK, RM, 3, 4><<<grid, 256, 0, ctx.stream>>>(A);      \
         else k_far_tiles<R, OPK, RM, 0, 0><<<grid, 256, 0, ctx.stream>>>(A);                \
     } while (0)
-    if (rowmajor) {
+    if (P.op.kind == BH_OP_MW_DOUBLELAYER_ROTATED) {
+        // any rule pair through the generic point loops
+        if (rowmajor) k_far_tiles<R, BH_OP_MW_DOUBLELAYER_ROTATED, true, 0, 0><<<grid, 256, 0, ctx.stream>>>(A);
+        else k_far_tiles<R, BH_OP_MW_DOUBLELAYER_ROTATED, false, 0, 0><<<grid, 256, 0, ctx.stream>>>(A);
+    } else if (rowmajor) {
         if (sl) BH_TILES(BH_OP_MW_SINGLELAYER, true);
         else BH_TILES(BH_OP_MW_DOUBLELAYER, true);
     } else {
@@ -1415,17 +1544,17 @@ void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long lon
 
 }  // namespace
 
-// Assemble into a device buffer (column-major or row-major, ld given).
-// NCross on RT x RT over one mesh (identityop.jl:20-22 with the matched-mesh
-// loop of localop.jl:114-146): Z[m, n] += scale a (sum_q w_q f_i(x_q) . (n x f_j(x_q))) b
-// over the cells m and n share, f_i = (x - v_i) / (2A).  One thread per test
-// dof, which alone writes its row, in shape order (deterministic).
-template <typename R, bool ROWMAJOR>
-__global__ __launch_bounds__(256) void k_ncross(const double *__restrict__ geo, const double *__restrict__ qp, int Q,
-                                                int ndofs, const int *__restrict__ ptr, const int *__restrict__ cell,
-                                                const int *__restrict__ ref, const double *__restrict__ coeff,
-                                                const int *__restrict__ e2d, const double *__restrict__ e2c, int kmax,
-                                                cx<R> *Z, long long ld, double sr, double si)
+// Local operators on RT x RT over one mesh (the matched-mesh loop of
+// localop.jl:114-146): Z[m, n] += scale a (sum_q w_q f_i(x_q) . T f_j(x_q)) b over the
+// cells m and n share, f_i = (x - v_i) / (2A); T f = n x f for NCross
+// (identityop.jl:20-22) and T f = f for Identity (identityop.jl:9-10).  One thread per
+// test dof, which alone writes its row, in shape order (deterministic).
+template <typename R, bool ROWMAJOR, int KIND>
+__global__ __launch_bounds__(256) void k_localop(const double *__restrict__ geo, const double *__restrict__ qp, int Q,
+                                                 int ndofs, const int *__restrict__ ptr, const int *__restrict__ cell,
+                                                 const int *__restrict__ ref, const double *__restrict__ coeff,
+                                                 const int *__restrict__ e2d, const double *__restrict__ e2c, int kmax,
+                                                 cx<R> *Z, long long ld, double sr, double si)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= ndofs) return;
@@ -1445,9 +1574,13 @@ __global__ __launch_bounds__(256) void k_ncross(const double *__restrict__ geo, 
             for (int j = 0; j < 3; ++j) {
                 const double fj[3] = {(x[0] - g[3 * j]) * s2a, (x[1] - g[3 * j + 1]) * s2a,
                                       (x[2] - g[3 * j + 2]) * s2a};
-                double nf[3];
-                cross3(nf, nrm, fj);
-                loc[j] += pq[3] * dot3(fi, nf);
+                if constexpr (KIND == BH_OP_NCROSS) {
+                    double nf[3];
+                    cross3(nf, nrm, fj);
+                    loc[j] += pq[3] * dot3(fi, nf);
+                } else {
+                    loc[j] += pq[3] * dot3(fi, fj);
+                }
             }
         }
         for (int j = 0; j < 3; ++j)
@@ -1462,13 +1595,14 @@ __global__ __launch_bounds__(256) void k_ncross(const double *__restrict__ geo, 
     }
 }
 
-void assemble_ncross(Ctx &ctx, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ, long long ld,
-                     bool rowmajor, bool accumulate, double sr, double si)
+void assemble_local(Ctx &ctx, int kind, const bh_quadstrat *qs, bool fp32, Space &X, Space &Y, void *dZ, long long ld,
+                    bool rowmajor, bool accumulate, double sr, double si)
 {
+    const char *name = kind == BH_OP_NCROSS ? "NCross" : "Identity";
     BH_REQUIRE(qs != nullptr, "quadstrat is NULL");
     if (qs->kind != BH_QS_SINGLENUM)
-        BH_FAIL(BH_ERR_UNSUPPORTED, "NCross needs SingleNumQStrat (quadstrat kind %d given)", qs->kind);
-    if (X.mesh != Y.mesh) BH_FAIL(BH_ERR_UNSUPPORTED, "NCross on different test and trial meshes has no GPU path");
+        BH_FAIL(BH_ERR_UNSUPPORTED, "%s needs SingleNumQStrat (quadstrat kind %d given)", name, qs->kind);
+    if (X.mesh != Y.mesh) BH_FAIL(BH_ERR_UNSUPPORTED, "%s on different test and trial meshes has no GPU path", name);
     const int degree = qs->params[0];
     BH_REQUIRE(degree >= 1 && degree <= 8, "SingleNumQStrat: triangle rule must be degree 1..8");
     const int64_t M = X.ndofs, N = Y.ndofs;
@@ -1480,19 +1614,26 @@ void assemble_ncross(Ctx &ctx, const bh_quadstrat *qs, bool fp32, Space &X, Spac
     int Q = 0;
     const double *qp = (const double *)quad_points(*X.mesh, degree, &Q, false, ctx.stream);
     const unsigned grid = (unsigned)((M + 255) / 256);
-    ProfScope prof(ctx, "k_ncross", (double)M * 2 * Q * 3 * 12.0);
-#define BH_NCROSS(R, RM)                                                                                          \
-    k_ncross<R, RM><<<grid, 256, 0, ctx.stream>>>(X.mesh->d_geo.p, qp, Q, (int)M, X.d_ptr.p, X.d_cell.p, X.d_ref.p, \
-                                                  X.d_coeff.p, Y.d_e2d.p, Y.d_e2c.p, Y.kmax, (cx<R> *)dZ, ld, sr, si)
+    ProfScope prof(ctx, "k_localop", (double)M * 2 * Q * 3 * 12.0);
+#define BH_LOCAL(R, RM, K)                                                                                        \
+    k_localop<R, RM, K><<<grid, 256, 0, ctx.stream>>>(X.mesh->d_geo.p, qp, Q, (int)M, X.d_ptr.p, X.d_cell.p,     \
+                                                      X.d_ref.p, X.d_coeff.p, Y.d_e2d.p, Y.d_e2c.p, Y.kmax,       \
+                                                      (cx<R> *)dZ, ld, sr, si)
+#define BH_LOCAL_K(R, RM)                                                                                         \
+    do {                                                                                                          \
+        if (kind == BH_OP_NCROSS) BH_LOCAL(R, RM, BH_OP_NCROSS);                                                 \
+        else BH_LOCAL(R, RM, BH_OP_IDENTITY);                                                                    \
+    } while (0)
     if (fp32) {
-        if (rowmajor) BH_NCROSS(float, true);
-        else BH_NCROSS(float, false);
+        if (rowmajor) BH_LOCAL_K(float, true);
+        else BH_LOCAL_K(float, false);
     } else {
-        if (rowmajor) BH_NCROSS(double, true);
-        else BH_NCROSS(double, false);
+        if (rowmajor) BH_LOCAL_K(double, true);
+        else BH_LOCAL_K(double, false);
     }
-#undef BH_NCROSS
-    check_launch("k_ncross");
+#undef BH_LOCAL_K
+#undef BH_LOCAL
+    check_launch("k_localop");
     BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
 }
 
@@ -1500,9 +1641,9 @@ bool assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bo
                      long long ld, bool rowmajor, bool accumulate, double sr, double si, void *dZ32, long long ld32)
 {
     BH_REQUIRE(op != nullptr, "operator is NULL");
-    if (op->kind == BH_OP_NCROSS) {
+    if (op->kind == BH_OP_NCROSS || op->kind == BH_OP_IDENTITY) {
         if (X.ndofs == 0 || Y.ndofs == 0) return false;
-        assemble_ncross(ctx, qs, fp32, X, Y, dZ, ld, rowmajor, accumulate, sr, si);
+        assemble_local(ctx, op->kind, qs, fp32, X, Y, dZ, ld, rowmajor, accumulate, sr, si);
         return false;
     }
     Plan P;
@@ -1587,6 +1728,9 @@ extern "C" int bh_local_blocks(bh_ctx *ctx, const bh_operator *op, const bh_quad
     const unsigned grid = (unsigned)((npairs + 127) / 128);
     if (P.op.kind == BH_OP_MW_SINGLELAYER)
         k_far_pairs<double, BH_OP_MW_SINGLELAYER><<<grid, 128, 0, ctx->stream>>>(P.far, tc.p, sc.p, (int)npairs, out.p);
+    else if (P.op.kind == BH_OP_MW_DOUBLELAYER_ROTATED)
+        k_far_pairs<double, BH_OP_MW_DOUBLELAYER_ROTATED><<<grid, 128, 0, ctx->stream>>>(P.far, tc.p, sc.p,
+                                                                                         (int)npairs, out.p);
     else
         k_far_pairs<double, BH_OP_MW_DOUBLELAYER><<<grid, 128, 0, ctx->stream>>>(P.far, tc.p, sc.p, (int)npairs, out.p);
     check_launch("k_far_pairs");

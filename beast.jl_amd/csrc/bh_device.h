@@ -155,4 +155,17 @@ __device__ __forceinline__ R dot3(const R *a, const R *b)
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
+// real and imaginary parts of the dot product of a real and a complex 3-vector
+template <typename R>
+__device__ __forceinline__ R dot3c_r(const R *a, const cx<R> *b)
+{
+    return a[0] * b[0].r + a[1] * b[1].r + a[2] * b[2].r;
+}
+
+template <typename R>
+__device__ __forceinline__ R dot3c_i(const R *a, const cx<R> *b)
+{
+    return a[0] * b[0].i + a[1] * b[1].i + a[2] * b[2].i;
+}
+
 }  // namespace bh

@@ -3,10 +3,13 @@
 Mirrors the reference's constructors and parameter handling:
 ``Maxwell3D.singlelayer/doublelayer/planewave/farfield`` (src/maxwell/maxwell.jl:16-107),
 ``MWSingleLayer3D``/``MWDoubleLayer3D`` (src/maxwell/mwops.jl:11-64),
+``DoubleLayerRotatedMW3D`` (src/maxwell/nxdbllayer.jl:17-44), the local operators
+``Identity`` and ``NCross`` (src/identityop.jl:6-22),
 ``gamma_wavenumber_handler`` (mwops.jl:213-238), the quadrature strategies
 (src/quadrature/doublenumwiltonsauterqstrat.jl:1-10, doublenumsauterqstrat.jl:1-8,
 doublenumqstrat.jl:1-4), ``defaultquadstrat`` (mwops.jl:55), and the traces
-``(n x E) x n`` (src/excitation.jl:149-198, src/maxwell/mwexc.jl:1-52).
+``(n x E) x n`` and ``n x E`` (src/excitation.jl:145-180) of plane waves and their curl
+(src/maxwell/mwexc.jl:1-54).
 """
 import numbers
 
@@ -78,6 +81,8 @@ def defaultquadstrat(op, X=None, Y=None):
         return SingleNumQStrat(8)
     if isinstance(op, LocalOperator):
         return SingleNumQStrat(6)
+    if isinstance(op, DoubleLayerRotatedMW3D):
+        return DoubleNumQStrat(6, 7)   # nxdbllayer.jl:23
     return DoubleNumWiltonSauterQStrat(2, 3, 6, 7, 5, 5, 4, 3)
 
 
@@ -117,6 +122,9 @@ class IntegralOperator:
     def __neg__(self):
         return self * -1
 
+    def __sub__(self, other):
+        return self + (-1.0) * other
+
 
 class MWSingleLayer3D(IntegralOperator):
     """EFIE kernel G (alpha f.g + beta div f div g), G = exp(-gamma R)/(4 pi R)."""
@@ -150,6 +158,20 @@ class MWDoubleLayer3D(IntegralOperator):
         return f"MWDoubleLayer3D(alpha={self.alpha}, gamma={self.gamma})"
 
 
+class DoubleLayerRotatedMW3D(IntegralOperator):
+    """n x K (nxdbllayer.jl:17-44): integrand f . (n x (grad G x g)), n the test cell's
+    normal; like the reference's integrand, alpha is not used by the kernel."""
+
+    kind = 3
+
+    def __init__(self, alpha, gamma):
+        self.alpha = complex(alpha)
+        self.gamma = complex(gamma)
+
+    def __repr__(self):
+        return f"DoubleLayerRotatedMW3D(alpha={self.alpha}, gamma={self.gamma})"
+
+
 class LocalOperator(IntegralOperator):
     """Operators whose matrix couples only functions sharing a cell (localop.jl:4)."""
 
@@ -158,8 +180,19 @@ class NCross(LocalOperator):
     """NCross (identityop.jl:14-22): the identity with the trial function rotated,
     integrand g . (n x f) for test g and trial f."""
 
+    kind = 2
+
     def __repr__(self):
         return "NCross()"
+
+
+class Identity(LocalOperator):
+    """Identity (identityop.jl:6-11): integrand f . g on the common cells."""
+
+    kind = 4
+
+    def __repr__(self):
+        return "Identity()"
 
 
 class LinearCombinationOfOperators(IntegralOperator):
@@ -172,6 +205,9 @@ class LinearCombinationOfOperators(IntegralOperator):
         return LinearCombinationOfOperators([(c * complex(a), op) for c, op in self.terms])
 
     __rmul__ = __mul__
+
+    def __sub__(self, other):
+        return self + (-1.0) * other
 
     def __add__(self, other):
         if isinstance(other, LinearCombinationOfOperators):
@@ -237,6 +273,20 @@ class PlaneWaveMW:
         x = np.asarray(x, dtype=np.float64)
         return self.amplitude * np.exp(-self.gamma * (x @ self.direction))[..., None] * self.polarisation
 
+    def __rmul__(self, a):
+        """a * E (mwexc.jl:54)."""
+        if not isinstance(a, numbers.Number):
+            return NotImplemented
+        return PlaneWaveMW(self.direction, self.polarisation, self.gamma, complex(a) * self.amplitude)
+
+
+def curl(field):
+    """curl of a plane wave: PlaneWaveMW(d, d x u, gamma, -a gamma) (mwexc.jl:44-52)."""
+    if not isinstance(field, PlaneWaveMW):
+        raise TypeError("curl is defined for plane waves")
+    return PlaneWaveMW(field.direction, np.cross(field.direction, field.polarisation), field.gamma,
+                       -field.amplitude * field.gamma)
+
 
 class NormalVector:
     """The symbol ``n`` of the reference (cross(n, E) builds traces)."""
@@ -270,6 +320,8 @@ def cross(a, b):
     """``cross(n, E) -> CrossTraceMW``, ``cross(CrossTraceMW, n) -> TangTraceMW`` (excitation.jl:166-167)."""
     if isinstance(a, NormalVector) and isinstance(b, FarField):
         return b._ncross()
+    if isinstance(a, NormalVector) and isinstance(b, MWDoubleLayer3D):
+        return DoubleLayerRotatedMW3D(b.alpha, b.gamma)   # nxdbllayer.jl:25
     if isinstance(a, NormalVector) and not isinstance(b, NormalVector):
         return CrossTraceMW(b)
     if isinstance(a, CrossTraceMW) and isinstance(b, NormalVector):

@@ -26,7 +26,7 @@ import numpy as np
 
 from . import assembly as asm
 from .mesh import meshcube, meshellipsoid, meshsphere
-from .operators import Maxwell3D, NCross, cross, n as nvec
+from .operators import Identity, Maxwell3D, cross, curl, n as nvec
 from .rt import raviartthomas
 
 
@@ -73,16 +73,6 @@ def random_rhs(rng, count):
     return out
 
 
-def served_kind(shape, kind):
-    """The formulation a request is served with.  The MFIE here is tested with RT
-    functions (K + N/2, N = NCross on RT x RT); on flat faces K vanishes for
-    coplanar pairs and the RT x (n x RT) pairing left is nearly singular (condition
-    numbers of 1e11-1e12 on the cube against 4e4 on a sphere of the same size; the
-    reference's examples/mfie.jl tests with Buffa-Christiansen functions instead),
-    so flat-faced geometries are served with the EFIE."""
-    return "efie" if (kind == "mfie" and shape == "cube") else kind
-
-
 def batched_requests(seed=4, count=64, nmin=500, nmax=8000, kinds=("efie",)):
     """The batched workload: all requests at t = 0, N log-uniform, k ~ U[0.5, 8], 1-4 right-hand sides."""
     rng = np.random.default_rng(seed)
@@ -91,7 +81,7 @@ def batched_requests(seed=4, count=64, nmin=500, nmax=8000, kinds=("efie",)):
         shape = ["sphere", "ellipsoid", "cube"][rng.integers(3)]
         dofs = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
         k = float(rng.uniform(0.5, 8.0))
-        kind = served_kind(shape, kinds[rng.integers(len(kinds))])
+        kind = kinds[rng.integers(len(kinds))]
         reqs.append(Request(i, 0.0, shape, dofs, kind, k, random_rhs(rng, int(rng.integers(1, 5)))))
     return reqs
 
@@ -107,7 +97,7 @@ def poisson_requests(seed=5, rate=20.0, count=200, nmin=500, nmax=30000, kinds=(
         else:
             shape = ["sphere", "ellipsoid", "cube"][rng.integers(3)]
             dofs = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
-            kind = served_kind(shape, kinds[rng.integers(len(kinds))])
+            kind = kinds[rng.integers(len(kinds))]
             k = float(rng.uniform(0.5, 8.0))
             seen.append((shape, dofs, kind, k))
         reqs.append(Request(i, t, shape, dofs, kind, k, random_rhs(rng, int(rng.integers(1, 5)))))
@@ -159,12 +149,13 @@ class Server:
 
     @staticmethod
     def _operator(kind, k):
-        """EFIE: T.  MFIE: K + N/2 with N = NCross, both tested with RT (examples/mfie.jl
-        tests with Buffa-Christiansen functions; the RT-tested form keeps one space)."""
+        """EFIE: T.  MFIE: n x K - Id/2 tested with RT, the reference's RT-tested MFIE
+        (examples/nxmfie.jl:9-22; n x K = DoubleLayerRotatedMW3D, its default quadstrat
+        DoubleNumQStrat(6,7), nxdbllayer.jl:23)."""
         if kind == "efie":
             return Maxwell3D.singlelayer(wavenumber=k)
         if kind == "mfie":
-            return Maxwell3D.doublelayer(wavenumber=k) + 0.5 * NCross()
+            return cross(nvec, Maxwell3D.doublelayer(wavenumber=k)) - 0.5 * Identity()
         raise ValueError(f"request kind {kind!r} is not served")
 
     def process(self, req, ctx):
@@ -217,11 +208,13 @@ class Server:
 
     @staticmethod
     def _rhs(X, kind, k, d, p, ctx):
-        """EFIE: (n x E) x n.  MFIE: (n x H) x n with H = -curl E / (i mu omega), which for
-        the plane wave p exp(-i k d.x) (eps = mu = 1, omega = k) is the plane wave d x p."""
-        pol = p if kind == "efie" else np.cross(d, p)
-        F = Maxwell3D.planewave(direction=d, polarization=pol, wavenumber=k)
-        return asm.assemble_rhs(cross(cross(nvec, F), nvec), X, ctx=ctx)
+        """EFIE: (n x E) x n.  MFIE: n x H with H = -1/(i mu omega) curl E (examples/nxmfie.jl:12-16),
+        eps = mu = 1, omega = k."""
+        E = Maxwell3D.planewave(direction=d, polarization=p, wavenumber=k)
+        if kind == "efie":
+            return asm.assemble_rhs(cross(cross(nvec, E), nvec), X, ctx=ctx)
+        H = (-1.0 / (1j * k)) * curl(E)
+        return asm.assemble_rhs(cross(nvec, H), X, ctx=ctx)
 
     def run(self, requests, realtime=True):
         """Serve ``requests`` (arrival times in seconds); returns the summary statistics."""

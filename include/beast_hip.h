@@ -37,10 +37,27 @@ enum bh_status {
     BH_ERR_NOCONV = -7        /* iterative refinement did not reach the tolerance */
 };
 
-/* Operator kinds (src/maxwell/maxwell.jl:16-79; BH_OP_NCROSS is the local
- * operator NCross, src/identityop.jl:14-22: <f, n x g> on common cells of one
- * mesh, gamma/alpha/beta unused, quadstrat BH_QS_SINGLENUM). */
-enum bh_op_kind { BH_OP_MW_SINGLELAYER = 0, BH_OP_MW_DOUBLELAYER = 1, BH_OP_NCROSS = 2 };
+/* Operator kinds:
+ *   BH_OP_MW_SINGLELAYER          MWSingleLayer3D (src/maxwell/mwops.jl:11-15,101-117)
+ *   BH_OP_MW_DOUBLELAYER          MWDoubleLayer3D (mwops.jl:61-64,141-155)
+ *   BH_OP_NCROSS                  NCross, <f, n x g> on the common cells of one mesh
+ *                                 (src/identityop.jl:14-22); quadstrat BH_QS_SINGLENUM
+ *   BH_OP_MW_DOUBLELAYER_ROTATED  DoubleLayerRotatedMW3D, f . (n x (grad G x g)) with n the
+ *                                 test cell's normal (src/maxwell/nxdbllayer.jl:17-44); its
+ *                                 default quadstrat is DoubleNumQStrat(6,7) (:23); it has no
+ *                                 singularity extraction, so BH_QS_WILTON_SAUTER is
+ *                                 BH_ERR_UNSUPPORTED (the reference has no regular part for it)
+ *   BH_OP_IDENTITY                Identity, <f, g> on the common cells of one mesh
+ *                                 (src/identityop.jl:6-11); quadstrat BH_QS_SINGLENUM
+ * gamma/alpha/beta are unused by the local operators; like the reference's integrands,
+ * the two double layers use gamma only. */
+enum bh_op_kind {
+    BH_OP_MW_SINGLELAYER = 0,
+    BH_OP_MW_DOUBLELAYER = 1,
+    BH_OP_NCROSS = 2,
+    BH_OP_MW_DOUBLELAYER_ROTATED = 3,
+    BH_OP_IDENTITY = 4
+};
 
 /* Quadrature strategy kinds; the integer parameters follow the field order of
  * the reference structs:
@@ -157,6 +174,16 @@ int bh_local_blocks(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, 
 int bh_assemble_planewave(bh_ctx *ctx, const bh_space *X, const double *direction, const double *pol_re,
                           const double *pol_im, const double *gamma, const double *amplitude, int rule,
                           int accumulate, double *b);
+/* The plane wave through either trace of the reference (src/excitation.jl:145-180):
+ * trace BH_TRACE_TANGENTIAL is (n x E) x n (TangTraceMW, as bh_assemble_planewave),
+ * BH_TRACE_CROSS is n x E (CrossTraceMW), n the cell's normal; e.g. the right-hand side
+ * n x H of the MFIE (examples/nxmfie.jl:14-22) with H = -1/(i mu omega) curl E, a plane
+ * wave itself (curl, src/maxwell/mwexc.jl:44-52). */
+#define BH_TRACE_TANGENTIAL 0
+#define BH_TRACE_CROSS 1
+int bh_assemble_excitation(bh_ctx *ctx, const bh_space *X, int trace, const double *direction, const double *pol_re,
+                           const double *pol_im, const double *gamma, const double *amplitude, int rule,
+                           int accumulate, double *b);
 
 /* Far field of a surface current (potential(op, points, coeffs, X), src/postproc.jl:138-150,174-211,
  * with the far-field operators of src/maxwell/farfield.jl:1-120): for each point y (a direction,

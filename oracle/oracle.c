@@ -2,7 +2,8 @@
  * CPU oracle -- TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C FP64 restatement of the reference's Galerkin assembly of the
- * Maxwell EFIE (MWSingleLayer3D) and MFIE-K (MWDoubleLayer3D) operators on
+ * Maxwell EFIE (MWSingleLayer3D), MFIE-K (MWDoubleLayer3D) and n x K
+ * (DoubleLayerRotatedMW3D) operators on
  * Raviart-Thomas spaces, and of the plane-wave right-hand side.  It is used
  * only by tests/, __graft_entry__.smoke() and bench.py's CPU baseline, as the
  * checker.  The product (the HIP library) never links or calls it.
@@ -27,7 +28,8 @@
  *                       which is not available here)
  *   scatter             src/integralop.jl:146-184, src/operator.jl:168-184
  *   plane-wave RHS      src/excitation.jl:38-115,149-198, src/maxwell/mwexc.jl:36-42
- *   NCross              src/identityop.jl:14-22, src/localop.jl:114-146,362-386
+ *   rotated double layer src/maxwell/nxdbllayer.jl:17-44
+ *   NCross, Identity    src/identityop.jl:6-22, src/localop.jl:114-146,362-386
  *   far fields          src/postproc.jl:174-211,273-288, src/maxwell/farfield.jl:56-75,98-100
  *
  * Parity pins (see tests/test_oracle_*.py): Dunavant exactness, Wilton vs brute
@@ -177,9 +179,15 @@ static double tri_area(const double *v /* 9 */)
 /* Operator description                                                 */
 /* ------------------------------------------------------------------ */
 typedef struct {
-    int kind;            /* 0 = MWSingleLayer3D, 1 = MWDoubleLayer3D */
+    int kind;            /* 0 = MWSingleLayer3D, 1 = MWDoubleLayer3D, 3 = DoubleLayerRotatedMW3D */
     cplx gamma, alpha, beta;
+    double nx[3];        /* unit normal of the test cell (kind 3) */
 } op_t;
+
+enum { OR_OP_SL = 0, OR_OP_DL = 1, OR_OP_DL_ROTATED = 3 };
+
+/* unit normal (v1 - v3) x (v2 - v3) / |.| of a cell, the chart normal */
+static void cell_normal(const double *tv, double *nrm);
 
 /* ------------------------------------------------------------------ */
 /* Classification (doublenumwiltonsauterqstrat.jl:32-69)                */
@@ -258,6 +266,21 @@ static void integrand_add(const op_t *op, int regular, const double *x, const do
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j)
                 z[i][j] += wgt * (aG * dot3(f[i], g[j]) + bG * (df[i] * dg[j]));
+    } else if (op->kind == OR_OP_DL_ROTATED) {
+        /* nxdbllayer.jl:27-44: K = -(gamma + 1/R) G r/R, f . (nx x (K x g)); the
+         * reference has no regular/singular split for this operator */
+        cplx green = cexp(-gam * R) / (4.0 * OR_PI * R);
+        cplx K[3];
+        for (int k = 0; k < 3; ++k) K[k] = -(gam + iR) * green * (iR * r[k]);
+        const double *nx = op->nx;
+        for (int j = 0; j < 3; ++j) {
+            cplx Kg[3] = {K[1] * g[j][2] - K[2] * g[j][1], K[2] * g[j][0] - K[0] * g[j][2],
+                          K[0] * g[j][1] - K[1] * g[j][0]};
+            cplx nxKg[3] = {nx[1] * Kg[2] - nx[2] * Kg[1], nx[2] * Kg[0] - nx[0] * Kg[2],
+                            nx[0] * Kg[1] - nx[1] * Kg[0]};
+            for (int i = 0; i < 3; ++i)
+                z[i][j] += wgt * (f[i][0] * nxKg[0] + f[i][1] * nxKg[1] + f[i][2] * nxKg[2]);
+        }
     } else {
         cplx gg;
         if (!regular) {
@@ -299,6 +322,16 @@ static int tri_points(const double *tv, int degree, double *pts, double *wts)
         wts[q] = w[q] * 2.0 * A;
     }
     return n;
+}
+
+static void cell_normal(const double *tv, double *nrm)
+{
+    double t1[3], t2[3];
+    sub3(t1, tv, tv + 6);
+    sub3(t2, tv + 3, tv + 6);
+    cross3(nrm, t1, t2);
+    const double nn = norm3(nrm);
+    for (int q = 0; q < 3; ++q) nrm[q] /= nn;
 }
 
 /* DoubleQuadRule (doublenumints.jl:12-43) */
@@ -598,7 +631,11 @@ static void block_ss(const op_t *op, int cls, const int *ti, const int *sj, cons
 int or_block(int op_kind, const double *g, const double *al, const double *be, int qs_kind, const int *qs,
              const double *tv, const double *sv, double *zout /* 18, column-major 3x3 complex */, int *cls_out)
 {
-    op_t op = {op_kind, g[0] + I * g[1], al[0] + I * al[1], be[0] + I * be[1]};
+    op_t op = {op_kind, g[0] + I * g[1], al[0] + I * al[1], be[0] + I * be[1], {0, 0, 0}};
+    /* the rotated double layer has no singularity extraction in the reference
+     * (no regularpart/singularpart), so DoubleNumWiltonSauterQStrat does not apply */
+    if (op_kind == OR_OP_DL_ROTATED && qs_kind == 0) return -3;
+    cell_normal(tv, op.nx);
     cplx z[3][3];
     memset(z, 0, sizeof z);
     int ti[3], sj[3];
@@ -627,9 +664,11 @@ int or_block(int op_kind, const double *g, const double *al, const double *be, i
 int or_block_strategy(int op_kind, const double *g, const double *al, const double *be, int strat,
                       int deg_o, int deg_i, int n, const double *tv, const double *sv, double *zout)
 {
-    op_t op = {op_kind, g[0] + I * g[1], al[0] + I * al[1], be[0] + I * be[1]};
+    op_t op = {op_kind, g[0] + I * g[1], al[0] + I * al[1], be[0] + I * be[1], {0, 0, 0}};
+    cell_normal(tv, op.nx);
     cplx z[3][3];
     memset(z, 0, sizeof z);
+    if (strat == 1 && op_kind == OR_OP_DL_ROTATED) return -3;
     if (strat == 0) block_dq(&op, 0, tv, sv, deg_o, deg_i, z);
     else if (strat == 1) block_wilton(&op, tv, sv, deg_o, deg_i, z);
     else {
@@ -699,9 +738,11 @@ int or_assemble(int op_kind, const double *g, const double *al, const double *be
                         sv[3 * a + k] = Vs[3 * Cs[3 * q + a] + k];
                     }
                 int cls = 0;
-                if (or_block(op_kind, g, al, be, qs_kind, qs, tv, sv, blk + 18 * ((size_t)pp * nfs + q), &cls) != 0) {
+                const int rc = or_block(op_kind, g, al, be, qs_kind, qs, tv, sv, blk + 18 * ((size_t)pp * nfs + q),
+                                        &cls);
+                if (rc != 0) {
                     #pragma omp atomic write
-                    err = -1;
+                    err = rc;
                 }
                 if (cls_out) cls_out[(size_t)p * nfs + q] = (int8_t)cls;
             }
@@ -735,17 +776,20 @@ int or_assemble(int op_kind, const double *g, const double *al, const double *be
 }
 
 /* ------------------------------------------------------------------ */
-/* NCross on RT x RT over one mesh (identityop.jl:20-22; the matched-   */
-/* mesh loop localop.jl:114-146 with cellinteractions_matched!,         */
-/* localop.jl:362-386; SingleNumQStrat(degree), identityop.jl:66-67):   */
-/*   locmat[i,j] = sum_q w_q f_i(x_q) . (n x f_j(x_q)) per cell,         */
-/*   Z[m,n] += scale * a * locmat[i,j] * b.                              */
-/* Z column-major, complex interleaved, ld = ntest.                      */
+/* Local operators on RT x RT over one mesh (the matched-mesh loop         */
+/* localop.jl:114-146 with cellinteractions_matched!, localop.jl:362-386;  */
+/* SingleNumQStrat(degree), identityop.jl:66-67):                          */
+/*   kind 0 NCross   (identityop.jl:20-22): locmat[i,j] = sum_q w_q f_i . (n x f_j) */
+/*   kind 1 Identity (identityop.jl:9-10):  locmat[i,j] = sum_q w_q f_i . f_j        */
+/*   Z[m,n] += scale * a * locmat[i,j] * b.                                  */
+/* Z column-major, complex interleaved, ld = ntest.                          */
 /* ------------------------------------------------------------------ */
-int or_ncross(const double *V, const int32_t *C, int nf, int nd, const int32_t *tptr, const int32_t *tcell,
-              const int32_t *tref, const double *tcoef, int ns, const int32_t *sptr, const int32_t *scell,
-              const int32_t *sref, const double *scoef, int degree, const double *scale, double *Z)
+int or_localop(int kind, const double *V, const int32_t *C, int nf, int nd, const int32_t *tptr,
+                const int32_t *tcell, const int32_t *tref, const double *tcoef, int ns, const int32_t *sptr,
+                const int32_t *scell, const int32_t *sref, const double *scoef, int degree, const double *scale,
+                double *Z)
 {
+    if (kind != 0 && kind != 1) return -2;
     int Ks, Kt;
     slot_t *sm = build_cellmap(nf, ns, sptr, scell, sref, scoef, &Ks);
     slot_t *tm = build_cellmap(nf, nd, tptr, tcell, tref, tcoef, &Kt);
@@ -774,8 +818,12 @@ int or_ncross(const double *V, const int32_t *C, int nf, int nd, const int32_t *
                         fi[q] = (x[q] - tv[3 * i + q]) / (2.0 * A);
                         fj[q] = (x[q] - tv[3 * j + q]) / (2.0 * A);
                     }
-                    cross3(nf_, nrm, fj);
-                    loc[i][j] += wx[k] * dot3(fi, nf_);
+                    if (kind == 0) {
+                        cross3(nf_, nrm, fj);
+                        loc[i][j] += wx[k] * dot3(fi, nf_);
+                    } else {
+                        loc[i][j] += wx[k] * dot3(fi, fj);
+                    }
                 }
         }
         /* store(a * locmat[i,j] * b, m, n) over the test and trial slots of the cell */
@@ -799,15 +847,19 @@ int or_ncross(const double *V, const int32_t *C, int nf, int nd, const int32_t *
 }
 
 /* ------------------------------------------------------------------ */
-/* Plane-wave right-hand side (excitation.jl:38-115, mwexc.jl:36-42):   */
-/* b[m] += a * sum_p w_p f_i(x_p) . (amp exp(-gamma d.x) pol)            */
-/* (the tangential trace (n x E) x n tested with tangential f_i equals   */
-/* f_i . E).                                                             */
+/* Plane-wave right-hand side (excitation.jl:38-115, mwexc.jl:36-42),   */
+/* E(x) = amp exp(-gamma d.x) pol:                                        */
+/*   trace 0, TangTraceMW (n x E) x n (excitation.jl:175-180):           */
+/*     b[m] += a * sum_p w_p f_i(x_p) . E(x_p) (f_i tangential);          */
+/*   trace 1, CrossTraceMW n x E (excitation.jl:168-173):                */
+/*     b[m] += a * sum_p w_p f_i(x_p) . (n x E(x_p)).                     */
 /* ------------------------------------------------------------------ */
-int or_planewave(const double *V, const int32_t *C, int nf, int nd, const int32_t *ptr, const int32_t *cell,
-                 const int32_t *ref, const double *coef, const double *dir, const double *pol_re,
-                 const double *pol_im, const double *gam, const double *amp, int degree, double *b)
+int or_planewave(int trace, const double *V, const int32_t *C, int nf, int nd, const int32_t *ptr,
+                 const int32_t *cell, const int32_t *ref, const double *coef, const double *dir,
+                 const double *pol_re, const double *pol_im, const double *gam, const double *amp, int degree,
+                 double *b)
 {
+    if (trace != 0 && trace != 1) return -2;
     cplx gm = gam[0] + I * gam[1], am = amp[0] + I * amp[1];
     for (int m = 0; m < nd; ++m)
         for (int k = ptr[m]; k < ptr[m + 1]; ++k) {
@@ -819,15 +871,24 @@ int or_planewave(const double *V, const int32_t *C, int nf, int nd, const int32_
             int np = tri_points(tv, degree, xp, wx);
             if (np < 0) return -1;
             double A = tri_area(tv);
+            double nrm[3];
+            cell_normal(tv, nrm);
             cplx acc = 0;
             for (int p = 0; p < np; ++p) {
                 const double *x = xp + 3 * p;
                 cplx ph = am * cexp(-gm * dot3(dir, x));
                 double f[3];
                 for (int q = 0; q < 3; ++q) f[q] = (x[q] - tv[3 * r + q]) / (2.0 * A);
-                cplx e = 0;
-                for (int q = 0; q < 3; ++q) e += f[q] * (pol_re[q] + I * pol_im[q]);
-                acc += wx[p] * (e * ph);
+                cplx E[3], e = 0;
+                for (int q = 0; q < 3; ++q) E[q] = ph * (pol_re[q] + I * pol_im[q]);
+                if (trace == 1) {
+                    cplx nE[3] = {nrm[1] * E[2] - nrm[2] * E[1], nrm[2] * E[0] - nrm[0] * E[2],
+                                  nrm[0] * E[1] - nrm[1] * E[0]};
+                    for (int q = 0; q < 3; ++q) e += f[q] * nE[q];
+                } else {
+                    for (int q = 0; q < 3; ++q) e += f[q] * E[q];
+                }
+                acc += wx[p] * e;
             }
             cplx v = coef[k] * acc;
             b[2 * m] += creal(v);

@@ -45,11 +45,12 @@ def lib():
                                   _D, _I32, C.c_int, C.c_int, _I32, _I32, _I32, _D,
                                   _D, C.POINTER(C.c_int8)]
         L.or_assemble.restype = C.c_int
-        L.or_planewave.argtypes = [_D, _I32, C.c_int, C.c_int, _I32, _I32, _I32, _D, _D, _D, _D, _D, _D, C.c_int, _D]
+        L.or_planewave.argtypes = [C.c_int, _D, _I32, C.c_int, C.c_int, _I32, _I32, _I32, _D, _D, _D, _D, _D, _D,
+                                   C.c_int, _D]
         L.or_planewave.restype = C.c_int
-        L.or_ncross.argtypes = [_D, _I32, C.c_int, C.c_int, _I32, _I32, _I32, _D, C.c_int, _I32, _I32, _I32, _D,
-                                C.c_int, _D, _D]
-        L.or_ncross.restype = C.c_int
+        L.or_localop.argtypes = [C.c_int, _D, _I32, C.c_int, C.c_int, _I32, _I32, _I32, _D, C.c_int, _I32, _I32,
+                                 _I32, _D, C.c_int, _D, _D]
+        L.or_localop.restype = C.c_int
         L.or_farfield.argtypes = [_D, _I32, C.c_int, C.c_int, _I32, _I32, _I32, _D, C.c_int, _D, _D, C.c_int, _D,
                                   _D, _D, C.c_int, _D]
         L.or_farfield.restype = C.c_int
@@ -104,8 +105,7 @@ def block(kind, gamma, alpha, beta, qs_kind, qs, tv, sv):
     out = np.zeros(18)
     cls = C.c_int()
     rc = lib().or_block(k, _d(g), _d(a), _d(b), int(qs_kind), q, _d(tv), _d(sv), _d(out), C.byref(cls))
-    if rc != 0:
-        raise AssertionError("hits <= 3")
+    _check_block(rc)
     z = (out[0::2] + 1j * out[1::2]).reshape(3, 3).T
     return z, cls.value
 
@@ -117,6 +117,8 @@ def block_strategy(kind, gamma, alpha, beta, strat, tv, sv, deg_o=6, deg_i=7, n=
     out = np.zeros(18)
     s = {"dq": 0, "se": 1, "ss": 2}[strat]
     rc = lib().or_block_strategy(k, _d(g), _d(a), _d(b), s, deg_o, deg_i, n, _d(tv), _d(sv), _d(out))
+    if rc == -3:
+        raise NotImplementedError("no singularity extraction for the rotated double layer")
     if rc != 0:
         raise ValueError("strategy not applicable to this pair")
     return (out[0::2] + 1j * out[1::2]).reshape(3, 3).T
@@ -143,36 +145,54 @@ def assemble(kind, gamma, alpha, beta, qs_kind, qs, X, Y, want_classes=False):
                            _d(Vt), i32(Ct), mt.numcells, nt, i32(tp), i32(tc), i32(tr), _d(tco),
                            _d(Vs), i32(Cs), ms.numcells, ns, i32(sp), i32(sc), i32(sr), _d(sco),
                            Z.ctypes.data_as(_D), None if cls is None else cls.ctypes.data_as(C.POINTER(C.c_int8)))
+    _check_block(rc)
+    return (Z, cls) if want_classes else Z
+
+
+def _check_block(rc):
+    if rc == -3:
+        # the reference has no regularpart/singularpart for DoubleLayerRotatedMW3D
+        raise NotImplementedError("DoubleNumWiltonSauterQStrat does not apply to the rotated double layer")
     if rc != 0:
         raise AssertionError("hits <= 3")
-    return (Z, cls) if want_classes else Z
 
 
 def ncross(X, Y, degree=6, scale=1.0):
     """Dense NCross matrix (test X, trial Y on the same mesh), column-major complex128."""
+    return localop(0, X, Y, degree, scale)
+
+
+def identity(X, Y, degree=6, scale=1.0):
+    """Dense Identity (RT x RT Gram) matrix on one mesh, column-major complex128."""
+    return localop(1, X, Y, degree, scale)
+
+
+def localop(kind, X, Y, degree=6, scale=1.0):
+    """Local operator on RT x RT over one mesh: kind 0 NCross, 1 Identity."""
     if X.geo is not Y.geo:
-        raise ValueError("NCross oracle: test and trial spaces must share the mesh")
+        raise ValueError("local operator oracle: test and trial spaces must share the mesh")
     m = X.geo
     V, Cc = np.ascontiguousarray(m.vertices, float), np.ascontiguousarray(m.cells, np.int32)
     nt, tp, tc, tr, tco = _space_arrays(X)
     ns, sp, sc, sr, sco = _space_arrays(Y)
     Z = np.zeros((nt, ns), dtype=np.complex128, order="F")
     i32 = lambda t: t.ctypes.data_as(_I32)
-    rc = lib().or_ncross(_d(V), i32(Cc), m.numcells, nt, i32(tp), i32(tc), i32(tr), _d(tco),
-                         ns, i32(sp), i32(sc), i32(sr), _d(sco), int(degree), _d(_c2(scale)), Z.ctypes.data_as(_D))
+    rc = lib().or_localop(int(kind), _d(V), i32(Cc), m.numcells, nt, i32(tp), i32(tc), i32(tr), _d(tco),
+                          ns, i32(sp), i32(sc), i32(sr), _d(sco), int(degree), _d(_c2(scale)), Z.ctypes.data_as(_D))
     if rc != 0:
         raise ValueError("bad rule degree")
     return Z
 
 
-def planewave(X, direction, polarization, gamma, amplitude=1.0, degree=8):
+def planewave(X, direction, polarization, gamma, amplitude=1.0, degree=8, trace="tangential"):
+    """Plane wave tested with X: trace "tangential" ((n x E) x n) or "cross" (n x E)."""
     m = X.geo
     V, Cc = np.ascontiguousarray(m.vertices, float), np.ascontiguousarray(m.cells, np.int32)
     n, p, c, r, co = _space_arrays(X)
     pol = np.asarray(polarization, dtype=np.complex128)
     b = np.zeros(2 * n)
     i32 = lambda t: t.ctypes.data_as(_I32)
-    rc = lib().or_planewave(_d(V), i32(Cc), m.numcells, n, i32(p), i32(c), i32(r), _d(co),
+    rc = lib().or_planewave({"tangential": 0, "cross": 1}[trace], _d(V), i32(Cc), m.numcells, n, i32(p), i32(c), i32(r), _d(co),
                             _d(np.ascontiguousarray(direction, float)), _d(np.ascontiguousarray(pol.real)),
                             _d(np.ascontiguousarray(pol.imag)), _d(_c2(gamma)), _d(_c2(amplitude)), int(degree), _d(b))
     if rc != 0:

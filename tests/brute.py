@@ -163,3 +163,36 @@ def mie_pec_sphere(ka, thetas):
         S1.append(np.sum(c * (a * pi[1:] + b * tau)))
         S2.append(np.sum(c * (a * tau + b * pi[1:])))
     return np.array(S1), np.array(S2)
+
+
+def dq_double_layer(trgauss, gamma, tv, sv, deg_o, deg_i, rotate_test=False):
+    """3x3 block z[i, j] = sum_p sum_q w_p w_q t_i(x_p) . (grad G(x_p - y_q) x g_j(y_q)) of the
+    double layer by plain double quadrature (numpy arrangement), grad G = -(gamma + 1/R) G r / R,
+    G = exp(-gamma R) / (4 pi R), with RT shapes f = (x - v) / (2A) as test functions t_i = f_i,
+    or their rotations t_i = n x f_i (n the test cell's normal) when rotate_test is set."""
+    tv, sv = np.asarray(tv, float), np.asarray(sv, float)
+    At, As = area(tv), area(sv)
+    nrm = np.cross(tv[0] - tv[2], tv[1] - tv[2])
+    nrm /= np.linalg.norm(nrm)
+
+    def pts(v, deg):
+        uv, w = trgauss(deg)
+        P = uv[0][:, None] * v[0] + uv[1][:, None] * v[1] + (1 - uv[0] - uv[1])[:, None] * v[2]
+        return P, w * 2 * area(v)
+
+    X, wx = pts(tv, deg_o)
+    Y, wy = pts(sv, deg_i)
+    F = (X[None, :, :] - tv[:, None, :]) / (2 * At)            # [i, p, 3]
+    if rotate_test:
+        F = np.cross(nrm[None, None, :], F)
+    Gv = (Y[None, :, :] - sv[:, None, :]) / (2 * As)           # [j, q, 3]
+    r = X[:, None, :] - Y[None, :, :]                          # [p, q, 3]
+    R = np.linalg.norm(r, axis=2)
+    grad = (-(gamma + 1 / R) * np.exp(-gamma * R) / (4 * np.pi * R) / R)[:, :, None] * r
+    W = wx[:, None] * wy[None, :]
+    z = np.zeros((3, 3), complex)
+    for j in range(3):
+        cr = np.cross(grad, Gv[j][None, :, :])                 # [p, q, 3]
+        for i in range(3):
+            z[i, j] = np.sum(W * np.einsum("pk,pqk->pq", F[i], cr))
+    return z

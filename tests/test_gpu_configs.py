@@ -29,8 +29,13 @@ def _serve(bj):
 
 
 def _oracle_Z(orc, bj, op, X, Y):
-    kind = 0 if isinstance(op, bj.MWSingleLayer3D) else 1
-    return orc.assemble(kind, op.gamma, op.alpha, op.beta, 0, QS, X, Y)
+    """The oracle's matrix of an operator or a linear combination (default quadstrats)."""
+    if isinstance(op, bj.LinearCombinationOfOperators):
+        return sum(c * _oracle_Z(orc, bj, t, X, Y) for c, t in op.terms)
+    if isinstance(op, bj.LocalOperator):
+        return orc.localop({2: 0, 4: 1}[op.kind], X, Y, bj.defaultquadstrat(op, X, Y).quad_rule)
+    qs = bj.defaultquadstrat(op, X, Y)
+    return orc.assemble(op.kind, op.gamma, op.alpha, getattr(op, "beta", 0.0), qs.kind, qs.params, X, Y)
 
 
 @pytest.fixture(scope="module")
@@ -90,8 +95,10 @@ def test_c3_mixed_vs_fp64_at_k4(bj):
 
 
 def _oracle_rhs(orc, X, kind, k, d, p):
-    pol = p if kind == "efie" else np.cross(d, p)
-    return orc.planewave(X, d, pol, 1j * k)
+    """EFIE: (n x E) x n; MFIE: n x H, H = -1/(i k) curl E = (d x p) exp(-i k d.x)."""
+    if kind == "efie":
+        return orc.planewave(X, d, p, 1j * k)
+    return orc.planewave(X, d, np.cross(d, p), 1j * k, trace="cross")
 
 
 def test_c4_batch_against_oracle(bj, orc):
