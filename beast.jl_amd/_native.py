@@ -82,6 +82,7 @@ _SIGS = [
     ("bh_space_e2d", C.c_int, [_P, _I32, _I32, _D, C.c_int64]),
     ("bh_assemble", C.c_int, [_P, C.POINTER(bh_operator), C.POINTER(bh_quadstrat), C.c_int, _P, _P, _D,
                               C.c_int, _P, C.c_int64, C.c_int64, C.c_int64]),
+    ("bh_check_strategy", C.c_int, [C.POINTER(bh_operator), C.POINTER(bh_quadstrat)]),
     ("bh_classify", C.c_int, [_P, C.POINTER(bh_operator), C.POINTER(bh_quadstrat), _P, _P,
                               C.POINTER(C.c_int8)]),
     ("bh_local_blocks", C.c_int, [_P, C.POINTER(bh_operator), C.POINTER(bh_quadstrat), _P, _P, C.c_int64,

@@ -1152,29 +1152,45 @@ struct Strategy {
     int ss_face, ss_edge, ss_vert;
 };
 
+// Triangle-rule degrees and Gauss-Legendre orders: below 1 is an invalid argument; above
+// the library's tables a valid strategy the GPU path does not cover (the caller falls back).
+void check_degree(int d, const char *what)
+{
+    if (d < 1) BH_FAIL(BH_ERR_INVALID, "%s: triangle rule degree %d must be at least 1", what, d);
+    if (d > kMaxTriDegree)
+        BH_FAIL(BH_ERR_UNSUPPORTED, "%s: no triangle rule of degree %d on the GPU path (1..%d)", what, d,
+                kMaxTriDegree);
+}
+
+void check_order(int n, const char *what)
+{
+    if (n < 1) BH_FAIL(BH_ERR_INVALID, "%s: Sauter-Schwab order %d must be at least 1", what, n);
+    if (n > kMaxGL)
+        BH_FAIL(BH_ERR_UNSUPPORTED, "%s: Sauter-Schwab order %d above the GPU path's %d", what, n, kMaxGL);
+}
+
 Strategy parse_qs(const bh_quadstrat *qs)
 {
     BH_REQUIRE(qs != nullptr, "quadstrat is NULL");
     Strategy s{};
     const int *p = qs->params;
-    auto tri_ok = [](int d) { return d >= 1 && d <= 8; };
-    auto gl_ok = [](int n) { return n >= 1 && n <= kMaxGL; };
     switch (qs->kind) {
     case BH_QS_WILTON_SAUTER:
         s = {kModeWiltonSauter, p[0], p[1], p[2], p[3], p[5], p[6], p[7]};
-        BH_REQUIRE(tri_ok(p[0]) && tri_ok(p[1]) && tri_ok(p[2]) && tri_ok(p[3]),
-                   "DoubleNumWiltonSauterQStrat: triangle rules must be degrees 1..8");
-        BH_REQUIRE(gl_ok(p[5]) && gl_ok(p[6]) && gl_ok(p[7]), "Sauter-Schwab orders must be 1..%d", kMaxGL);
+        for (int k = 0; k < 4; ++k) check_degree(p[k], "DoubleNumWiltonSauterQStrat");
+        for (int k = 5; k < 8; ++k) check_order(p[k], "DoubleNumWiltonSauterQStrat");
         break;
     case BH_QS_SAUTER:
         s = {kModeSauter, p[0], p[1], 0, 0, p[3], p[4], p[5]};
-        BH_REQUIRE(tri_ok(p[0]) && tri_ok(p[1]), "DoubleNumSauterQstrat: triangle rules must be degrees 1..8");
-        BH_REQUIRE(gl_ok(p[3]) && gl_ok(p[4]) && gl_ok(p[5]), "Sauter-Schwab orders must be 1..%d", kMaxGL);
+        for (int k = 0; k < 2; ++k) check_degree(p[k], "DoubleNumSauterQstrat");
+        for (int k = 3; k < 6; ++k) check_order(p[k], "DoubleNumSauterQstrat");
         break;
     case BH_QS_DOUBLENUM:
         s = {kModeDoubleNum, p[0], p[1], 0, 0, 0, 0, 0};
-        BH_REQUIRE(tri_ok(p[0]) && tri_ok(p[1]), "DoubleNumQStrat: triangle rules must be degrees 1..8");
+        for (int k = 0; k < 2; ++k) check_degree(p[k], "DoubleNumQStrat");
         break;
+    case BH_QS_SINGLENUM:
+        BH_FAIL(BH_ERR_UNSUPPORTED, "SingleNumQStrat applies to local operators only");
     default:
         BH_FAIL(BH_ERR_UNSUPPORTED, "quadrature strategy kind %d is not supported on the GPU path", qs->kind);
     }
@@ -1201,6 +1217,8 @@ OpParams parse_op(const bh_operator *op)
 const void *quad_points(Mesh &m, int degree, int *Q, bool fp32, hipStream_t s)
 {
     double u[kMaxTriPts], v[kMaxTriPts], w[kMaxTriPts];
+    if (degree > kMaxTriDegree)
+        BH_FAIL(BH_ERR_UNSUPPORTED, "no triangle rule of degree %d on the GPU path (1..%d)", degree, kMaxTriDegree);
     const int nq = triangle_rule(degree, u, v, w);
     BH_REQUIRE(nq > 0, "no triangle rule of degree %d", degree);
     *Q = nq;
@@ -1604,7 +1622,7 @@ void assemble_local(Ctx &ctx, int kind, const bh_quadstrat *qs, bool fp32, Space
         BH_FAIL(BH_ERR_UNSUPPORTED, "%s needs SingleNumQStrat (quadstrat kind %d given)", name, qs->kind);
     if (X.mesh != Y.mesh) BH_FAIL(BH_ERR_UNSUPPORTED, "%s on different test and trial meshes has no GPU path", name);
     const int degree = qs->params[0];
-    BH_REQUIRE(degree >= 1 && degree <= 8, "SingleNumQStrat: triangle rule must be degree 1..8");
+    check_degree(degree, "SingleNumQStrat");
     const int64_t M = X.ndofs, N = Y.ndofs;
     const size_t es = fp32 ? 8 : 16;
     if (!accumulate) {
@@ -1736,5 +1754,22 @@ extern "C" int bh_local_blocks(bh_ctx *ctx, const bh_operator *op, const bh_quad
     check_launch("k_far_pairs");
     BH_CHECK_HIP(hipMemcpyAsync(blocks, out.p, sizeof(double) * 18 * npairs, hipMemcpyDeviceToHost, ctx->stream));
     BH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    BH_API_END
+}
+
+extern "C" int bh_check_strategy(const bh_operator *op, const bh_quadstrat *qs)
+{
+    BH_API_BEGIN
+    const OpParams p = parse_op(op);
+    BH_REQUIRE(qs != nullptr, "quadstrat is NULL");
+    if (p.kind == BH_OP_NCROSS || p.kind == BH_OP_IDENTITY) {
+        if (qs->kind != BH_QS_SINGLENUM)
+            BH_FAIL(BH_ERR_UNSUPPORTED, "local operators need SingleNumQStrat (quadstrat kind %d given)", qs->kind);
+        check_degree(qs->params[0], "SingleNumQStrat");
+    } else {
+        const Strategy st = parse_qs(qs);
+        if (p.kind == BH_OP_MW_DOUBLELAYER_ROTATED && st.mode == kModeWiltonSauter)
+            BH_FAIL(BH_ERR_UNSUPPORTED, "DoubleLayerRotatedMW3D has no singularity extraction");
+    }
     BH_API_END
 }

@@ -110,6 +110,7 @@ struct DevBuf {
 int triangle_rule(int degree, double *u, double *v, double *w);  // returns npts or -1
 void legendre01(int n, double *x, double *w);                   // nodes ascending on [0,1]
 constexpr int kMaxTriPts = 32;
+constexpr int kMaxTriDegree = 10;   // Dunavant degrees 1..10 (25 points)
 constexpr int kMaxGL = 16;
 
 // ---------------------------------------------------------------- objects

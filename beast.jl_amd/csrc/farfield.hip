@@ -273,7 +273,10 @@ extern "C" int bh_farfield(bh_ctx *ctx, int kind, const double *gamma, const dou
     BH_REQUIRE(kind == BH_FF_SINGLELAYER || kind == BH_FF_DOUBLELAYER || kind == BH_FF_DOUBLELAYER_ROTATED ||
                    kind == BH_FF_DROPCONSTANT,
                "unknown far-field kind %d", kind);
-    BH_REQUIRE(rule >= 1 && rule <= 10, "SingleNumQStrat: triangle rule must be degree 1..10");
+    BH_REQUIRE(rule >= 1, "SingleNumQStrat: triangle rule degree %d must be at least 1", rule);
+    if (rule > kMaxTriDegree)
+        BH_FAIL(BH_ERR_UNSUPPORTED, "SingleNumQStrat: no triangle rule of degree %d on the GPU path (1..%d)", rule,
+                kMaxTriDegree);
     BH_REQUIRE(npoints >= 0, "npoints < 0");
     BH_REQUIRE(std::isfinite(gamma[0]) && std::isfinite(gamma[1]), "gamma is not finite");
     if (npoints == 0) return BH_OK;

@@ -7,9 +7,11 @@ for `Maxwell3D.singlelayer` / `Maxwell3D.doublelayer` on `raviartthomas` spaces
 resident-matrix solve (FP64 LU, or FP32 LU + FP64 GMRES-IR).
 
 Selection mirrors the reference's threading traits (operator.jl:106-108):
-`assemble(op, X, Y; threading=:gpu)` takes the GPU path; any input the library
-reports as unsupported (other quadstrats, non-RT spaces, dofs with more than two
-shapes) falls back to BEAST's own `:dofsplitting` assembly.
+`assemble(op, X, Y; threading=:gpu)` or `threading=BEAST.Threading{:gpu}` takes the
+GPU path, also for the terms of linear combinations, transposes and direct-product
+spaces (the `assemble!` methods below); any input the library does not cover (other
+quadstrats or rule degrees, non-RT spaces, Float32 or refined meshes, dofs with more
+than two shapes) falls back to BEAST's own multithreaded CPU assembly.
 
 Not exercised in this repository (no Julia toolchain here); the C ABI it binds
 is tested from Python with the same argument conventions.
@@ -80,6 +82,8 @@ _gamma(op) = BEAST.isstatic(op.gamma) ? 0.0 : op.gamma
 bhop(op::BEAST.MWSingleLayer3D) = BHOperator(0, c2(_gamma(op)), c2(op.α), c2(op.β))
 bhop(op::BEAST.MWDoubleLayer3D) = BHOperator(1, c2(_gamma(op)), c2(op.alpha), (0.0, 0.0))
 bhop(op::BEAST.NCross) = BHOperator(2, (0.0, 0.0), (0.0, 0.0), (0.0, 0.0))
+bhop(op::BEAST.DoubleLayerRotatedMW3D) = BHOperator(3, c2(_gamma(op)), c2(op.alpha), (0.0, 0.0))
+bhop(op::BEAST.Identity) = BHOperator(4, (0.0, 0.0), (0.0, 0.0), (0.0, 0.0))
 
 pad8(t) = ntuple(i -> i <= length(t) ? Int32(t[i]) : Int32(0), 8)
 bhqs(q::BEAST.DoubleNumWiltonSauterQStrat) = BHQuadStrat(0, pad8((q.outer_rule_far, q.inner_rule_far,
@@ -125,7 +129,31 @@ function spacehandle(X::BEAST.RTBasis, ctx)
     end
 end
 
-const GPUOp = Union{BEAST.MWSingleLayer3D,BEAST.MWDoubleLayer3D,BEAST.NCross}
+# Operators with a GPU path on RT x RT: the integral ones (pair kernels) and the local ones.
+const GPUIntegralOp = Union{BEAST.MWSingleLayer3D,BEAST.MWDoubleLayer3D,BEAST.DoubleLayerRotatedMW3D}
+const GPULocalOp = Union{BEAST.NCross,BEAST.Identity}
+const GPUOp = Union{GPUIntegralOp,GPULocalOp}
+
+"""
+    gpu_path(op, X, Y, qs) -> Bool
+
+Whether the library covers `op` on `X x Y` with strategy `qs`; everything else takes
+BEAST's CPU path: Float32 meshes (`test/test_assemblerow.jl:8-27`; the library
+classifies and integrates in FP64), meshes refining one another (the reference's
+`TestRefinesTrialQStrat`, `integralop.jl:93-99`), strategy types or rule degrees the
+library lacks and operator/strategy pairs the reference does not define on this path
+(`bh_check_strategy`: BH_ERR_UNSUPPORTED).
+"""
+function gpu_path(op::GPUOp, X::BEAST.RTBasis, Y::BEAST.RTBasis, qs)
+    (X isa BEAST.RTBasis{Float64} && Y isa BEAST.RTBasis{Float64}) || return false
+    tg, bg = BEAST.geometry(X), BEAST.geometry(Y)
+    (CompScienceMeshes.refines(tg, bg) || CompScienceMeshes.refines(bg, tg)) && return false
+    q = try bhqs(qs) catch e; e isa Unsupported || rethrow(); return false end
+    rc = ccall((:bh_check_strategy, libbeast_hip), Cint, (Ref{BHOperator}, Ref{BHQuadStrat}), Ref(bhop(op)), Ref(q))
+    rc == BH_ERR_UNSUPPORTED && return false
+    check(rc)
+    return true
+end
 
 """
     gpu_assemble!(Z, op, X, Y, qs; scale=1, row_offset=0, col_offset=0)
@@ -135,6 +163,10 @@ operator.jl:168-184, 319-356) computed on the GPU.
 """
 function gpu_assemble!(Z::Matrix{ComplexF64}, op::GPUOp, X::BEAST.RTBasis, Y::BEAST.RTBasis, qs;
         scale=1.0, row_offset=0, col_offset=0, ctx=context())
+    M, N = numfunctions(X), numfunctions(Y)
+    # the block must lie inside Z, as the reference's store would check (BoundsError)
+    (row_offset >= 0 && col_offset >= 0 && row_offset + M <= size(Z, 1) && col_offset + N <= size(Z, 2)) ||
+        throw(BoundsError(Z, (row_offset + M, col_offset + N)))
     o = Ref(bhop(op)); q = Ref(bhqs(qs)); s = Float64[real(scale), imag(scale)]
     GC.@preserve Z begin
         check(ccall((:bh_assemble, libbeast_hip), Cint,
@@ -145,18 +177,71 @@ function gpu_assemble!(Z::Matrix{ComplexF64}, op::GPUOp, X::BEAST.RTBasis, Y::BE
     return Z
 end
 
+# ---------------------------------------------------------------------------------------
+# The seam: a threading trait, as :dofsplitting and :cellcoloring are mapped to
+# Threading{:multi} and Threading{:cellcoloring} (operator.jl:106-108).  The reference's
+# assemble passes any other threading value on unchanged, and its compositions pass it to
+# their terms: LinearCombinationOfOperators (operator.jl:294-303), DirectProductSpace
+# factors (:319-356), TransposedOperator (:284-291, handled below).  So
+#     assemble(op, X, Y; threading=BEAST.Threading{:gpu})   (or threading=:gpu)
+# reaches the methods here for every term; terms without a GPU path take Threading{:multi}.
+
+# GPU operators on RT x RT: the dense block on the device, then the caller's store
+function _gpu_store!(op::GPUOp, X::BEAST.RTBasis, Y::BEAST.RTBasis, store, quadstrat, kwargs)
+    qs = quadstrat(op, X, Y)   # a strategy object returns itself (strategies/quadstrat.jl:3-5)
+    if !gpu_path(op, X, Y, qs)
+        return BEAST.assemble!(op, X, Y, store, BEAST.Threading{:multi}; quadstrat=qs, kwargs...)
+    end
+    (numfunctions(X) == 0 || numfunctions(Y) == 0) && return   # integralop.jl:72-73
+    Z = zeros(ComplexF64, numfunctions(X), numfunctions(Y))
+    gpu_assemble!(Z, op, X, Y, qs)
+    @inbounds for n in axes(Z, 2), m in axes(Z, 1)
+        store(Z[m, n], m, n)
+    end
+    return
+end
+
+BEAST.assemble!(op::GPUIntegralOp, X::BEAST.RTBasis, Y::BEAST.RTBasis, store, ::Type{BEAST.Threading{:gpu}};
+    quadstrat=BEAST.defaultquadstrat, kwargs...) = _gpu_store!(op, X, Y, store, quadstrat, kwargs)
+
+BEAST.assemble!(op::GPULocalOp, X::BEAST.RTBasis, Y::BEAST.RTBasis, store, ::Type{BEAST.Threading{:gpu}};
+    quadstrat=BEAST.defaultquadstrat, kwargs...) = _gpu_store!(op, X, Y, store, quadstrat, kwargs)
+
+# any other operator under the trait: the reference's own multithreaded CPU assembly
+BEAST.assemble!(op::BEAST.Operator, X::BEAST.Space, Y::BEAST.Space, store, ::Type{BEAST.Threading{:gpu}};
+    quadstrat=BEAST.defaultquadstrat, kwargs...) =
+    BEAST.assemble!(op, X, Y, store, BEAST.Threading{:multi}; quadstrat, kwargs...)
+
+# transposes: indices swapped, the inner operator assembled under the trait (operator.jl:284-291)
+function BEAST.assemble!(op::BEAST.TransposedOperator, X::BEAST.Space, Y::BEAST.Space, store,
+        ::Type{BEAST.Threading{:gpu}}; quadstrat=BEAST.defaultquadstrat(op, X, Y), kwargs...)
+    store1(v, m, n) = store(v, n, m)
+    BEAST.assemble!(op.op, Y, X, store1, BEAST.Threading{:gpu}; quadstrat, kwargs...)
+end
+
+# threading=:gpu as a Symbol, as assemble passes it on: converted to the trait
+function _gpu_trait(threading::Symbol)
+    threading == :gpu || throw(ArgumentError("unknown threading $(repr(threading))"))
+    return BEAST.Threading{:gpu}
+end
+BEAST.assemble!(op::BEAST.Operator, X::BEAST.Space, Y::BEAST.Space, store, threading::Symbol; kwargs...) =
+    BEAST.assemble!(op, X, Y, store, _gpu_trait(threading); kwargs...)
+BEAST.assemble!(op::BEAST.LinearCombinationOfOperators, X::BEAST.Space, Y::BEAST.Space, store, threading::Symbol;
+    kwargs...) = BEAST.assemble!(op, X, Y, store, _gpu_trait(threading); kwargs...)
+BEAST.assemble!(op::BEAST.TransposedOperator, X::BEAST.Space, Y::BEAST.Space, store, threading::Symbol;
+    kwargs...) = BEAST.assemble!(op, X, Y, store, _gpu_trait(threading); kwargs...)
+
+# a single GPU operator on RT x RT: the dense matrix written by the library directly
 function BEAST.assemble(op::GPUOp, X::BEAST.RTBasis, Y::BEAST.RTBasis;
         threading=:dofsplitting, quadstrat=BEAST.defaultquadstrat, kwargs...)
-    if threading == :gpu
+    if threading == :gpu || threading == BEAST.Threading{:gpu}
         qs = quadstrat(op, X, Y)
-        Z = zeros(ComplexF64, numfunctions(X), numfunctions(Y))
-        (numfunctions(X) == 0 || numfunctions(Y) == 0) && return Z
-        try
+        if gpu_path(op, X, Y, qs)
+            Z = zeros(ComplexF64, numfunctions(X), numfunctions(Y))
+            (numfunctions(X) == 0 || numfunctions(Y) == 0) && return Z
             return gpu_assemble!(Z, op, X, Y, qs)
-        catch e
-            e isa Unsupported || rethrow()
-            threading = :dofsplitting      # fall back to the reference's CPU path
         end
+        threading = :dofsplitting      # the reference's CPU path
     end
     return invoke(BEAST.assemble, Tuple{BEAST.AbstractOperator,Any,Any}, op, X, Y;
         threading=threading, quadstrat=quadstrat, kwargs...)
@@ -179,6 +264,22 @@ function DeviceMatrix(op::GPUOp, X::BEAST.RTBasis; quadstrat=BEAST.defaultquadst
         (Ptr{Cvoid}, Ref{BHOperator}, Ref{BHQuadStrat}, Ptr{Cvoid}, Ptr{Ptr{Cvoid}}),
         ctx.ptr, Ref(bhop(op)), Ref(bhqs(quadstrat(op, X, X))), spacehandle(X, ctx), r))
     return DeviceMatrix(r[], numfunctions(X))
+end
+
+# a linear combination of GPU operators (operator.jl:294-303), e.g. the MFIE n x K - Id/2 of
+# examples/nxmfie.jl: the first term assembled in place, then the others accumulated, each
+# with its own default strategy; a first coefficient other than 1 is applied by accumulating
+# (c - 1) times the first term
+function DeviceMatrix(op::BEAST.LinearCombinationOfOperators, X::BEAST.RTBasis; quadstrat=BEAST.defaultquadstrat,
+        ctx=context())
+    all(A -> A isa GPUOp, op.ops) || throw(Unsupported("linear combination with a term that has no GPU path"))
+    M = DeviceMatrix(op.ops[1], X; quadstrat, ctx)
+    c1 = op.coeffs[1]
+    c1 == 1 || accumulate!(M, op.ops[1], X; scale=c1 - 1, quadstrat, ctx)
+    for (a, A) in zip(op.coeffs[2:end], op.ops[2:end])
+        accumulate!(M, A, X; scale=a, quadstrat, ctx)
+    end
+    return M
 end
 
 function DeviceMatrix(Z::Matrix{ComplexF64}; ctx=context())
@@ -222,26 +323,37 @@ end
 
 Base.:\(M::DeviceMatrix, b::AbstractVecOrMat) = ldiv!(similar(b, ComplexF64), M, ComplexF64.(b))
 
-# Plane-wave right-hand side: assemble((n x E) x n, X) (excitation.jl:17-115,149-198,
-# mwexc.jl:1-52) for E a PlaneWaveMW, on the device; rule = the SingleNumQStrat degree.
-function gpu_assemble_rhs(e::BEAST.TangTraceMW, X::BEAST.RTBasis; quadstrat=BEAST.defaultquadstrat(e, X),
-        ctx=context())
+# Plane-wave right-hand sides: assemble((n x E) x n, X) and assemble(n x E, X)
+# (excitation.jl:17-115,145-180, mwexc.jl:1-54) for E a PlaneWaveMW (also curl E, a plane
+# wave itself), on the device; rule = the SingleNumQStrat degree.
+const GPUTrace = Union{BEAST.TangTraceMW,BEAST.CrossTraceMW}
+_trace(::BEAST.TangTraceMW) = Cint(0)    # BH_TRACE_TANGENTIAL
+_trace(::BEAST.CrossTraceMW) = Cint(1)   # BH_TRACE_CROSS
+
+function gpu_assemble_rhs(e::GPUTrace, X::BEAST.RTBasis; quadstrat=BEAST.defaultquadstrat(e, X), ctx=context())
     E = e.field
     E isa BEAST.PlaneWaveMW || throw(Unsupported("right-hand side of $(typeof(E))"))
+    X isa BEAST.RTBasis{Float64} || throw(Unsupported("Float32 meshes take the CPU path"))
     d = Float64[E.direction[k] for k in 1:3]
     pr = Float64[real(E.polarisation[k]) for k in 1:3]
     pi_ = Float64[imag(E.polarisation[k]) for k in 1:3]
     g = ComplexF64(E.gamma); a = ComplexF64(E.amplitude)
     b = zeros(ComplexF64, numfunctions(X))
-    check(ccall((:bh_assemble_planewave, libbeast_hip), Cint,
-        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{ComplexF64}, Ref{ComplexF64}, Cint,
-         Cint, Ptr{ComplexF64}),
-        ctx.ptr, spacehandle(X, ctx), d, pr, pi_, g, a, quadstrat.quad_rule, 1, b))
+    check(ccall((:bh_assemble_excitation, libbeast_hip), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{ComplexF64}, Ref{ComplexF64},
+         Cint, Cint, Ptr{ComplexF64}),
+        ctx.ptr, spacehandle(X, ctx), _trace(e), d, pr, pi_, g, a, quadstrat.quad_rule, 1, b))
     return b
 end
 
-function BEAST.assemble(e::BEAST.TangTraceMW, X::BEAST.RTBasis; threading=nothing, kwargs...)
-    threading == :gpu && return gpu_assemble_rhs(e, X; kwargs...)
+function BEAST.assemble(e::GPUTrace, X::BEAST.RTBasis; threading=nothing, kwargs...)
+    if threading == :gpu
+        try
+            return gpu_assemble_rhs(e, X; kwargs...)
+        catch err
+            err isa Unsupported || rethrow()
+        end
+    end
     return invoke(BEAST.assemble, Tuple{BEAST.Functional,Any}, e, X; kwargs...)
 end
 

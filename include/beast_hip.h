@@ -69,8 +69,11 @@ enum bh_op_kind {
  *   BH_QS_DOUBLENUM      DoubleNumQStrat(outer, inner)   src/quadrature/doublenumqstrat.jl:1-4
  *   BH_QS_SINGLENUM      SingleNumQStrat(quad_rule)   src/quadrature/quadstrats.jl:48-50
  *                        (local operators; RT x RT default 6, src/identityop.jl:66-67)
- * Triangle rule parameters are Dunavant polynomial degrees 1..8; Sauter-Schwab
- * parameters are Gauss-Legendre orders 1..16. */
+ * Triangle rule parameters are Dunavant polynomial degrees 1..10; Sauter-Schwab
+ * parameters are Gauss-Legendre orders 1..16.  A degree or order beyond those (valid for
+ * the reference, e.g. the rules 12 and 13 of test/test_sauterschwabints.jl:29-30) is
+ * BH_ERR_UNSUPPORTED, so the caller falls back to its CPU path; one below 1 is
+ * BH_ERR_INVALID. */
 enum bh_qs_kind { BH_QS_WILTON_SAUTER = 0, BH_QS_SAUTER = 1, BH_QS_DOUBLENUM = 2, BH_QS_SINGLENUM = 3 };
 
 /* Arithmetic.  Assembly: BH_FP64 everywhere, or BH_FP32 for the far-field
@@ -154,6 +157,11 @@ int bh_space_e2d(const bh_space *space, int32_t *kmax, int32_t *dof, double *coe
 int bh_assemble(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, int precision,
                 const bh_space *X, const bh_space *Y, const double *scale, int accumulate, void *Z,
                 int64_t ld, int64_t row_off, int64_t col_off);
+
+/* Whether (op, qs) has a GPU path, without a context or a device: BH_OK, BH_ERR_UNSUPPORTED
+ * (a valid combination the caller's CPU path must take: a rule degree beyond the tables, a
+ * strategy the operator's reference kernel does not define on the GPU) or BH_ERR_INVALID. */
+int bh_check_strategy(const bh_operator *op, const bh_quadstrat *qs);
 
 /* Pair classes (0 far, 1 Wilton near, 2 common vertex, 3 common edge, 4 common
  * face) for all test x trial cell pairs, row-major [nf_test][nf_trial];
