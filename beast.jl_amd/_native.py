@@ -91,6 +91,8 @@ _SIGS = [
     ("bh_assemble_excitation", C.c_int, [_P, _P, C.c_int, _D, _D, _D, _D, _D, C.c_int, C.c_int, _D]),
     ("bh_farfield", C.c_int, [_P, C.c_int, _D, _D, _P, _D, C.c_int64, _D, C.c_int, _D]),
     ("bh_matrix_assemble", C.c_int, [_P, C.POINTER(bh_operator), C.POINTER(bh_quadstrat), _P, C.POINTER(_P)]),
+    ("bh_matrix_assemble_batch", C.c_int, [_P, C.c_int64, C.POINTER(bh_operator), C.POINTER(bh_quadstrat),
+                                           C.POINTER(_P), C.POINTER(_P)]),
     ("bh_matrix_accumulate", C.c_int, [_P, C.POINTER(bh_operator), C.POINTER(bh_quadstrat), _P, _D]),
     ("bh_matrix_from_host", C.c_int, [_P, C.c_int64, _D, C.c_int64, C.POINTER(_P)]),
     ("bh_matrix_destroy", C.c_int, [_P]),

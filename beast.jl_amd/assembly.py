@@ -5,7 +5,10 @@ src/excitation.jl:17-30 (``assemble(field, tfs)``): operators, spaces and
 quadrature strategies are resolved here, and the native library does all the
 arithmetic.  There is no CPU path: unsupported inputs raise.
 """
+import atexit
 import ctypes as C
+import threading
+import weakref
 
 import numpy as np
 
@@ -18,7 +21,10 @@ from .rt import RTBasis
 
 
 class Context:
-    """One GPU and one stream (replaces the reference's Threading{:multi} driver)."""
+    """One GPU and one stream (replaces the reference's Threading{:multi} driver).
+
+    A context is not re-entrant: ``lock`` serialises the calls of several threads on it
+    (the calls of this module take it), e.g. a solve on a matrix another thread factors."""
 
     def __init__(self, device=0):
         lib = nat.load()
@@ -27,9 +33,12 @@ class Context:
         self.handle = h
         self.device = device
         self._lib = lib
+        self.lock = threading.RLock()
+        _live["contexts"].add(self)
 
     def synchronize(self):
-        nat.check(self._lib.bh_ctx_synchronize(self.handle))
+        with self.lock:
+            nat.check(self._lib.bh_ctx_synchronize(self.handle))
 
     def profile(self, enable=True, only=None):
         """Bracket library kernel launches with HIP events (all of them, or only kernel ``only``)."""
@@ -65,6 +74,21 @@ class Context:
 
 _default_ctx = None
 
+# Live native objects, released in dependency order (matrices, spaces, meshes, then
+# contexts) by an exit handler that runs before the interpreter's teardown, so that no
+# finaliser calls into the HIP runtime while or after the runtime shuts down.
+_live = {"matrices": weakref.WeakSet(), "handles": weakref.WeakSet(), "contexts": weakref.WeakSet()}
+
+
+@atexit.register
+def _release_native_objects():
+    for M in list(_live["matrices"]):
+        M.__del__()
+    for h in sorted(list(_live["handles"]), key=lambda h: h.order):
+        h.__del__()
+    for c in list(_live["contexts"]):
+        c.close()
+
 
 def default_context():
     global _default_ctx
@@ -74,15 +98,20 @@ def default_context():
 
 
 class _Handle:
-    def __init__(self, h, destroy, keep):
+    """A mesh (order 1) or space (order 0) handle; spaces are released before their meshes."""
+
+    def __init__(self, h, destroy, keep, order):
         self.h = h
         self._destroy = destroy
         self._keep = keep
+        self.order = order
+        _live["handles"].add(self)
 
     def __del__(self):
         try:
             if self.h:
                 self._destroy(self.h)
+                self.h = None
         except Exception:
             pass
 
@@ -95,8 +124,10 @@ def mesh_handle(mesh, ctx):
         V = nat.as_f64(mesh.vertices)
         Cc = nat.as_i32(mesh.cells)
         h = C.c_void_p()
-        nat.check(lib.bh_mesh_create(ctx.handle, nat.dptr(V), V.shape[0], nat.i32ptr(Cc), Cc.shape[0], C.byref(h)))
-        cache[key] = _Handle(h, lib.bh_mesh_destroy, ctx)
+        with ctx.lock:
+            nat.check(lib.bh_mesh_create(ctx.handle, nat.dptr(V), V.shape[0], nat.i32ptr(Cc), Cc.shape[0],
+                                         C.byref(h)))
+        cache[key] = _Handle(h, lib.bh_mesh_destroy, ctx, 1)
     return cache[key].h
 
 
@@ -107,10 +138,11 @@ def space_handle(space, ctx):
         lib = nat.load()
         mh = mesh_handle(space.geo, ctx)
         h = C.c_void_p()
-        nat.check(lib.bh_space_create(mh, space.numfunctions(), nat.i32ptr(space.shape_ptr),
-                                      nat.i32ptr(space.shape_cell), nat.i32ptr(space.shape_ref),
-                                      nat.dptr(space.shape_coeff), C.byref(h)))
-        cache[key] = _Handle(h, lib.bh_space_destroy, (ctx, mh))
+        with ctx.lock:
+            nat.check(lib.bh_space_create(mh, space.numfunctions(), nat.i32ptr(space.shape_ptr),
+                                          nat.i32ptr(space.shape_cell), nat.i32ptr(space.shape_ref),
+                                          nat.dptr(space.shape_coeff), C.byref(h)))
+        cache[key] = _Handle(h, lib.bh_space_destroy, (ctx, mh), 0)
     return cache[key].h
 
 
@@ -187,12 +219,13 @@ def assemble(op, X, Y=None, quadstrat=defaultquadstrat, precision="fp64", ctx=No
         raise IndexError(f"a {M} x {N} block at offsets ({row_offset}, {col_offset}) does not fit out of shape "
                          f"{out.shape}")
     lib = nat.load()
-    xh, yh = space_handle(X, ctx), space_handle(Y, ctx)
-    for coef, term in _terms(op):
-        qs = resolve_quadstrat(quadstrat, term, X, Y)
-        sc = (C.c_double * 2)(complex(coef).real, complex(coef).imag)
-        nat.check(lib.bh_assemble(ctx.handle, C.byref(op_struct(term)), C.byref(qs_struct(qs)), prec, xh, yh, sc, 1,
-                                  out.ctypes.data_as(C.c_void_p), out.shape[0], row_offset, col_offset))
+    with ctx.lock:
+        xh, yh = space_handle(X, ctx), space_handle(Y, ctx)
+        for coef, term in _terms(op):
+            qs = resolve_quadstrat(quadstrat, term, X, Y)
+            sc = (C.c_double * 2)(complex(coef).real, complex(coef).imag)
+            nat.check(lib.bh_assemble(ctx.handle, C.byref(op_struct(term)), C.byref(qs_struct(qs)), prec, xh, yh, sc,
+                                      1, out.ctypes.data_as(C.c_void_p), out.shape[0], row_offset, col_offset))
     return out
 
 
@@ -215,9 +248,10 @@ def assemble_rhs(field, X, quadstrat=defaultquadstrat, ctx=None):
     pi = nat.as_f64(E.polarisation.imag)
     g = np.array([E.gamma.real, E.gamma.imag])
     a = np.array([E.amplitude.real, E.amplitude.imag])
-    nat.check(lib.bh_assemble_excitation(ctx.handle, space_handle(X, ctx), trace, nat.dptr(d), nat.dptr(pr),
-                                         nat.dptr(pi), nat.dptr(g), nat.dptr(a), int(qs.quad_rule), 1,
-                                         b.ctypes.data_as(nat._D)))
+    with ctx.lock:
+        nat.check(lib.bh_assemble_excitation(ctx.handle, space_handle(X, ctx), trace, nat.dptr(d), nat.dptr(pr),
+                                             nat.dptr(pi), nat.dptr(g), nat.dptr(a), int(qs.quad_rule), 1,
+                                             b.ctypes.data_as(nat._D)))
     return b
 
 
@@ -274,6 +308,7 @@ class DeviceMatrix:
         self.n = n
         self.ctx = ctx
         self.stats = None
+        _live["matrices"].add(self)
 
     @classmethod
     def assemble(cls, op, X, quadstrat=defaultquadstrat, ctx=None):
@@ -290,18 +325,59 @@ class DeviceMatrix:
             terms.insert(0, terms.pop(ones[0]))
         c0, t0 = terms[0]
         qs = resolve_quadstrat(quadstrat, t0, X, X)
-        nat.check(lib.bh_matrix_assemble(ctx.handle, C.byref(op_struct(t0)), C.byref(qs_struct(qs)), xh, C.byref(h)))
+        with ctx.lock:
+            nat.check(lib.bh_matrix_assemble(ctx.handle, C.byref(op_struct(t0)), C.byref(qs_struct(qs)), xh, C.byref(h)))
         M = cls(h, X.numfunctions(), ctx)
         if complex(c0) != 1.0:
             # the first term assembled unscaled: rescale by accumulating (c0 - 1) times it
             d = complex(c0) - 1.0
             sc = (C.c_double * 2)(d.real, d.imag)
-            nat.check(lib.bh_matrix_accumulate(M.h, C.byref(op_struct(t0)), C.byref(qs_struct(qs)), xh, sc))
+            with M.ctx.lock:
+                nat.check(lib.bh_matrix_accumulate(M.h, C.byref(op_struct(t0)), C.byref(qs_struct(qs)), xh, sc))
         for coef, term in terms[1:]:
             qs = resolve_quadstrat(quadstrat, term, X, X)
             sc = (C.c_double * 2)(complex(coef).real, complex(coef).imag)
-            nat.check(lib.bh_matrix_accumulate(M.h, C.byref(op_struct(term)), C.byref(qs_struct(qs)), xh, sc))
+            with M.ctx.lock:
+                nat.check(lib.bh_matrix_accumulate(M.h, C.byref(op_struct(term)), C.byref(qs_struct(qs)), xh, sc))
         return M
+
+    @classmethod
+    def assemble_batch(cls, requests, quadstrat=defaultquadstrat, ctx=None):
+        """Resident matrices of several (op, X) requests with shared launches
+        (bh_matrix_assemble_batch): the integral operator terms of every request are
+        assembled together, then each request's further terms are accumulated in place as
+        in ``assemble``.  Each matrix equals the one ``assemble`` gives its request."""
+        ctx = ctx or default_context()
+        lib = nat.load()
+        firsts, rests = [], []
+        for op, X in requests:
+            terms = list(_terms(op))
+            # the first integral term with coefficient 1 goes into the batch, the rest is accumulated
+            pick = next((i for i, (c, t) in enumerate(terms)
+                         if complex(c) == 1.0 and not isinstance(t, (NCross, Identity))), None)
+            if pick is None:
+                pick = next(i for i, (c, t) in enumerate(terms) if not isinstance(t, (NCross, Identity)))
+            c0, t0 = terms.pop(pick)
+            if complex(c0) != 1.0:
+                terms.insert(0, (complex(c0) - 1.0, t0))
+            firsts.append((t0, X))
+            rests.append(terms)
+        n = len(firsts)
+        ops = (nat.bh_operator * n)(*[op_struct(t) for t, _ in firsts])
+        qss = (nat.bh_quadstrat * n)(*[qs_struct(resolve_quadstrat(quadstrat, t, X, X)) for t, X in firsts])
+        sps = (C.c_void_p * n)(*[space_handle(X, ctx).value for _, X in firsts])
+        outs = (C.c_void_p * n)()
+        with ctx.lock:
+            nat.check(lib.bh_matrix_assemble_batch(ctx.handle, n, ops, qss, sps, outs))
+        mats = [cls(C.c_void_p(outs[i]), firsts[i][1].numfunctions(), ctx) for i in range(n)]
+        for M, (t0, X), terms in zip(mats, firsts, rests):
+            xh = space_handle(X, ctx)
+            for coef, term in terms:
+                qs = resolve_quadstrat(quadstrat, term, X, X)
+                sc = (C.c_double * 2)(complex(coef).real, complex(coef).imag)
+                with M.ctx.lock:
+                    nat.check(lib.bh_matrix_accumulate(M.h, C.byref(op_struct(term)), C.byref(qs_struct(qs)), xh, sc))
+        return mats
 
     @classmethod
     def from_host(cls, Z, ctx=None):
@@ -311,22 +387,26 @@ class DeviceMatrix:
         if Z.shape != (n, n):
             raise ValueError("square matrix expected")
         h = C.c_void_p()
-        nat.check(nat.load().bh_matrix_from_host(ctx.handle, n, Z.ctypes.data_as(nat._D), n, C.byref(h)))
+        with ctx.lock:
+            nat.check(nat.load().bh_matrix_from_host(ctx.handle, n, Z.ctypes.data_as(nat._D), n, C.byref(h)))
         return cls(h, n, ctx)
 
     def download(self):
         Z = np.zeros((self.n, self.n), dtype=np.complex128, order="F")
-        nat.check(nat.load().bh_matrix_download(self.h, Z.ctypes.data_as(nat._D), self.n))
+        with self.ctx.lock:
+            nat.check(nat.load().bh_matrix_download(self.h, Z.ctypes.data_as(nat._D), self.n))
         return Z
 
     def factor(self, precision="fp64"):
-        nat.check(nat.load().bh_matrix_factor(self.h, {"fp64": nat.FP64, "fp32": nat.FP32,
-                                                       "fp32split": nat.FP32_SPLIT}[precision]))
+        with self.ctx.lock:
+            nat.check(nat.load().bh_matrix_factor(self.h, {"fp64": nat.FP64, "fp32": nat.FP32,
+                                                           "fp32split": nat.FP32_SPLIT}[precision]))
         return self
 
     def pivots(self):
         ip = np.zeros(self.n, dtype=np.int64)
-        nat.check(nat.load().bh_matrix_pivots(self.h, nat.i64ptr(ip)))
+        with self.ctx.lock:
+            nat.check(nat.load().bh_matrix_pivots(self.h, nat.i64ptr(ip)))
         return ip
 
     def solve(self, B, tol=0.0, maxit=0):
@@ -336,8 +416,9 @@ class DeviceMatrix:
         Bf = np.asfortranarray(B.reshape(self.n, -1))
         Xf = np.zeros_like(Bf, order="F")
         st = nat.bh_solve_stats()
-        nat.check(nat.load().bh_matrix_solve(self.h, Bf.shape[1], Bf.ctypes.data_as(nat._D),
-                                             Xf.ctypes.data_as(nat._D), float(tol), int(maxit), C.byref(st)))
+        with self.ctx.lock:
+            nat.check(nat.load().bh_matrix_solve(self.h, Bf.shape[1], Bf.ctypes.data_as(nat._D),
+                                                 Xf.ctypes.data_as(nat._D), float(tol), int(maxit), C.byref(st)))
         self.stats = {"iterations": st.iterations, "inner": st.inner, "residual": st.residual,
                       "seconds_factor": st.seconds_factor, "seconds_solve": st.seconds_solve}
         return Xf[:, 0] if vec else Xf
@@ -345,7 +426,8 @@ class DeviceMatrix:
     def __del__(self):
         try:
             if self.h:
-                nat.load().bh_matrix_destroy(self.h)
+                with self.ctx.lock:
+                    nat.load().bh_matrix_destroy(self.h)
                 self.h = None
         except Exception:
             pass

@@ -30,6 +30,7 @@
 
 #include "bh_device.h"
 #include "bh_internal.h"
+#include "bh_matrix.h"
 
 namespace bh {
 
@@ -41,12 +42,12 @@ struct OpParams {
 };
 
 // ============================================================ K0 classification
-__global__ __launch_bounds__(256) void k_classify_count(const double *__restrict__ tgeo, int nft,
-                                                        const double *__restrict__ sgeo, int nfs, double k2,
-                                                        int mode, int *__restrict__ counts, int *__restrict__ err)
+// Bodies of the per-request kernels take the block's (or wave's) item explicitly, so that the
+// batched kernels (one launch for the items of several requests) run the same arithmetic.
+__device__ void classify_count_dev(const double *__restrict__ tgeo, const double *__restrict__ sgeo, int nfs,
+                                   double k2, int mode, int p, int *__restrict__ counts, int *__restrict__ err)
 {
     __shared__ int wsum[4];
-    const int p = blockIdx.x;
     double tv[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) tv[k] = tgeo[(size_t)p * kGeo + k];
@@ -68,12 +69,18 @@ __global__ __launch_bounds__(256) void k_classify_count(const double *__restrict
     if (threadIdx.x == 0) counts[p] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-__global__ __launch_bounds__(256) void k_classify_fill(const double *__restrict__ tgeo, int nft,
-                                                       const double *__restrict__ sgeo, int nfs, double k2, int mode,
-                                                       const int *__restrict__ offsets, int4 *__restrict__ entries)
+__global__ __launch_bounds__(256) void k_classify_count(const double *__restrict__ tgeo, int nft,
+                                                        const double *__restrict__ sgeo, int nfs, double k2,
+                                                        int mode, int *__restrict__ counts, int *__restrict__ err)
+{
+    classify_count_dev(tgeo, sgeo, nfs, k2, mode, blockIdx.x, counts, err);
+}
+
+__device__ void classify_fill_dev(const double *__restrict__ tgeo, const double *__restrict__ sgeo, int nfs,
+                                  double k2, int mode, const int *__restrict__ offsets, int4 *__restrict__ entries,
+                                  int p)
 {
     __shared__ int wtot[4];
-    const int p = blockIdx.x;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double tv[9];
 #pragma unroll
@@ -102,6 +109,13 @@ __global__ __launch_bounds__(256) void k_classify_fill(const double *__restrict_
         base += tot;
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void k_classify_fill(const double *__restrict__ tgeo, int nft,
+                                                       const double *__restrict__ sgeo, int nfs, double k2, int mode,
+                                                       const int *__restrict__ offsets, int4 *__restrict__ entries)
+{
+    classify_fill_dev(tgeo, sgeo, nfs, k2, mode, offsets, entries, blockIdx.x);
 }
 
 __global__ void k_class_lists(const int4 *__restrict__ entries, int n, int *__restrict__ counters,
@@ -165,13 +179,11 @@ __device__ __forceinline__ int grid_candidate(const PairGridView &g, const int *
     return g.cells[b0[r] + idx - pre[r]];
 }
 
-__global__ __launch_bounds__(256) void k_classify_count_grid(const double *__restrict__ tgeo, int nft,
-                                                             const double *__restrict__ sgeo, PairGridView g,
-                                                             double k2, int mode, int *__restrict__ counts,
-                                                             int *__restrict__ err)
+__device__ void classify_count_grid_dev(const double *__restrict__ tgeo, const double *__restrict__ sgeo,
+                                        const PairGridView &g, double k2, int mode, int p, int *__restrict__ counts,
+                                        int *__restrict__ err)
 {
     __shared__ int wsum[4], b0[27], pre[28];
-    const int p = blockIdx.x;
     double tv[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) tv[k] = tgeo[(size_t)p * kGeo + k];
@@ -196,14 +208,20 @@ __global__ __launch_bounds__(256) void k_classify_count_grid(const double *__res
     if (threadIdx.x == 0) counts[p] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-__global__ __launch_bounds__(256) void k_classify_fill_grid(const double *__restrict__ tgeo, int nft,
-                                                            const double *__restrict__ sgeo, PairGridView g,
-                                                            double k2, int mode, const int *__restrict__ offsets,
-                                                            int4 *__restrict__ entries)
+__global__ __launch_bounds__(256) void k_classify_count_grid(const double *__restrict__ tgeo, int nft,
+                                                             const double *__restrict__ sgeo, PairGridView g,
+                                                             double k2, int mode, int *__restrict__ counts,
+                                                             int *__restrict__ err)
+{
+    classify_count_grid_dev(tgeo, sgeo, g, k2, mode, blockIdx.x, counts, err);
+}
+
+__device__ void classify_fill_grid_dev(const double *__restrict__ tgeo, const double *__restrict__ sgeo,
+                                       const PairGridView &g, double k2, int mode, const int *__restrict__ offsets,
+                                       int4 *__restrict__ entries, int p)
 {
     __shared__ int b0[27], pre[28], s_n;
     __shared__ int s_q[kGridFillCap], s_c[kGridFillCap], s_pk[kGridFillCap];
-    const int p = blockIdx.x;
     double tv[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) tv[k] = tgeo[(size_t)p * kGeo + k];
@@ -237,6 +255,14 @@ __global__ __launch_bounds__(256) void k_classify_fill_grid(const double *__rest
         for (int f = 0; f < m; ++f) rank += s_q[f] < q;
         entries[base + rank] = make_int4(p, q, s_c[e], s_pk[e]);
     }
+}
+
+__global__ __launch_bounds__(256) void k_classify_fill_grid(const double *__restrict__ tgeo, int nft,
+                                                            const double *__restrict__ sgeo, PairGridView g,
+                                                            double k2, int mode, const int *__restrict__ offsets,
+                                                            int4 *__restrict__ entries)
+{
+    classify_fill_grid_dev(tgeo, sgeo, g, k2, mode, offsets, entries, blockIdx.x);
 }
 
 // Full class matrix (debug / bit-exact test entry).
@@ -376,16 +402,12 @@ __device__ __forceinline__ void ss_order(int cls, int pk, int *pi, int *pj)
     }
 }
 
+// one wave (lane = 0..63) computes the block of entry e
 template <int OPK>
-__global__ __launch_bounds__(256) void k_sauter_schwab(const double *__restrict__ tgeo, const double *__restrict__ sgeo,
-                                                       const int4 *__restrict__ entries, const int *__restrict__ list,
-                                                       int nlist, int cls, GLRule gl, OpParams op,
-                                                       double *__restrict__ blocks)
+__device__ void ss_dev(const double *__restrict__ tgeo, const double *__restrict__ sgeo,
+                       const int4 *__restrict__ entries, int e, int cls, const GLRule &gl, const OpParams &op,
+                       double *__restrict__ blocks, int lane)
 {
-    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wv >= nlist) return;
-    const int lane = threadIdx.x & 63;
-    const int e = list[wv];
     const int4 en = entries[e];
     const double *tg = tgeo + (size_t)en.x * kGeo, *sg = sgeo + (size_t)en.y * kGeo;
     const double c[3] = {tg[kGeoC], tg[kGeoC + 1], tg[kGeoC + 2]};
@@ -516,6 +538,17 @@ __global__ __launch_bounds__(256) void k_sauter_schwab(const double *__restrict_
     for (int k = 0; k < 9; ++k) { out[2 * k] = z[k].r; out[2 * k + 1] = z[k].i; }
 }
 
+template <int OPK>
+__global__ __launch_bounds__(256) void k_sauter_schwab(const double *__restrict__ tgeo, const double *__restrict__ sgeo,
+                                                       const int4 *__restrict__ entries, const int *__restrict__ list,
+                                                       int nlist, int cls, GLRule gl, OpParams op,
+                                                       double *__restrict__ blocks)
+{
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= nlist) return;
+    ss_dev<OPK>(tgeo, sgeo, entries, list[wv], cls, gl, op, blocks, threadIdx.x & 63);
+}
+
 // ============================================================ K3 Wilton near
 // Analytic integrals over the triangle (b0, b1, b2) seen from x (all in one
 // frame): S[-1,0,1] = int R^k, V[-1,0,1] = int (y - rho) R^k, G[-1] = int grad_y 1/R,
@@ -596,16 +629,11 @@ __device__ __forceinline__ cx<double> cexpm1(double a, double b)
 }
 
 template <int OPK>
-__global__ __launch_bounds__(256) void k_wilton(const double *__restrict__ tgeo, const double *__restrict__ sgeo,
-                                                const int4 *__restrict__ entries, const int *__restrict__ list,
-                                                int nlist, const double *__restrict__ tq, int Qo,
-                                                const double *__restrict__ sq, int Qi, OpParams op,
-                                                double *__restrict__ blocks)
+__device__ void wilton_dev(const double *__restrict__ tgeo, const double *__restrict__ sgeo,
+                           const int4 *__restrict__ entries, int e, const double *__restrict__ tq, int Qo,
+                           const double *__restrict__ sq, int Qi, const OpParams &op, double *__restrict__ blocks,
+                           int lane)
 {
-    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wv >= nlist) return;
-    const int lane = threadIdx.x & 63;
-    const int e = list[wv];
     const int4 en = entries[e];
     const double *tg = tgeo + (size_t)en.x * kGeo, *sg = sgeo + (size_t)en.y * kGeo;
     const double c[3] = {tg[kGeoC], tg[kGeoC + 1], tg[kGeoC + 2]};
@@ -727,6 +755,18 @@ __global__ __launch_bounds__(256) void k_wilton(const double *__restrict__ tgeo,
     double *out = blocks + (size_t)e * 18;
 #pragma unroll
     for (int k = 0; k < 9; ++k) { out[2 * k] = z[k].r; out[2 * k + 1] = z[k].i; }
+}
+
+template <int OPK>
+__global__ __launch_bounds__(256) void k_wilton(const double *__restrict__ tgeo, const double *__restrict__ sgeo,
+                                                const int4 *__restrict__ entries, const int *__restrict__ list,
+                                                int nlist, const double *__restrict__ tq, int Qo,
+                                                const double *__restrict__ sq, int Qi, OpParams op,
+                                                double *__restrict__ blocks)
+{
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= nlist) return;
+    wilton_dev<OPK>(tgeo, sgeo, entries, list[wv], tq, Qo, sq, Qi, op, blocks, threadIdx.x & 63);
 }
 
 // ============================================================ K1 far field
@@ -852,21 +892,35 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
         efie_block<R>(A.op, s1, a, b, T0, Tx, Ty, T1, z, stride);
     } else if constexpr (OPK == BH_OP_MW_DOUBLELAYER_ROTATED) {
+        // On a flat cell paired with itself n . r = n . g_j = 0 for every point pair, so the
+        // block is exactly zero; in FP32 the rounding of those dot products times the 1/R^2
+        // kernel would leave noise far above the FP32 far-field error, so it is written as zero
+        // (FP64 keeps the computed value, below 1e-15 of the neighbouring entries).
+        if constexpr (sizeof(R) == 4) {
+            if (p == q && A.tgeo == A.sgeo) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) z[k * stride] = {(R)0, (R)0};
+                return;
+            }
+        }
         // f_i . (n x (K x g_j)) = (f_i . K)(n . g_j) - (f_i . g_j)(n . K), K = h r, r = xd - y,
-        // n the test normal, f_i = (x - a_i)/(2 A_t), g_j = (y - b_j)/(2 A_s).  Per test point,
-        // with Q = sum w h, P = sum w h (n.y), V = sum w h y and s = n.xd, the y (n.y) terms
-        // cancel and 4 A_t A_s z_ij sums (x - a).xd (P - nb Q) + (nb - s)(x - a).V
-        // + ((x - a).b)(s Q - P), nb = n.b_j; expanded in a and b into the moments below.
+        // n the test normal, f_i = u_i / (2 A_t), g_j = (y - b_j) / (2 A_s).  With u_i = x - a_i
+        // and v_j = xd - b_j (so y - b_j = v_j - r, and the r (n . r) terms cancel):
+        // 4 A_t A_s f_i . (n x (K x g_j)) = h [(u_i . r)(n . v_j) - (u_i . v_j)(n . r)].
+        // Per test point only Rs = sum w h r and Ss = sum w h (n . r) over the trial points
+        // are needed; r itself enters, so close point pairs do not cancel.
         const R nrm[3] = {(R)tg[kGeoN], (R)tg[kGeoN + 1], (R)tg[kGeoN + 2]};
-        cx<R> Ma{0, 0}, Mb{0, 0}, Me{0, 0}, Mg{0, 0}, Mk{0, 0}, Ml{0, 0};
-        cx<R> Mc[3] = {{0, 0}, {0, 0}, {0, 0}}, Md[3] = {{0, 0}, {0, 0}, {0, 0}}, Mf[3] = {{0, 0}, {0, 0}, {0, 0}},
-              Mh[3] = {{0, 0}, {0, 0}, {0, 0}}, Mi[3] = {{0, 0}, {0, 0}, {0, 0}}, Mj[3] = {{0, 0}, {0, 0}, {0, 0}};
+        R a[9], b[9];
+        load_verts(a, b);
+        cx<R> zz[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) zz[k] = {0, 0};
 #pragma unroll 1
         for (int ip = 0; ip < Qo; ++ip) {
             const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
             const R wp = xq[4 * ip + 3];
             const R xd[3] = {x[0] + dc[0], x[1] + dc[1], x[2] + dc[2]};
-            cx<R> Qs{0, 0}, Ps{0, 0}, V[3] = {{0, 0}, {0, 0}, {0, 0}};
+            cx<R> Rs[3] = {{0, 0}, {0, 0}, {0, 0}}, Ss{0, 0};
 #pragma unroll
             for (int iq = 0; iq < Qi; ++iq) {
                 const R y[3] = {yq[4 * iq], yq[4 * iq + 1], yq[4 * iq + 2]};
@@ -880,53 +934,27 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
                 cx<R> h = cmul(cx<R>{gr + idist, gi}, G);
                 const R f = -idist * wq;
                 h.r *= f; h.i *= f;
-                const R ny = dot3(nrm, y);
-                Qs.r += h.r; Qs.i += h.i;
-                Ps.r += h.r * ny; Ps.i += h.i * ny;
+                const R nr = dot3(nrm, r);
+                Ss.r += h.r * nr; Ss.i += h.i * nr;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) { V[k].r += h.r * y[k]; V[k].i += h.i * y[k]; }
+                for (int k = 0; k < 3; ++k) { Rs[k].r += h.r * r[k]; Rs[k].i += h.i * r[k]; }
             }
-            const R s = dot3(nrm, xd), xxd = dot3(x, xd);
-            const cx<R> xV{x[0] * V[0].r + x[1] * V[1].r + x[2] * V[2].r, x[0] * V[0].i + x[1] * V[1].i + x[2] * V[2].i};
-            Ma.r += wp * xxd * Ps.r; Ma.i += wp * xxd * Ps.i;
-            Mb.r += wp * xxd * Qs.r; Mb.i += wp * xxd * Qs.i;
-            Me.r += wp * xV.r; Me.i += wp * xV.i;
-            Mg.r += wp * s * xV.r; Mg.i += wp * s * xV.i;
-            Mk.r += wp * s * Qs.r; Mk.i += wp * s * Qs.i;
-            Ml.r += wp * Ps.r; Ml.i += wp * Ps.i;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                Mc[k].r += wp * xd[k] * Ps.r; Mc[k].i += wp * xd[k] * Ps.i;
-                Md[k].r += wp * xd[k] * Qs.r; Md[k].i += wp * xd[k] * Qs.i;
-                Mf[k].r += wp * V[k].r; Mf[k].i += wp * V[k].i;
-                Mh[k].r += wp * s * V[k].r; Mh[k].i += wp * s * V[k].i;
-                Mi[k].r += wp * s * x[k] * Qs.r; Mi[k].i += wp * s * x[k] * Qs.i;
-                Mj[k].r += wp * x[k] * Ps.r; Mj[k].i += wp * x[k] * Ps.i;
+            for (int j = 0; j < 3; ++j) {
+                const R v[3] = {xd[0] - b[3 * j], xd[1] - b[3 * j + 1], xd[2] - b[3 * j + 2]};
+                const R nv = wp * dot3(nrm, v);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const R u[3] = {x[0] - a[3 * i], x[1] - a[3 * i + 1], x[2] - a[3 * i + 2]};
+                    const R uv = wp * dot3(u, v);
+                    zz[i + 3 * j].r += dot3c_r(u, Rs) * nv - uv * Ss.r;
+                    zz[i + 3 * j].i += dot3c_i(u, Rs) * nv - uv * Ss.i;
+                }
             }
         }
-        R a[9], b[9];
-        load_verts(a, b);
         const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const R *bj = b + 3 * j;
-            const R nb = dot3(nrm, bj);
-            // the j-only part: Ma - nb Mb + nb Me - Mg + b.Mi - b.Mj
-            const cx<R> tj{Ma.r - nb * Mb.r + nb * Me.r - Mg.r + dot3c_r(bj, Mi) - dot3c_r(bj, Mj),
-                           Ma.i - nb * Mb.i + nb * Me.i - Mg.i + dot3c_i(bj, Mi) - dot3c_i(bj, Mj)};
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const R *ai = a + 3 * i;
-                const R ab = dot3(ai, bj);
-                cx<R> t;
-                // - a.Mc + nb a.Md - nb a.Mf + a.Mh - (a.b) Mk + (a.b) Ml
-                t.r = tj.r - dot3c_r(ai, Mc) + nb * dot3c_r(ai, Md) - nb * dot3c_r(ai, Mf) + dot3c_r(ai, Mh) -
-                      ab * Mk.r + ab * Ml.r;
-                t.i = tj.i - dot3c_i(ai, Mc) + nb * dot3c_i(ai, Md) - nb * dot3c_i(ai, Mf) + dot3c_i(ai, Mh) -
-                      ab * Mk.i + ab * Ml.i;
-                z[(i + 3 * j) * stride] = {s1 * t.r, s1 * t.i};
-            }
-        }
+        for (int k = 0; k < 9; ++k) z[k * stride] = {s1 * zz[k].r, s1 * zz[k].i};
     } else {
         // M1 = sum w x.U, MU = sum w U, MX = sum w x x Rv, MR = sum w Rv with
         // U = xd x H1, Rv = xd H0 - H1, H0 = sum w h, H1 = sum w h y, h = -(gamma + 1/R) G / R
@@ -1020,14 +1048,13 @@ struct TileArgs {
 };
 
 template <typename R, int OPK, bool ROWMAJOR, int QO, int QI, bool SHIFT = false>
-__global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
+__device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
 {
     __shared__ int s_rc[kTileCells], s_cc[kTileCells];
     __shared__ int s_rslot[kTile][2], s_cslot[kTile][2];
     __shared__ R s_rcoef[kTile][2], s_ccoef[kTile][2];
     __shared__ cx<R> s_blk[kPairChunk * 9];
     __shared__ unsigned s_near[kTileCells * kTileCells / 32];   // bit rc * kTileCells + cc: non-far pair
-    const int tr = blockIdx.y, tc = blockIdx.x;
     const int tid = threadIdx.x;
     const int r0 = A.row.ptr[tr], nr = A.row.ptr[tr + 1] - r0;
     const int c0 = A.col.ptr[tc], nc = A.col.ptr[tc + 1] - c0;
@@ -1130,6 +1157,12 @@ __global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
             if (A.Z32) ((float2 *)A.Z32)[(size_t)m * A.ld32 + n] = float2{(float)v.r, (float)v.i};
         }
     }
+}
+
+template <typename R, int OPK, bool ROWMAJOR, int QO, int QI, bool SHIFT = false>
+__global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
+{
+    far_tiles_dev<R, OPK, ROWMAJOR, QO, QI, SHIFT>(A, blockIdx.y, blockIdx.x);
 }
 
 // Far blocks for an explicit pair list (inspection entry).
@@ -1467,6 +1500,27 @@ struct Plan {
     long long nft = 0, nfs = 0;
 };
 
+// Lossless FP64: phases relative to the centroid distance when |gi| (r_p + r_q) stays below
+// 0.369, where the degree-12/13 Taylor remainders of cos and sin are below 1e-17 (r: the
+// largest vertex-to-centroid distance of each mesh).
+bool shifted_phases(const OpParams &op, const Mesh &tm, const Mesh &sm, bool fp32)
+{
+    if (fp32 || op.gr != 0.0) return false;
+    auto radius = [](const Mesh &m) {
+        double r = 0.0;
+        for (int64_t c = 0; c < m.nf; ++c) {
+            const double *g = &m.geo[(size_t)c * kGeo];
+            for (int a = 0; a < 3; ++a) {
+                const double d0 = g[3 * a] - g[kGeoC], d1 = g[3 * a + 1] - g[kGeoC + 1], d2 = g[3 * a + 2] - g[kGeoC + 2];
+                r = std::max(r, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+            }
+        }
+        return r;
+    };
+    const double th = std::fabs(op.gi) * (radius(tm) + radius(sm)) * (1.0 + 1e-12);
+    return th <= 0.369;
+}
+
 void make_plan(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, Mesh &tm, Mesh &sm, bool fp32, Plan &P)
 {
     P.st = parse_qs(qs);
@@ -1486,25 +1540,7 @@ void make_plan(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, Mesh &tm
     F.sq = quad_points(sm, P.st.far_i, &F.Qi, fp32, ctx.stream);
     F.mode = P.st.mode;
     F.k2 = P.op.gr * P.op.gr + P.op.gi * P.op.gi;
-    // lossless FP64: phases relative to the centroid distance when |gi| (r_p + r_q) stays
-    // below 0.369, where the degree-12/13 Taylor remainders of cos and sin are below 1e-17
-    F.shifted = 0;
-    if (!fp32 && P.op.gr == 0.0) {
-        auto radius = [](const Mesh &m) {
-            double r = 0.0;
-            for (int64_t c = 0; c < m.nf; ++c) {
-                const double *g = &m.geo[(size_t)c * kGeo];
-                for (int a = 0; a < 3; ++a) {
-                    const double d0 = g[3 * a] - g[kGeoC], d1 = g[3 * a + 1] - g[kGeoC + 1],
-                                 d2 = g[3 * a + 2] - g[kGeoC + 2];
-                    r = std::max(r, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
-                }
-            }
-            return r;
-        };
-        const double th = std::fabs(P.op.gi) * (radius(tm) + radius(sm)) * (1.0 + 1e-12);
-        F.shifted = th <= 0.369 ? 1 : 0;
-    }
+    F.shifted = shifted_phases(P.op, tm, sm, fp32) ? 1 : 0;
     F.nptr = P.nd.nptr.p;
     F.nent = P.nd.entries.p;
     F.nblk = P.nd.blocks.p;
@@ -1546,9 +1582,21 @@ void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long lon
         else k_far_tiles<R, OPK, RM, 0, 0><<<grid, 256, 0, ctx.stream>>>(A);                \
     } while (0)
     if (P.op.kind == BH_OP_MW_DOUBLELAYER_ROTATED) {
-        // any rule pair through the generic point loops
-        if (rowmajor) k_far_tiles<R, BH_OP_MW_DOUBLELAYER_ROTATED, true, 0, 0><<<grid, 256, 0, ctx.stream>>>(A);
-        else k_far_tiles<R, BH_OP_MW_DOUBLELAYER_ROTATED, false, 0, 0><<<grid, 256, 0, ctx.stream>>>(A);
+        // its default rules (Dunavant degrees 6 and 7: 12 x 13 points) get unrolled inner
+        // point loops, with centroid-shifted phases where they apply; other rules the
+        // generic loops
+        constexpr int DLR = BH_OP_MW_DOUBLELAYER_ROTATED;
+        const bool r67 = P.far.Qo == 12 && P.far.Qi == 13;
+        const bool s67 = sizeof(R) == 8 && r67 && P.far.shifted;
+#define BH_TILES_DLR(RM)                                                                     \
+    do {                                                                                     \
+        if (s67) k_far_tiles<R, DLR, RM, 12, 13, true><<<grid, 256, 0, ctx.stream>>>(A);    \
+        else if (r67) k_far_tiles<R, DLR, RM, 12, 13><<<grid, 256, 0, ctx.stream>>>(A);     \
+        else k_far_tiles<R, DLR, RM, 0, 0><<<grid, 256, 0, ctx.stream>>>(A);                \
+    } while (0)
+        if (rowmajor) BH_TILES_DLR(true);
+        else BH_TILES_DLR(false);
+#undef BH_TILES_DLR
     } else if (rowmajor) {
         if (sl) BH_TILES(BH_OP_MW_SINGLELAYER, true);
         else BH_TILES(BH_OP_MW_DOUBLELAYER, true);
@@ -1676,6 +1724,349 @@ bool assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bo
     return copy32;
 }
 
+// ============================================================ batched assembly
+// The items of a batch (test cells, non-far pairs, dof tiles) of all requests are numbered
+// consecutively; a block or wave finds its request in the table of each request's first item
+// and runs the same body as the single-request kernels, so every entry is bit-identical to
+// the one a request assembled alone would get.
+__device__ __forceinline__ int batch_find(const int *__restrict__ first, int n, int item)
+{
+    int lo = 0, hi = n - 1;   // first[0] = 0 <= item < first[n]
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One request's classification and non-far blocks.
+struct BatchNear {
+    const double *tgeo, *sgeo;
+    int nfs;
+    int mode;
+    double k2;
+    int use_grid, fill_grid;
+    PairGridView grid;
+    int *counts;          // [nft]
+    const int *nptr;      // [nft + 1]
+    int4 *entries;        // [nent]
+    double *blocks;       // [nent][18]
+    int *err;             // hits > 3 seen
+    OpParams op;
+    GLRule gl[3];         // common face, common edge, common vertex
+    const double *tq, *sq;
+    int Qo, Qi;           // near rules (Wilton)
+};
+
+__global__ __launch_bounds__(256) void k_classify_count_batch(const BatchNear *reqs, const int *first, int n)
+{
+    const int r = batch_find(first, n, blockIdx.x);
+    const BatchNear &B = reqs[r];
+    const int p = blockIdx.x - first[r];
+    if (B.use_grid) classify_count_grid_dev(B.tgeo, B.sgeo, B.grid, B.k2, B.mode, p, B.counts, B.err);
+    else classify_count_dev(B.tgeo, B.sgeo, B.nfs, B.k2, B.mode, p, B.counts, B.err);
+}
+
+__global__ __launch_bounds__(256) void k_classify_fill_batch(const BatchNear *reqs, const int *first, int n)
+{
+    const int r = batch_find(first, n, blockIdx.x);
+    const BatchNear &B = reqs[r];
+    const int p = blockIdx.x - first[r];
+    if (B.fill_grid) classify_fill_grid_dev(B.tgeo, B.sgeo, B.grid, B.k2, B.mode, B.nptr, B.entries, p);
+    else classify_fill_dev(B.tgeo, B.sgeo, B.nfs, B.k2, B.mode, B.nptr, B.entries, p);
+}
+
+// (request, entry) lists per (operator, class) over all entries of the batch: list
+// 5 * op.kind + class, with op.kind in {0, 1, 3} (the integral operators)
+constexpr int kBatchLists = 5 * (BH_OP_MW_DOUBLELAYER_ROTATED + 1);
+__global__ void k_class_lists_batch(const BatchNear *reqs, const int *first, int n, int total, int *counters,
+                                    int2 *lists)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const int r = batch_find(first, n, g), e = g - first[r];
+    const int l = 5 * reqs[r].op.kind + reqs[r].entries[e].z;   // class 1..4
+    const int slot = atomicAdd(&counters[l], 1);
+    lists[(size_t)l * total + slot] = make_int2(r, e);
+}
+
+template <int OPK>
+__global__ __launch_bounds__(256) void k_sauter_schwab_batch(const BatchNear *reqs, const int2 *list, int nlist,
+                                                             int cls)
+{
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= nlist) return;
+    const int2 re = list[wv];
+    const BatchNear &B = reqs[re.x];
+    const GLRule &gl = B.gl[cls == kCF ? 0 : (cls == kCE ? 1 : 2)];
+    ss_dev<OPK>(B.tgeo, B.sgeo, B.entries, re.y, cls, gl, B.op, B.blocks, threadIdx.x & 63);
+}
+
+template <int OPK>
+__global__ __launch_bounds__(256) void k_wilton_batch(const BatchNear *reqs, const int2 *list, int nlist)
+{
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= nlist) return;
+    const int2 re = list[wv];
+    const BatchNear &B = reqs[re.x];
+    wilton_dev<OPK>(B.tgeo, B.sgeo, B.entries, re.y, B.tq, B.Qo, B.sq, B.Qi, B.op, B.blocks, threadIdx.x & 63);
+}
+
+// dof tiles of several requests of one kernel configuration; tile t of a request is
+// (row t / column tiles, column t mod column tiles)
+template <typename R, int OPK, bool ROWMAJOR, int QO, int QI, bool SHIFT = false>
+__global__ __launch_bounds__(256) void k_far_tiles_batch(const TileArgs *args, const int *first, int n)
+{
+    const int r = batch_find(first, n, blockIdx.x);
+    const TileArgs A = args[r];
+    const int lt = blockIdx.x - first[r];
+    const int ntc = (A.col.ndofs + kTile - 1) / kTile;
+    far_tiles_dev<R, OPK, ROWMAJOR, QO, QI, SHIFT>(A, lt / ntc, lt % ntc);
+}
+
+namespace {
+
+struct BatchItem {
+    bh_matrix *M;
+    Space *X;
+    Strategy st;
+    OpParams op;
+    FarArgs far;
+    long long nent = 0;   // non-far pairs (classified with their own blocks)
+};
+
+// Classification, compaction and the non-far blocks of all items with a classifying
+// strategy, in shared launches.  Fills each item's far.nptr / nent / nblk.
+void batch_near(Ctx &ctx, std::vector<BatchItem> &items, std::vector<PairGrid> &grids, DevBuf<int> &nptr_all,
+                DevBuf<int4> &entries_all, DevBuf<double> &blocks_all)
+{
+    hipStream_t s = ctx.stream;
+    std::vector<int> idx;
+    for (int i = 0; i < (int)items.size(); ++i)
+        if (items[i].st.mode != kModeDoubleNum && items[i].X->mesh->nf > 0) idx.push_back(i);
+    const int m = (int)idx.size();
+    // DoubleNumQStrat items (and empty meshes) need an empty non-far list: nptr all zero
+    size_t zero_len = 1;
+    for (auto &it : items) zero_len = std::max(zero_len, (size_t)it.X->mesh->nf + 1);
+    std::vector<int> cell_first(m + 1, 0);
+    for (int k = 0; k < m; ++k) cell_first[k + 1] = cell_first[k] + (int)items[idx[k]].X->mesh->nf;
+    const int ncell = cell_first[m];
+    nptr_all.alloc((size_t)ncell + m + zero_len);
+    {
+        std::vector<int> z(zero_len, 0);
+        BH_CHECK_HIP(hipMemcpyAsync(nptr_all.p + ncell + m, z.data(), sizeof(int) * zero_len, hipMemcpyHostToDevice, s));
+        BH_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    for (auto &it : items) {
+        it.far.nptr = nptr_all.p + ncell + m;
+        it.far.nent = nullptr;
+        it.far.nblk = nullptr;
+    }
+    if (m == 0) return;
+    grids.resize(m);
+    std::vector<BatchNear> tab(m);
+    DevBuf<int> counts(ncell), errs(m), dfirst(m + 1);
+    for (int k = 0; k < m; ++k) {
+        BatchItem &it = items[idx[k]];
+        Mesh &tm = *it.X->mesh;
+        build_pair_grid(tm, tm, it.st.mode, s, grids[k]);
+        BatchNear &B = tab[k];
+        B.tgeo = tm.d_geo.p;
+        B.sgeo = tm.d_geo.p;
+        B.nfs = (int)tm.nf;
+        B.mode = it.st.mode;
+        B.k2 = it.op.gr * it.op.gr + it.op.gi * it.op.gi;
+        B.use_grid = grids[k].use ? 1 : 0;
+        B.grid = grids[k].view;
+        B.counts = counts.p + cell_first[k];
+        B.err = errs.p + k;
+        B.op = it.op;
+        const int ord[3] = {it.st.ss_face, it.st.ss_edge, it.st.ss_vert};
+        for (int c = 0; c < 3; ++c) {
+            B.gl[c].n = ord[c];
+            legendre01(ord[c], B.gl[c].x, B.gl[c].w);
+        }
+        if (it.st.mode == kModeWiltonSauter) {
+            B.tq = (const double *)quad_points(tm, it.st.near_o, &B.Qo, false, s);
+            B.sq = (const double *)quad_points(tm, it.st.near_i, &B.Qi, false, s);
+            BH_REQUIRE(B.Qo <= 64, "near outer rule too large");
+        }
+    }
+    DevBuf<char> dtab(sizeof(BatchNear) * m);
+    BH_CHECK_HIP(hipMemsetAsync(errs.p, 0, sizeof(int) * m, s));
+    BH_CHECK_HIP(hipMemcpyAsync(dtab.p, tab.data(), sizeof(BatchNear) * m, hipMemcpyHostToDevice, s));
+    BH_CHECK_HIP(hipMemcpyAsync(dfirst.p, cell_first.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, s));
+    const BatchNear *dreq = (const BatchNear *)dtab.p;
+    {
+        ProfScope prof(ctx, "k_classify_count", (double)ncell);
+        k_classify_count_batch<<<ncell, 256, 0, s>>>(dreq, dfirst.p, m);
+    }
+    check_launch("k_classify_count_batch");
+    std::vector<int> hc(ncell), herr(m);
+    BH_CHECK_HIP(hipMemcpyAsync(hc.data(), counts.p, sizeof(int) * ncell, hipMemcpyDeviceToHost, s));
+    BH_CHECK_HIP(hipMemcpyAsync(herr.data(), errs.p, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+    BH_CHECK_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < m; ++k)
+        if (herr[k]) BH_FAIL(BH_ERR_ASSERT, "AssertionError: hits <= 3 (two cells share more than 3 vertex positions)"
+                                            " in batch item %d", idx[k]);
+    // per-request offsets; entries of request k start at ent_first[k]
+    std::vector<int> off(ncell + m), ent_first(m + 1, 0);
+    for (int k = 0; k < m; ++k) {
+        const int nft = cell_first[k + 1] - cell_first[k];
+        int *o = off.data() + cell_first[k] + k;
+        o[0] = 0;
+        int hmax = 0;
+        for (int p = 0; p < nft; ++p) {
+            o[p + 1] = o[p] + hc[cell_first[k] + p];
+            hmax = std::max(hmax, hc[cell_first[k] + p]);
+        }
+        tab[k].fill_grid = (tab[k].use_grid && hmax <= kGridFillCap) ? 1 : 0;
+        ent_first[k + 1] = ent_first[k] + o[nft];
+    }
+    const int nent = ent_first[m];
+    BH_CHECK_HIP(hipMemcpyAsync(nptr_all.p, off.data(), sizeof(int) * (ncell + m), hipMemcpyHostToDevice, s));
+    entries_all.alloc(std::max(nent, 1));
+    blocks_all.alloc((size_t)std::max(nent, 1) * 18);
+    for (int k = 0; k < m; ++k) {
+        BatchItem &it = items[idx[k]];
+        tab[k].nptr = nptr_all.p + cell_first[k] + k;
+        tab[k].entries = entries_all.p + ent_first[k];
+        tab[k].blocks = blocks_all.p + (size_t)ent_first[k] * 18;
+        it.far.nptr = tab[k].nptr;
+        it.far.nent = tab[k].entries;
+        it.far.nblk = tab[k].blocks;
+        it.nent = ent_first[k + 1] - ent_first[k];
+    }
+    if (nent == 0) return;
+    BH_CHECK_HIP(hipMemcpyAsync(dtab.p, tab.data(), sizeof(BatchNear) * m, hipMemcpyHostToDevice, s));
+    k_classify_fill_batch<<<ncell, 256, 0, s>>>(dreq, dfirst.p, m);
+    check_launch("k_classify_fill_batch");
+    DevBuf<int> efirst(m + 1), counters(kBatchLists);
+    DevBuf<int2> lists((size_t)kBatchLists * nent);
+    BH_CHECK_HIP(hipMemcpyAsync(efirst.p, ent_first.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, s));
+    BH_CHECK_HIP(hipMemsetAsync(counters.p, 0, kBatchLists * sizeof(int), s));
+    k_class_lists_batch<<<(nent + 255) / 256, 256, 0, s>>>(dreq, efirst.p, m, nent, counters.p, lists.p);
+    check_launch("k_class_lists_batch");
+    int hcnt[kBatchLists];
+    BH_CHECK_HIP(hipMemcpyAsync(hcnt, counters.p, kBatchLists * sizeof(int), hipMemcpyDeviceToHost, s));
+    BH_CHECK_HIP(hipStreamSynchronize(s));
+    for (int opk : {BH_OP_MW_SINGLELAYER, BH_OP_MW_DOUBLELAYER, BH_OP_MW_DOUBLELAYER_ROTATED}) {
+        for (int cls : {kCF, kCE, kCV}) {
+            const int l = 5 * opk + cls, cnt = hcnt[l];
+            if (cnt == 0) continue;
+            const int2 *lp = lists.p + (size_t)l * nent;
+            const unsigned grid = (unsigned)((cnt + 3) / 4);
+            ProfScope prof(ctx, "k_sauter_schwab", (double)cnt);
+            if (opk == BH_OP_MW_SINGLELAYER)
+                k_sauter_schwab_batch<BH_OP_MW_SINGLELAYER><<<grid, 256, 0, s>>>(dreq, lp, cnt, cls);
+            else if (opk == BH_OP_MW_DOUBLELAYER)
+                k_sauter_schwab_batch<BH_OP_MW_DOUBLELAYER><<<grid, 256, 0, s>>>(dreq, lp, cnt, cls);
+            else
+                k_sauter_schwab_batch<BH_OP_MW_DOUBLELAYER_ROTATED><<<grid, 256, 0, s>>>(dreq, lp, cnt, cls);
+            check_launch("k_sauter_schwab_batch");
+        }
+        const int l = 5 * opk + kNear, cnt = hcnt[l];
+        if (cnt == 0) continue;
+        BH_REQUIRE(opk != BH_OP_MW_DOUBLELAYER_ROTATED, "near pairs of a rotated double layer");
+        const int2 *lp = lists.p + (size_t)l * nent;
+        ProfScope prof(ctx, "k_wilton", (double)cnt);
+        if (opk == BH_OP_MW_SINGLELAYER)
+            k_wilton_batch<BH_OP_MW_SINGLELAYER><<<(cnt + 3) / 4, 256, 0, s>>>(dreq, lp, cnt);
+        else
+            k_wilton_batch<BH_OP_MW_DOUBLELAYER><<<(cnt + 3) / 4, 256, 0, s>>>(dreq, lp, cnt);
+        check_launch("k_wilton_batch");
+    }
+    BH_CHECK_HIP(hipStreamSynchronize(s));   // the tables go out of scope
+}
+
+// One far-tile launch per kernel configuration (operator, default rules, shifted phases).
+void batch_far(Ctx &ctx, std::vector<BatchItem> &items)
+{
+    hipStream_t s = ctx.stream;
+    struct Group {
+        int opk;
+        int variant;   // rule specialisation and shifted phases (see below)
+        std::vector<int> members;
+    };
+    std::vector<Group> groups;
+    for (int i = 0; i < (int)items.size(); ++i) {
+        BatchItem &it = items[i];
+        if (it.X->ndofs == 0) continue;
+        // 0 generic rules, 1 and 2 the default far rules (3 x 4 points) without and with
+        // shifted phases, 3 and 4 the rotated double layer's default (12 x 13 points)
+        const bool rot = it.op.kind == BH_OP_MW_DOUBLELAYER_ROTATED;
+        const bool dflt = it.far.Qo == 3 && it.far.Qi == 4 && !rot;
+        const bool r67 = it.far.Qo == 12 && it.far.Qi == 13 && rot;
+        const int variant = dflt ? (it.far.shifted ? 2 : 1) : (r67 ? (it.far.shifted ? 4 : 3) : 0);
+        Group *g = nullptr;
+        for (auto &gg : groups)
+            if (gg.opk == it.op.kind && gg.variant == variant) g = &gg;
+        if (!g) {
+            groups.push_back({it.op.kind, variant, {}});
+            g = &groups.back();
+        }
+        g->members.push_back(i);
+    }
+    std::vector<DevBuf<char>> keep;
+    for (const Group &g : groups) {
+        const int n = (int)g.members.size();
+        std::vector<TileArgs> args(n);
+        std::vector<int> first(n + 1, 0);
+        double far_pp = 0.0;
+        for (int k = 0; k < n; ++k) {
+            BatchItem &it = items[g.members[k]];
+            Space &X = *it.X;
+            bh_matrix &M = *it.M;
+            TileArgs &A = args[k];
+            A = TileArgs{};
+            A.far = it.far;
+            A.row = {X.t_ptr.p, X.t_cells.p, X.t_slot.p, X.t_coef.p, (int)X.ndofs};
+            A.col = A.row;
+            A.Z = M.a64.p;
+            A.ld = M.n;
+            A.Z32 = M.lu32.p;
+            A.ld32 = padded_ld(M.n);
+            A.rowmajor = 1;
+            A.accumulate = 0;
+            A.scale_r = 1.0;
+            A.scale_i = 0.0;
+            BH_REQUIRE((long long)first[k] + X.ntiles * X.ntiles < (1LL << 31), "batch has too many dof tiles");
+            first[k + 1] = first[k] + (int)(X.ntiles * X.ntiles);
+            const double nf = (double)X.mesh->nf;
+            far_pp += (nf * nf - (double)it.nent) * it.far.Qo * it.far.Qi;
+        }
+        DevBuf<char> dargs(sizeof(TileArgs) * n), dfirst(sizeof(int) * (n + 1));
+        BH_CHECK_HIP(hipMemcpyAsync(dargs.p, args.data(), sizeof(TileArgs) * n, hipMemcpyHostToDevice, s));
+        BH_CHECK_HIP(hipMemcpyAsync(dfirst.p, first.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+        const TileArgs *da = (const TileArgs *)dargs.p;
+        const int *df = (const int *)dfirst.p;
+        const double per_pp = g.opk == BH_OP_MW_SINGLELAYER ? 170.0 : 210.0;
+        ProfScope prof(ctx, "k_far_tiles", far_pp * per_pp);
+        const unsigned grid = (unsigned)first[n];
+#define BH_BTILES(OPK)                                                                                           \
+    do {                                                                                                         \
+        if (g.variant == 2) k_far_tiles_batch<double, OPK, true, 3, 4, true><<<grid, 256, 0, s>>>(da, df, n);    \
+        else if (g.variant == 1) k_far_tiles_batch<double, OPK, true, 3, 4><<<grid, 256, 0, s>>>(da, df, n);     \
+        else k_far_tiles_batch<double, OPK, true, 0, 0><<<grid, 256, 0, s>>>(da, df, n);                         \
+    } while (0)
+        if (g.opk == BH_OP_MW_SINGLELAYER) BH_BTILES(BH_OP_MW_SINGLELAYER);
+        else if (g.opk == BH_OP_MW_DOUBLELAYER) BH_BTILES(BH_OP_MW_DOUBLELAYER);
+        else if (g.variant == 4)
+            k_far_tiles_batch<double, BH_OP_MW_DOUBLELAYER_ROTATED, true, 12, 13, true><<<grid, 256, 0, s>>>(da, df, n);
+        else if (g.variant == 3)
+            k_far_tiles_batch<double, BH_OP_MW_DOUBLELAYER_ROTATED, true, 12, 13><<<grid, 256, 0, s>>>(da, df, n);
+        else k_far_tiles_batch<double, BH_OP_MW_DOUBLELAYER_ROTATED, true, 0, 0><<<grid, 256, 0, s>>>(da, df, n);
+#undef BH_BTILES
+        check_launch("k_far_tiles_batch");
+        keep.push_back(std::move(dargs));
+        keep.push_back(std::move(dfirst));
+    }
+    BH_CHECK_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
 }  // namespace bh
 
 using namespace bh;
@@ -1771,5 +2162,75 @@ extern "C" int bh_check_strategy(const bh_operator *op, const bh_quadstrat *qs)
         if (p.kind == BH_OP_MW_DOUBLELAYER_ROTATED && st.mode == kModeWiltonSauter)
             BH_FAIL(BH_ERR_UNSUPPORTED, "DoubleLayerRotatedMW3D has no singularity extraction");
     }
+    BH_API_END
+}
+
+namespace bh {
+// Resident matrices of integral operators (FP64 row-major, with the FP32 copy): the batched
+// kernels serve one request as well as many, so a matrix has the same bits whatever batch
+// it was assembled in.
+void assemble_resident_batch(bh_ctx *ctx, int64_t count, const bh_operator *ops, const bh_quadstrat *qs,
+                             const bh_space *const *spaces, bh_matrix **out)
+{
+    for (int64_t i = 0; i < count; ++i) out[i] = nullptr;
+    if (count == 0) return;
+    std::vector<std::unique_ptr<bh_matrix>> mats(count);
+    std::vector<BatchItem> items(count);
+    for (int64_t i = 0; i < count; ++i) {
+        BH_REQUIRE(spaces[i] != nullptr, "space %lld is NULL", (long long)i);
+        Space &X = *const_cast<bh_space *>(spaces[i]);
+        BH_REQUIRE(X.mesh->ctx == ctx, "space %lld belongs to another context", (long long)i);
+        BatchItem &it = items[i];
+        it.op = parse_op(&ops[i]);
+        if (it.op.kind == BH_OP_NCROSS || it.op.kind == BH_OP_IDENTITY)
+            BH_FAIL(BH_ERR_UNSUPPORTED, "batch item %lld: local operators are added with bh_matrix_accumulate",
+                    (long long)i);
+        it.st = parse_qs(&qs[i]);
+        if (it.op.kind == BH_OP_MW_DOUBLELAYER_ROTATED && it.st.mode == kModeWiltonSauter)
+            BH_FAIL(BH_ERR_UNSUPPORTED, "batch item %lld: DoubleLayerRotatedMW3D has no singularity extraction",
+                    (long long)i);
+        it.X = &X;
+        build_tiles(X, ctx->stream);
+        Mesh &m = *X.mesh;
+        FarArgs &F = it.far;
+        F = FarArgs{};
+        F.tgeo = m.d_geo.p;
+        F.sgeo = m.d_geo.p;
+        F.tq = quad_points(m, it.st.far_o, &F.Qo, false, ctx->stream);
+        F.sq = quad_points(m, it.st.far_i, &F.Qi, false, ctx->stream);
+        F.mode = it.st.mode;
+        F.k2 = it.op.gr * it.op.gr + it.op.gi * it.op.gi;
+        F.shifted = shifted_phases(it.op, m, m, false) ? 1 : 0;
+        F.op = it.op;
+        std::unique_ptr<bh_matrix> M(new bh_matrix());
+        M->ctx = ctx;
+        M->n = X.ndofs;
+        pool_alloc(*ctx, M->a64, (size_t)M->n * M->n * 2);
+        if (M->n > 0) pool_alloc(*ctx, M->lu32, (size_t)padded_ld(M->n) * M->n * 2);
+        it.M = M.get();
+        mats[i] = std::move(M);
+    }
+    std::vector<PairGrid> grids;
+    DevBuf<int> nptr_all;
+    DevBuf<int4> entries_all;
+    DevBuf<double> blocks_all;
+    batch_near(*ctx, items, grids, nptr_all, entries_all, blocks_all);
+    batch_far(*ctx, items);
+    for (int64_t i = 0; i < count; ++i) {
+        bh_matrix *M = mats[i].release();
+        M->lu32_fresh = M->n > 0;   // the tiles wrote the FP32 copy
+        ctx_retain(ctx);
+        out[i] = M;
+    }
+}
+}  // namespace bh
+
+extern "C" int bh_matrix_assemble_batch(bh_ctx *ctx, int64_t count, const bh_operator *ops, const bh_quadstrat *qs,
+                                        const bh_space *const *spaces, bh_matrix **out)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctx && ops && qs && spaces && out, "NULL argument");
+    BH_REQUIRE(count >= 0 && count <= (1 << 20), "bad batch size %lld", (long long)count);
+    assemble_resident_batch(ctx, count, ops, qs, spaces, out);
     BH_API_END
 }

@@ -125,6 +125,10 @@ bool assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bo
                      long long ld, bool rowmajor, bool accumulate, double sr, double si, void *dZ32 = nullptr,
                      long long ld32 = 0);
 
+// Resident matrices of integral operators through the batched kernels (one request or many).
+void assemble_resident_batch(bh_ctx *ctx, int64_t count, const bh_operator *ops, const bh_quadstrat *qs,
+                             const bh_space *const *spaces, bh_matrix **out);
+
 // Per-kernel timing with HIP events on the context stream (measurement only).
 struct KernelStat {
     long long launches = 0;

@@ -50,7 +50,13 @@ extern "C" int bh_matrix_assemble(bh_ctx *ctx, const bh_operator *op, const bh_q
 {
     BH_API_BEGIN
     BH_REQUIRE(ctx && X && out, "NULL argument");
+    BH_REQUIRE(op != nullptr, "operator is NULL");
     *out = nullptr;
+    if (op->kind != BH_OP_NCROSS && op->kind != BH_OP_IDENTITY) {
+        // a batch of one: the same kernels as bh_matrix_assemble_batch, the same bits
+        assemble_resident_batch(ctx, 1, op, qs, &X, out);
+        return BH_OK;
+    }
     std::unique_ptr<bh_matrix> M(new bh_matrix());
     M->ctx = ctx;
     M->n = X->ndofs;

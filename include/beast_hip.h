@@ -228,6 +228,16 @@ typedef struct {
  * matrix is factored: an FP32 factorisation starts from it, an FP64 one frees it). */
 int bh_matrix_assemble(bh_ctx *ctx, const bh_operator *op, const bh_quadstrat *qs, const bh_space *X,
                        bh_matrix **out);
+/* Resident matrices of several requests assembled in shared launches (no counterpart in
+ * the reference, which assembles one operator per call, src/operator.jl:100-116): out[i]
+ * = ops[i] on (spaces[i], spaces[i]) under qs[i], for integral operators (local operators
+ * are added with bh_matrix_accumulate).  The classification, the Sauter-Schwab and Wilton
+ * blocks and the far-field tiles of all requests run in one launch per kernel
+ * (configuration), so that requests too small to fill the GPU share it; every matrix (and
+ * its FP32 copy) is bit-identical to the one bh_matrix_assemble gives the request alone.
+ * On failure no matrix is returned. */
+int bh_matrix_assemble_batch(bh_ctx *ctx, int64_t count, const bh_operator *ops, const bh_quadstrat *qs,
+                             const bh_space *const *spaces, bh_matrix **out);
 /* Z += scale * op(X, X) on a resident FP64 matrix (before it is factored); the
  * resident form of a LinearCombinationOfOperators term (operator.jl:294-306). */
 int bh_matrix_accumulate(bh_matrix *M, const bh_operator *op, const bh_quadstrat *qs, const bh_space *X,

@@ -196,3 +196,13 @@ def dq_double_layer(trgauss, gamma, tv, sv, deg_o, deg_i, rotate_test=False):
         for i in range(3):
             z[i, j] = np.sum(W * np.einsum("pk,pqk->pq", F[i], cr))
     return z
+
+
+def read_julia_complex_vector(path):
+    """A column of Julia complex literals ("a + bim"), one per line."""
+    vals = []
+    for line in open(path):
+        line = line.strip()
+        if line:
+            vals.append(complex(line.replace("im", "j").replace(" ", "")))
+    return np.array(vals)

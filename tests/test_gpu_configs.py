@@ -49,8 +49,12 @@ def c2(bj):
 @pytest.mark.parametrize("which", ["efie", "mfie_k"])
 def test_c2_fp64_rows_match_oracle_and_solve(bj, orc, c2, which):
     """C2 in plain FP64: rows of the resident matrix equal the oracle's to 1e-12 (a row
-    sample through the host entry point, which must equal the resident matrix's rows),
-    and the FP64 LU solve leaves a relative residual below 100 eps (test_itsolver.jl:21-24)."""
+    sample through the host entry point, which must equal the resident matrix's rows).
+    The EFIE's FP64 LU solve leaves a relative residual below 100 eps (test_itsolver.jl:21-24).
+    The double layer alone has no identity term and is ill conditioned tested with RT, so
+    its relative residual is amplified by the condition number (2e-12 measured); its solve
+    is held to the normwise backward error ||A x - b|| / (||A|| ||x|| + ||b||) < 100 eps
+    (infinity norms), the bound a backward-stable LU meets whatever the conditioning."""
     m, X = c2
     k = 2 * np.pi
     op = bj.Maxwell3D.singlelayer(wavenumber=k) if which == "efie" else bj.Maxwell3D.doublelayer(wavenumber=k)
@@ -66,9 +70,13 @@ def test_c2_fp64_rows_match_oracle_and_solve(bj, orc, c2, which):
     E = bj.Maxwell3D.planewave(direction=bj.x_hat, polarization=bj.z_hat, wavenumber=k)
     b = bj.assemble(bj.cross(bj.cross(bj.n, E), bj.n), X)
     u = M.solve(b)
-    res = np.linalg.norm(Z @ u - b) / np.linalg.norm(b)
-    _log(f"C2 {which}: FP64 solve relative residual {res:.3e}")
-    assert res < 100 * EPS
+    r = Z @ u - b
+    res = np.linalg.norm(r) / np.linalg.norm(b)
+    bwd = np.abs(r).max() / (np.abs(Z).sum(axis=1).max() * np.abs(u).max() + np.abs(b).max())
+    _log(f"C2 {which}: FP64 solve relative residual {res:.3e}, backward error {bwd:.3e}")
+    assert bwd < 100 * EPS
+    if which == "efie":
+        assert res < 100 * EPS
 
 
 def test_c3_mixed_vs_fp64_at_k4(bj):
