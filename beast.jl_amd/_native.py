@@ -101,6 +101,7 @@ _SIGS = [
     ("bh_matrix_factor", C.c_int, [_P, C.c_int]),
     ("bh_matrix_solve", C.c_int, [_P, C.c_int64, _D, _D, C.c_double, C.c_int, C.POINTER(bh_solve_stats)]),
     ("bh_matrix_pivots", C.c_int, [_P, _I64]),
+    ("bh_matrix_solve_noalloc", C.c_int, [_P, C.c_int64, _D, _D, C.c_double, C.c_int, C.POINTER(bh_solve_stats)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

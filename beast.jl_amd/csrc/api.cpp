@@ -42,6 +42,7 @@ extern "C" int bh_ctx_create(int device, bh_ctx **out)
     BH_CHECK_HIP(hipGetDeviceCount(&n));
     BH_REQUIRE(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
     BH_CHECK_HIP(hipSetDevice(device));
+    bh::scratch_register(device);
     hipDeviceProp_t prop;
     BH_CHECK_HIP(hipGetDeviceProperties(&prop, device));
     std::unique_ptr<bh_ctx> c(new bh_ctx());

@@ -165,6 +165,9 @@ struct Ctx {
 };
 void ctx_retain(Ctx *c);
 void ctx_release(Ctx *c);
+// device address of the allocation-free solve's scratch array, looked up once per device
+// (with that device current) when a context is created there
+void scratch_register(int device);
 
 // Pool of large device buffers (see Ctx::pool).  pool_alloc returns a pooled
 // buffer of at least `bytes` and at most 5/4 of it, else a fresh allocation

@@ -2369,6 +2369,13 @@ void invert_diagonal_blocks(hipStream_t s, const float2 *LU, long long lda, int 
 }
 
 // Synchronisation words of the one-launch triangular solves of one matrix.
+// words[0 .. cnt) = 0 (a kernel rather than a memset node, for callers that capture graphs)
+__global__ void k_fill_int(int *words, long long cnt)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) words[i] = 0;
+}
+
 struct TrsvSync {
     int *words;           // [0] ticket counter, [1] error flag, [2 ..] ready flags
     long long &tickets;   // tickets handed out so far
@@ -2383,7 +2390,8 @@ void lu_solve_inv_n(hipStream_t s, const float2 *A, long long lda, int n, const 
     const float2 *DL = Dinv, *DU = Dinv + bsz, *GL = Dinv + 2 * bsz, *GU = Dinv + 3 * bsz;
     for (int tri = 0; tri < 2; ++tri) {
         if (ts.tickets + nb >= (1LL << 30) || ts.epoch >= (1 << 30)) {   // restart the counters
-            BH_CHECK_HIP(hipMemsetAsync(ts.words, 0, (size_t)(nb + 2) * sizeof(int), s));
+            k_fill_int<<<(unsigned)((nb + 2 + 255) / 256), 256, 0, s>>>(ts.words, nb + 2);
+            check("k_fill_int");
             ts.tickets = 0;
             ts.epoch = 0;
         }
@@ -2459,6 +2467,16 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
         pool_alloc(ctx, M->dinv32, 4 * nb * kIB * kIB * 2);   // L and U diagonal inverses, then their couplings
         lu_factor<float>(ctx, (float2 *)M->lu32.p, lda, (int)n, M->ipiv.p, &info, precision == BH_FP32_SPLIT,
                          (float2 *)M->dinv32.p);
+        // the solves' FP32 vector and their block flags, here so that solving allocates nothing
+        if (M->x32.n < (size_t)n * kMaxRhs * 2) M->x32.alloc((size_t)n * kMaxRhs * 2);
+        const size_t nflags = (size_t)((n + kIB - 1) / kIB) + 2;
+        if (M->tsync.n < nflags) {
+            M->tsync.alloc(nflags);
+            k_fill_int<<<(unsigned)((nflags + 255) / 256), 256, 0, ctx.stream>>>(M->tsync.p, (long long)nflags);
+            check("k_fill_int");
+            M->tsync_tickets = 0;
+            M->tsync_epoch = 0;
+        }
     }
     BH_CHECK_HIP(hipEventRecord(e1, ctx.stream));
     BH_CHECK_HIP(hipEventSynchronize(e1));
@@ -2503,7 +2521,8 @@ void apply_factors(bh_matrix *M, const double2 *B, double2 *Xd, int nrhs)
             const int nb = (n + kIB - 1) / kIB;
             if (M->tsync.n < (size_t)nb + 2) {
                 M->tsync.alloc((size_t)nb + 2);
-                BH_CHECK_HIP(hipMemsetAsync(M->tsync.p, 0, ((size_t)nb + 2) * sizeof(int), s));
+                k_fill_int<<<(unsigned)((nb + 2 + 255) / 256), 256, 0, s>>>(M->tsync.p, (long long)nb + 2);
+                check("k_fill_int");
                 M->tsync_tickets = 0;
                 M->tsync_epoch = 0;
             }
@@ -2539,39 +2558,53 @@ std::complex<double> sum_partials(const std::vector<double2> &hp, int m, int j)
 // The Arnoldi residual estimate is the true residual ||r0 - A d|| in exact
 // arithmetic, so the iteration stops at ||r0 - A d|| <= atol.  Restarts use
 // the recomputed true residual.  Returns the number of iterations.
-int gmres_right(bh_matrix *M, const double2 *r0, double r0norm, double2 *d, double atol, int restart, int maxinner)
+// Device workspace of GMRES (restart r, n unknowns): Vb (r + 1) n, Zb r n, w n, r n,
+// part 256 (r + 1), hd r + 1 complex doubles.
+struct GmresWork {
+    double2 *Vb, *Zb, *w, *r, *part, *hd;
+};
+
+__global__ void k_zero(double2 *x, long long cnt)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) x[i] = double2{0.0, 0.0};
+}
+
+int gmres_right(bh_matrix *M, const double2 *r0, double r0norm, double2 *d, double atol, int restart, int maxinner,
+                const GmresWork &G)
 {
     using C = std::complex<double>;
     hipStream_t s = M->ctx->stream;
     const int n = (int)M->n;
     const unsigned g = (unsigned)((n + 255) / 256);
-    DevBuf<double2> Vb((size_t)(restart + 1) * n), Zb((size_t)restart * n), w(n), r(n),
-        part((size_t)256 * (restart + 1)), hd(restart + 1);
+    double2 *const Vb = G.Vb, *const Zb = G.Zb, *const w = G.w, *const r = G.r, *const part = G.part,
+                   *const hd = G.hd;
     std::vector<double2> hp((size_t)256 * (restart + 1));
-    BH_CHECK_HIP(hipMemsetAsync(d, 0, sizeof(double2) * n, s));
+    k_zero<<<g, 256, 0, s>>>(d, n);
+    check("k_zero");
     double beta = r0norm;
     const double2 *rcur = r0;
     int total = 0;
     while (beta > atol && total < maxinner) {
-        k_scale_copy<<<g, 256, 0, s>>>(rcur, n, 1.0 / beta, Vb.p);
+        k_scale_copy<<<g, 256, 0, s>>>(rcur, n, 1.0 / beta, Vb);
         check("k_scale_copy");
         std::vector<C> H((size_t)(restart + 1) * restart, 0.0), cs(restart), sn(restart), gvec(restart + 1, 0.0);
         gvec[0] = beta;
         int k = 0;
         bool done = false;
         for (; k < restart && total < maxinner; ++k, ++total) {
-            double2 *zk = Zb.p + (size_t)k * n;
-            apply_factors(M, Vb.p + (size_t)k * n, zk, 1);
+            double2 *zk = Zb + (size_t)k * n;
+            apply_factors(M, Vb + (size_t)k * n, zk, 1);
             {
                 ProfScope prof(*M->ctx, "k_matvec", 16.0 * n * (double)n);
-                k_matvec<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, zk, w.p);
+                k_matvec<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, zk, w);
             }
             check("k_matvec");
             std::vector<C> h(k + 1, 0.0);
             for (int pass = 0; pass < 2; ++pass) {
-                k_dots<<<256, 256, 0, s>>>(Vb.p, n, k + 1, w.p, part.p);
+                k_dots<<<256, 256, 0, s>>>(Vb, n, k + 1, w, part);
                 check("k_dots");
-                BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * 256 * (k + 1), hipMemcpyDeviceToHost, s));
+                BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double2) * 256 * (k + 1), hipMemcpyDeviceToHost, s));
                 BH_CHECK_HIP(hipStreamSynchronize(s));
                 std::vector<double2> hh(k + 1);
                 for (int j = 0; j <= k; ++j) {
@@ -2579,13 +2612,13 @@ int gmres_right(bh_matrix *M, const double2 *r0, double r0norm, double2 *d, doub
                     hh[j] = {v.real(), v.imag()};
                     h[j] += v;
                 }
-                hd.upload(hh.data(), k + 1, s);
-                k_axpys<<<g, 256, 0, s>>>(Vb.p, n, k + 1, hd.p, w.p);
+                BH_CHECK_HIP(hipMemcpyAsync(hd, hh.data(), sizeof(double2) * (k + 1), hipMemcpyHostToDevice, s));
+                k_axpys<<<g, 256, 0, s>>>(Vb, n, k + 1, hd, w);
                 check("k_axpys");
             }
-            k_dots<<<256, 256, 0, s>>>(w.p, n, 1, w.p, part.p);
+            k_dots<<<256, 256, 0, s>>>(w, n, 1, w, part);
             check("k_dots");
-            BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * 256, hipMemcpyDeviceToHost, s));
+            BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double2) * 256, hipMemcpyDeviceToHost, s));
             BH_CHECK_HIP(hipStreamSynchronize(s));
             const double hn = std::sqrt(std::max(0.0, sum_partials(hp, 1, 0).real()));
             for (int j = 0; j <= k; ++j) H[(size_t)j * restart + k] = h[j];
@@ -2604,7 +2637,7 @@ int gmres_right(bh_matrix *M, const double2 *r0, double r0norm, double2 *d, doub
             gvec[k + 1] = -sn[k] * gvec[k];
             gvec[k] = std::conj(cs[k]) * gvec[k];
             if (hn != 0 && k + 1 < restart) {
-                k_scale_copy<<<g, 256, 0, s>>>(w.p, n, 1.0 / hn, Vb.p + (size_t)(k + 1) * n);
+                k_scale_copy<<<g, 256, 0, s>>>(w, n, 1.0 / hn, Vb + (size_t)(k + 1) * n);
                 check("k_scale_copy");
             }
             if (std::abs(gvec[k + 1]) <= atol || hn == 0) { ++k; ++total; done = true; break; }
@@ -2618,22 +2651,22 @@ int gmres_right(bh_matrix *M, const double2 *r0, double r0norm, double2 *d, doub
         }
         std::vector<double2> yn(k);
         for (int j = 0; j < k; ++j) yn[j] = {-y[j].real(), -y[j].imag()};
-        hd.upload(yn.data(), k, s);
-        k_axpys<<<g, 256, 0, s>>>(Zb.p, n, k, hd.p, d);
+        BH_CHECK_HIP(hipMemcpyAsync(hd, yn.data(), sizeof(double2) * k, hipMemcpyHostToDevice, s));
+        k_axpys<<<g, 256, 0, s>>>(Zb, n, k, hd, d);
         check("k_axpys");
         if (done || total >= maxinner) break;
         // restart from the true residual r0 - A d
         {
             ProfScope prof(*M->ctx, "k_residual", 16.0 * n * (double)n);
-            k_residual<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, d, r0, r.p, 1);
+            k_residual<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, d, r0, r, 1);
         }
         check("k_residual");
-        k_dots<<<256, 256, 0, s>>>(r.p, n, 1, r.p, part.p);
+        k_dots<<<256, 256, 0, s>>>(r, n, 1, r, part);
         check("k_dots");
-        BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * 256, hipMemcpyDeviceToHost, s));
+        BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double2) * 256, hipMemcpyDeviceToHost, s));
         BH_CHECK_HIP(hipStreamSynchronize(s));
         beta = std::sqrt(std::max(0.0, sum_partials(hp, 1, 0).real()));
-        rcur = r.p;
+        rcur = r;
     }
     BH_CHECK_HIP(hipStreamSynchronize(s));
     return total;
@@ -2657,29 +2690,63 @@ __global__ void k_add_col(double2 *__restrict__ X, int n, int q, int r, const do
 
 }  // namespace
 
-extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, double *X, double tol, int maxit,
-                               bh_solve_stats *stats)
+namespace {
+
+constexpr int kGmresRestart = 30;
+
+// Device workspace of one solve: db, dx, dr (n x nrhs each), rcol, dcol (n each) and the
+// GMRES workspace, each piece 256-byte aligned inside one buffer.
+struct SolveWork {
+    double2 *db, *dx, *dr, *rcol, *dcol;
+    GmresWork g;
+};
+
+size_t solve_work_layout(int n, int nrhs, char *base, SolveWork *W)
 {
-    BH_API_BEGIN
-    BH_REQUIRE(M && B && X, "NULL argument");
-    BH_REQUIRE(M->factored == BH_FP64 || M->factored == BH_FP32, "matrix is not factored");
-    BH_REQUIRE(nrhs >= 1 && nrhs <= 8, "nrhs must be 1..8");
-    std::lock_guard<std::mutex> lock(M->mu);
+    size_t off = 0;
+    auto take = [&](size_t count) {
+        double2 *p = base ? (double2 *)(base + off) : nullptr;
+        off += (count * sizeof(double2) + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t nq = (size_t)n * nrhs, r = kGmresRestart;
+    SolveWork w{};
+    w.db = take(nq);
+    w.dx = take(nq);
+    w.dr = take(nq);
+    w.rcol = take(n);
+    w.dcol = take(n);
+    w.g.Vb = take((r + 1) * n);
+    w.g.Zb = take(r * n);
+    w.g.w = take(n);
+    w.g.r = take(n);
+    w.g.part = take(256 * (r + 1));
+    w.g.hd = take(r + 1);
+    if (W) *W = w;
+    return off;
+}
+
+// X = Z \ B with the factors of M (see bh_matrix_solve); W the device workspace.  With
+// timed = false no HIP events are created (stats->seconds_solve is then host wall time).
+void solve_impl(bh_matrix *M, int64_t nrhs, const double *B, double *X, double tol, int maxit,
+                bh_solve_stats *stats, const SolveWork &W, bool timed)
+{
     hipStream_t s = M->ctx->stream;
     const int n = (int)M->n, q = (int)nrhs;
     const long long cnt = (long long)n * q;
-    const unsigned g = (unsigned)((cnt + 255) / 256);
     // host column-major -> device row-major n x nrhs
     std::vector<double2> hb(cnt), hx(cnt);
     for (int r = 0; r < q; ++r)
         for (int i = 0; i < n; ++i) hb[(size_t)i * q + r] = {B[2 * ((size_t)r * n + i)], B[2 * ((size_t)r * n + i) + 1]};
-    DevBuf<double2> db(cnt), dx(cnt), dr(cnt);
-    db.upload(hb.data(), cnt, s);
-    hipEvent_t e0, e1;
-    BH_CHECK_HIP(hipEventCreate(&e0));
-    BH_CHECK_HIP(hipEventCreate(&e1));
-    BH_CHECK_HIP(hipEventRecord(e0, s));
-    apply_factors(M, db.p, dx.p, q);
+    BH_CHECK_HIP(hipMemcpyAsync(W.db, hb.data(), sizeof(double2) * cnt, hipMemcpyHostToDevice, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const auto h0 = std::chrono::steady_clock::now();
+    if (timed) {
+        BH_CHECK_HIP(hipEventCreate(&e0));
+        BH_CHECK_HIP(hipEventCreate(&e1));
+        BH_CHECK_HIP(hipEventRecord(e0, s));
+    }
+    apply_factors(M, W.db, W.dx, q);
     int iters = 0, inner = 0;
     double res = 0;
     std::vector<double> bn(q);
@@ -2694,7 +2761,8 @@ extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, doub
         BH_CHECK_HIP(hipStreamSynchronize(s));
         if (!trsv_err) return;
         // results are void; the counters restart
-        BH_CHECK_HIP(hipMemsetAsync(M->tsync.p, 0, M->tsync.n * sizeof(int), s));
+        k_fill_int<<<(unsigned)((M->tsync.n + 255) / 256), 256, 0, s>>>(M->tsync.p, (long long)M->tsync.n);
+        check("k_fill_int");
         BH_CHECK_HIP(hipStreamSynchronize(s));
         M->tsync_tickets = 0;
         M->tsync_epoch = 0;
@@ -2703,15 +2771,14 @@ extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, doub
     const bool refine = M->factored == BH_FP32 || tol > 0;
     if (refine) {
         const int maxit_ = maxit > 0 ? maxit : 20;
-        DevBuf<double2> rcol(n), dcol(n);
         for (;;) {
             {
                 ProfScope prof(*M->ctx, "k_residual", 16.0 * n * (double)n);   // one FP64 read of A
-                k_residual<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, dx.p, db.p, dr.p, q);
+                k_residual<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, W.dx, W.db, W.dr, q);
             }
             check("k_residual");
             std::vector<double2> hr(cnt);
-            BH_CHECK_HIP(hipMemcpyAsync(hr.data(), dr.p, sizeof(double2) * cnt, hipMemcpyDeviceToHost, s));
+            BH_CHECK_HIP(hipMemcpyAsync(hr.data(), W.dr, sizeof(double2) * cnt, hipMemcpyDeviceToHost, s));
             BH_CHECK_HIP(hipStreamSynchronize(s));
             check_trsv();
             res = 0;
@@ -2728,24 +2795,27 @@ extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, doub
             for (int r = 0; r < q; ++r) {
                 if (rel[r] <= target) continue;
                 // solve A d = r_r to 1/2 of the remaining target, then x_r += d
-                k_get_col<<<gn, 256, 0, s>>>(dr.p, n, q, r, rcol.p);
+                k_get_col<<<gn, 256, 0, s>>>(W.dr, n, q, r, W.rcol);
                 check("k_get_col");
                 const double rn = rel[r] * bn[r];
-                inner += gmres_right(M, rcol.p, rn, dcol.p, 0.5 * target * bn[r], 30, 200);
-                k_add_col<<<gn, 256, 0, s>>>(dx.p, n, q, r, dcol.p);
+                inner += gmres_right(M, W.rcol, rn, W.dcol, 0.5 * target * bn[r], kGmresRestart, 200, W.g);
+                k_add_col<<<gn, 256, 0, s>>>(W.dx, n, q, r, W.dcol);
                 check("k_add_col");
             }
         }
     }
-    BH_CHECK_HIP(hipEventRecord(e1, s));
-    BH_CHECK_HIP(hipEventSynchronize(e1));
     float ms = 0;
-    BH_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    BH_CHECK_HIP(hipMemcpyAsync(hx.data(), dx.p, sizeof(double2) * cnt, hipMemcpyDeviceToHost, s));
+    if (timed) {
+        BH_CHECK_HIP(hipEventRecord(e1, s));
+        BH_CHECK_HIP(hipEventSynchronize(e1));
+        BH_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    BH_CHECK_HIP(hipMemcpyAsync(hx.data(), W.dx, sizeof(double2) * cnt, hipMemcpyDeviceToHost, s));
     BH_CHECK_HIP(hipStreamSynchronize(s));
     check_trsv();
+    if (!timed) ms = (float)(1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count());
     for (int r = 0; r < q; ++r)
         for (int i = 0; i < n; ++i) {
             X[2 * ((size_t)r * n + i)] = hx[(size_t)i * q + r].x;
@@ -2760,6 +2830,79 @@ extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, doub
     }
     if (refine && tol > 0 && res > tol)
         BH_FAIL(BH_ERR_NOCONV, "refinement stopped at relative residual %.3e > %.3e after %d steps", res, tol, iters);
+}
+
+}  // namespace
+
+extern "C" int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, double *X, double tol, int maxit,
+                               bh_solve_stats *stats)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(M && B && X, "NULL argument");
+    BH_REQUIRE(M->factored == BH_FP64 || M->factored == BH_FP32, "matrix is not factored");
+    BH_REQUIRE(nrhs >= 1 && nrhs <= 8, "nrhs must be 1..8");
+    std::lock_guard<std::mutex> lock(M->mu);
+    DevBuf<char> work(solve_work_layout((int)M->n, (int)nrhs, nullptr, nullptr));
+    SolveWork W;
+    solve_work_layout((int)M->n, (int)nrhs, work.p, &W);
+    solve_impl(M, nrhs, B, X, tol, maxit, stats, W, true);
+    BH_API_END
+}
+
+// ---------------------------------------------------------------- allocation-free solve
+namespace {
+
+// Scratch memory of bh_matrix_solve_noalloc: one copy per device, held until the process
+// ends.  Host code never uses the array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device, when a context is created there.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+namespace bh {
+// Called by bh_ctx_create with the context's device current.
+void scratch_register(int device)
+{
+    BH_REQUIRE(device >= 0 && device < kMaxDevices, "device %d beyond the scratch table", device);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    BH_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BH_REQUIRE(p != nullptr, "no device address for the solve scratch on device %d", device);
+    g_scratch_ptr[device] = p;
+}
+}  // namespace bh
+
+extern "C" int bh_matrix_solve_noalloc(bh_matrix *M, int64_t nrhs, const double *B, double *X, double tol,
+                                       int maxit, bh_solve_stats *stats)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(M && B && X, "NULL argument");
+    BH_REQUIRE(M->factored == BH_FP64 || M->factored == BH_FP32, "matrix is not factored");
+    BH_REQUIRE(nrhs >= 1 && nrhs <= 8, "nrhs must be 1..8");
+    // apply_factors allocates nothing when bh_matrix_factor has sized these buffers
+    BH_REQUIRE(M->factored == BH_FP64 ||
+                   (M->x32.n >= (size_t)M->n * nrhs * 2 && M->tsync.n >= (size_t)((M->n + kIB - 1) / kIB) + 2),
+               "FP32 factors without their solve buffers");
+    const size_t need = solve_work_layout((int)M->n, (int)nrhs, nullptr, nullptr);
+    BH_REQUIRE(need <= kScratchBytes,
+               "bh_matrix_solve_noalloc: n = %lld with %lld right-hand sides needs %zu bytes of scratch, more than "
+               "its %zu (use bh_matrix_solve)", (long long)M->n, (long long)nrhs, need, kScratchBytes);
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        base = (M->ctx->device >= 0 && M->ctx->device < kMaxDevices) ? g_scratch_ptr[M->ctx->device] : nullptr;
+    }
+    BH_REQUIRE(base != nullptr, "no solve scratch registered for device %d", M->ctx->device);
+    std::lock_guard<std::mutex> lock(M->mu);
+    SolveWork W;
+    solve_work_layout((int)M->n, (int)nrhs, (char *)base, &W);
+    solve_impl(M, nrhs, B, X, tol, maxit, stats, W, false);
     BH_API_END
 }
 
