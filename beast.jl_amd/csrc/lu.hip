@@ -1168,7 +1168,10 @@ __global__ __launch_bounds__(256) void k_split_cols(const float2 *__restrict__ B
 }
 
 // C[M x N] -= A B from split planes PA[4][M][K], PB[4][N][K]; K a multiple of 32.
-// WM x WN waves, wave tile (16 TM) x (16 TN) complex.
+// WM x WN waves, wave tile (16 TM) x (16 TN) complex.  Two LDS stages (160 KB for the
+// 128 x 128 tile): the global loads of step k+1 are in flight while step k's products run
+// from the other stage, one barrier per step; the first row block of the C tile is read
+// during the last step's products.
 template <int TM, int TN, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *__restrict__ PA,
                                                                  const __bf16 *__restrict__ PB, float2 *C,
@@ -1176,8 +1179,8 @@ __global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *
 {
     constexpr int NT = 64 * WM * WN;
     constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN, BK = kSplitBK, RS = BK + kSplitPad;
-    __shared__ __align__(16) __bf16 As[4][BM][RS];
-    __shared__ __align__(16) __bf16 Bs[4][BN][RS];
+    __shared__ __align__(16) __bf16 As[2][4][BM][RS];
+    __shared__ __align__(16) __bf16 Bs[2][4][BN][RS];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv / WN, wn = wv % WN;
     int tm, tn;
@@ -1208,13 +1211,13 @@ __global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *
             }
         }
     };
-    auto store = [&]() {
+    auto store = [&](int st) {
 #pragma unroll
         for (int q = 0; q < AP; ++q) {
             const int c = tid + NT * q;
             if (c < ACH) {
                 const int p = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, g = rem % 4;
-                *(uint4 *)&As[p][r][8 * g] = ra[q];
+                *(uint4 *)&As[st][p][r][8 * g] = ra[q];
             }
         }
 #pragma unroll
@@ -1222,7 +1225,7 @@ __global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *
             const int c = tid + NT * q;
             if (c < BCH) {
                 const int p = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, g = rem % 4;
-                *(uint4 *)&Bs[p][r][8 * g] = rb[q];
+                *(uint4 *)&Bs[st][p][r][8 * g] = rb[q];
             }
         }
     };
@@ -1236,24 +1239,20 @@ __global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *
             ari[a][b] = f32x4{0, 0, 0, 0};
         }
     const int li = lane & 15, lg = lane >> 4;   // row/col within a 16-block, k group of 8
-    load(0);
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        store();
-        __syncthreads();
-        if (k0 + BK < K) load(k0 + BK);
+    // products of one k-step from LDS stage st
+    auto compute = [&](int st) {
         bf16x8 afr[TM][4];
 #pragma unroll
         for (int a = 0; a < TM; ++a)
 #pragma unroll
-            for (int p = 0; p < 4; ++p) afr[a][p] = *(const bf16x8 *)&As[p][wm * 16 * TM + a * 16 + li][8 * lg];
+            for (int p = 0; p < 4; ++p) afr[a][p] = *(const bf16x8 *)&As[st][p][wm * 16 * TM + a * 16 + li][8 * lg];
 #pragma unroll
         for (int b = 0; b < TN; ++b) {
             bf16x8 bfr[4];
 #pragma unroll
-            for (int p = 0; p < 4; ++p) bfr[p] = *(const bf16x8 *)&Bs[p][wn * 16 * TN + b * 16 + li][8 * lg];
+            for (int p = 0; p < 4; ++p) bfr[p] = *(const bf16x8 *)&Bs[st][p][wn * 16 * TN + b * 16 + li][8 * lg];
 #pragma unroll
             for (int a = 0; a < TM; ++a) {
-                // planes: 0 Re hi, 1 Re lo, 2 Im hi, 3 Im lo
                 arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[0], arr[a][b], 0, 0, 0);
                 aii[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], bfr[2], aii[a][b], 0, 0, 0);
                 ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[2], ari[a][b], 0, 0, 0);
@@ -1268,27 +1267,55 @@ __global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *
                 ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][3], bfr[0], ari[a][b], 0, 0, 0);
             }
         }
+    };
+    auto cidx = [&](int a, int b, int r, long long &gi, long long &gj) {
+        gi = m0 + wm * 16 * TM + a * 16 + 4 * (lane >> 4) + r;
+        gj = n0 + wn * 16 * TN + b * 16 + (lane & 15);
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    int st = 0;
+    for (int k0 = 0; k0 + BK < K; k0 += BK, st ^= 1) {
+        load(k0 + BK);
+        compute(st);
+        store(st ^ 1);   // the other stage was last read before the previous barrier
         __syncthreads();
     }
-    float2 cv[TM][TN][4];   // all C reads before any write
+    // last k-step: the first row block of this wave's C tile is read while its products run
+    float2 c0[TN][4];
 #pragma unroll
-    for (int a = 0; a < TM; ++a)
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            long long gi, gj;
+            cidx(0, b, r, gi, gj);
+            c0[b][r] = (gi < M && gj < N) ? C[gi * ldc + gj] : float2{0, 0};
+        }
+    compute(st);
+    float2 cv[TM][TN][4];
+#pragma unroll
+    for (int a = 1; a < TM; ++a)
 #pragma unroll
         for (int b = 0; b < TN; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const long long gi = m0 + wm * 16 * TM + a * 16 + 4 * (lane >> 4) + r;
-                const long long gj = n0 + wn * 16 * TN + b * 16 + (lane & 15);
+                long long gi, gj;
+                cidx(a, b, r, gi, gj);
                 cv[a][b][r] = (gi < M && gj < N) ? C[gi * ldc + gj] : float2{0, 0};
             }
 #pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[0][b][r] = c0[b][r];
+#pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
         for (int b = 0; b < TN; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const long long gi = m0 + wm * 16 * TM + a * 16 + 4 * (lane >> 4) + r;
-                const long long gj = n0 + wn * 16 * TN + b * 16 + (lane & 15);
+                long long gi, gj;
+                cidx(a, b, r, gi, gj);
                 if (gi < M && gj < N) {
                     float2 c = cv[a][b][r];
                     c.x -= arr[a][b][r] - aii[a][b][r];
