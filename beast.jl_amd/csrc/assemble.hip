@@ -1070,8 +1070,9 @@ __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
         s_ccoef[l][s] = n < A.col.ndofs ? (R)A.col.coef[2 * n + s] : (R)0;
     }
     for (int k = tid; k < kTileCells * kTileCells / 32; k += 256) s_near[k] = 0u;
-    // ownership: ROWMAJOR -> one column, 16 rows; else one row, 16 columns
-    const int own = tid & 63, grp = tid >> 6;
+    // ownership: a lane owns one row dof and 16 column dofs (its wave's), so that in the
+    // gather a column dof's cell lies in the current chunk or not for the whole wave
+    const int own = tid & 63, grp = __builtin_amdgcn_readfirstlane(tid >> 6);
     cx<R> acc[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = {(R)0, (R)0};
@@ -1113,11 +1114,11 @@ __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
             far_block<R, OPK, QO, QI, SHIFT>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kPairChunk, near);
         }
         __syncthreads();
-        // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184
+        // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184;
+        // the column dof (and so whether its cell is in this chunk) is uniform per wave
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-            const int ml = ROWMAJOR ? grp * 16 + e : own;
-            const int nl = ROWMAJOR ? own : grp * 16 + e;
+            const int nl = grp * 16 + e;
 #pragma unroll
             for (int sb = 0; sb < 2; ++sb) {
                 const int cslt = s_cslot[nl][sb];
@@ -1128,11 +1129,11 @@ __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
                 const R bc = s_ccoef[nl][sb];
 #pragma unroll
                 for (int sa = 0; sa < 2; ++sa) {
-                    const int rslt = s_rslot[ml][sa];
+                    const int rslt = s_rslot[own][sa];
                     if (rslt < 0) continue;
                     const int i = rslt & 3;
                     const cx<R> zz = s_blk[(lc * nr + (rslt >> 2)) + (i + 3 * j) * kPairChunk];
-                    const R ac = s_rcoef[ml][sa];
+                    const R ac = s_rcoef[own][sa];
                     acc[e].r += ac * (zz.r * bc);
                     acc[e].i += ac * (zz.i * bc);
                 }
@@ -1140,9 +1141,19 @@ __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
         }
         __syncthreads();
     }
-    // write once
+    // write once; a row-major Z is written along rows through an LDS transpose (the pair
+    // blocks' space, free after the last chunk's barrier)
     cx<R> *Z = (cx<R> *)A.Z;
     const R sr = (R)A.scale_r, si = (R)A.scale_i;
+    if constexpr (ROWMAJOR) {
+        static_assert(kTile * (kTile + 1) <= kPairChunk * 9, "transpose does not fit the pair-block space");
+        cx<R> *s_t = s_blk;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) s_t[own * (kTile + 1) + grp * 16 + e] = acc[e];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[e] = s_t[(grp * 16 + e) * (kTile + 1) + own];
+    }
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
         int m, n;
