@@ -298,26 +298,30 @@ __device__ __forceinline__ cx<R> green(R gr, R gi, R dist, R idist)
     return {c * amp, -s * amp};
 }
 
-// Block from EFIE moments: z_ij = s1 [alpha (Mxy - a_i.My - b_j.Mx + (a_i.b_j) M0) + 4 beta M0]
+// Block from EFIE moments: z_ij = s1 [alpha (Mxy - a_i.My - b_j.Mx + (a_i.b_j) M0) + 4 beta M0],
+// with a_i.My and b_j.Mx formed once per i and per j, and s1 folded into alpha and beta
 template <typename R>
 __device__ __forceinline__ void efie_block(const OpParams &op, R s1, const R *a, const R *b, cx<R> M0, const cx<R> *Mx,
                                           const cx<R> *My, cx<R> Mxy, cx<R> *z, int stride)
 {
-    const cx<R> al{(R)op.ar, (R)op.ai}, be{(R)op.br, (R)op.bi};
-    const cx<R> bM0 = cmul(be, M0);
+    const cx<R> sa{s1 * (R)op.ar, s1 * (R)op.ai};
+    const cx<R> bM0 = cmul(cx<R>{(R)op.br, (R)op.bi}, M0);
+    const cx<R> c4{(R)4 * s1 * bM0.r, (R)4 * s1 * bM0.i};
+    cx<R> AY[3], BX[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const R *ai = a + 3 * i, *bi = b + 3 * i;
+        AY[i] = {ai[0] * My[0].r + ai[1] * My[1].r + ai[2] * My[2].r, ai[0] * My[0].i + ai[1] * My[1].i + ai[2] * My[2].i};
+        BX[i] = {Mxy.r - (bi[0] * Mx[0].r + bi[1] * Mx[1].r + bi[2] * Mx[2].r),
+                 Mxy.i - (bi[0] * Mx[0].i + bi[1] * Mx[1].i + bi[2] * Mx[2].i)};
+    }
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            const R *ai = a + 3 * i, *bj = b + 3 * j;
-            const R ab = dot3(ai, bj);
-            cx<R> t;
-            t.r = Mxy.r - (ai[0] * My[0].r + ai[1] * My[1].r + ai[2] * My[2].r) -
-                  (bj[0] * Mx[0].r + bj[1] * Mx[1].r + bj[2] * Mx[2].r) + ab * M0.r;
-            t.i = Mxy.i - (ai[0] * My[0].i + ai[1] * My[1].i + ai[2] * My[2].i) -
-                  (bj[0] * Mx[0].i + bj[1] * Mx[1].i + bj[2] * Mx[2].i) + ab * M0.i;
-            const cx<R> at = cmul(al, t);
-            z[(i + 3 * j) * stride] = {s1 * (at.r + (R)4 * bM0.r), s1 * (at.i + (R)4 * bM0.i)};
+            const R ab = dot3(a + 3 * i, b + 3 * j);
+            const cx<R> t{BX[j].r - AY[i].r + ab * M0.r, BX[j].i - AY[i].i + ab * M0.i};
+            z[(i + 3 * j) * stride] = {sa.r * t.r - sa.i * t.i + c4.r, sa.r * t.i + sa.i * t.r + c4.i};
         }
 }
 
