@@ -99,6 +99,7 @@ _SIGS = [
     ("bh_matrix_size", C.c_int, [_P, _I64]),
     ("bh_matrix_download", C.c_int, [_P, _D, C.c_int64]),
     ("bh_matrix_factor", C.c_int, [_P, C.c_int]),
+    ("bh_matrix_factor_batch", C.c_int, [_P, C.c_int64, C.POINTER(_P), C.c_int]),
     ("bh_matrix_solve", C.c_int, [_P, C.c_int64, _D, _D, C.c_double, C.c_int, C.POINTER(bh_solve_stats)]),
     ("bh_matrix_pivots", C.c_int, [_P, _I64]),
     ("bh_matrix_solve_noalloc", C.c_int, [_P, C.c_int64, _D, _D, C.c_double, C.c_int, C.POINTER(bh_solve_stats)]),

@@ -403,6 +403,22 @@ class DeviceMatrix:
                                                            "fp32split": nat.FP32_SPLIT}[precision]))
         return self
 
+    @staticmethod
+    def factor_batch(mats, precision="fp32split"):
+        """Factor several matrices of one context in shared launches
+        (bh_matrix_factor_batch); each gets the factors ``factor`` gives it alone."""
+        mats = list(mats)
+        if not mats:
+            return mats
+        ctx = mats[0].ctx
+        if any(M.ctx is not ctx for M in mats):
+            raise ValueError("factor_batch: the matrices belong to different contexts")
+        hs = (C.c_void_p * len(mats))(*[M.h.value for M in mats])
+        prec = {"fp64": nat.FP64, "fp32": nat.FP32, "fp32split": nat.FP32_SPLIT}[precision]
+        with ctx.lock:
+            nat.check(nat.load().bh_matrix_factor_batch(ctx.handle, len(mats), hs, prec))
+        return mats
+
     def pivots(self):
         ip = np.zeros(self.n, dtype=np.int64)
         with self.ctx.lock:

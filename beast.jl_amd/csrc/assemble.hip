@@ -1744,16 +1744,6 @@ bool assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bo
 // consecutively; a block or wave finds its request in the table of each request's first item
 // and runs the same body as the single-request kernels, so every entry is bit-identical to
 // the one a request assembled alone would get.
-__device__ __forceinline__ int batch_find(const int *__restrict__ first, int n, int item)
-{
-    int lo = 0, hi = n - 1;   // first[0] = 0 <= item < first[n]
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (first[mid] <= item) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 // One request's classification and non-far blocks.
 struct BatchNear {

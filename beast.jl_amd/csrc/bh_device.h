@@ -165,4 +165,17 @@ __device__ __forceinline__ R dot3c_i(const R *a, const cx<R> *b)
     return a[0] * b[0].i + a[1] * b[1].i + a[2] * b[2].i;
 }
 
+// Request of a batched launch's item: first[r] <= item < first[r + 1] (first[0] = 0,
+// first[n] = the number of items), by binary search.
+__device__ __forceinline__ int batch_find(const int *__restrict__ first, int n, int item)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 }  // namespace bh

@@ -48,9 +48,11 @@
 #include <climits>
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <memory>
 #include <vector>
 
+#include "bh_device.h"
 #include "bh_internal.h"
 #include "bh_matrix.h"
 
@@ -443,11 +445,11 @@ struct TsluWork {
 // before the moves and the row-move plan.
 // Workgroups [nsets0, ...) are the tiles of A[c:, c+kPW : c+kPW+upd_n) -=
 // L21 U12 with the same L21 and U12 (the rest of the previous panel's update).
-__global__ __launch_bounds__(kTsluT, 2) void k_tslu_panel(float2 *A, long long lda, int n, int c, int pw, int nsets0,
-                                                          int prev, int upd_n, TsluWork tw, TsluFinal fin)
+__device__ __forceinline__ void tslu_panel_body(float2 *A, long long lda, int n, int c, int pw, int nsets0, int prev,
+                                                int upd_n, const TsluWork &tw, const TsluFinal &fin, int bid)
 {
-    if ((int)blockIdx.x >= nsets0) {
-        cgemm_tile<float, 4, 2, true>((int)blockIdx.x - nsets0, A + (long long)c * lda + c - kPW, lda,
+    if (bid >= nsets0) {
+        cgemm_tile<float, 4, 2, true>(bid - nsets0, A + (long long)c * lda + c - kPW, lda,
                                       A + (long long)(c - kPW) * lda + c + kPW, lda, A + (long long)c * lda + c + kPW,
                                       lda, n - c, upd_n, kPW);
         return;
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(kTsluT, 2) void k_tslu_panel(float2 *A, long long l
     __shared__ int s_win[kPW], s_at[kPW], s_pos[kPW];
     __shared__ __align__(16) float2 s_u[kPW][kPW];
     const int tid = threadIdx.x;
-    int g = blockIdx.x, nl = nsets0, off = 0, coff = 0;
+    int g = bid, nl = nsets0, off = 0, coff = 0;
     int rowid = -1;
     float2 v[kPW], orig[kPW];
     {
@@ -612,14 +614,20 @@ __global__ __launch_bounds__(kTsluT, 2) void k_tslu_panel(float2 *A, long long l
     }
 }
 
+__global__ __launch_bounds__(kTsluT, 2) void k_tslu_panel(float2 *A, long long lda, int n, int c, int pw, int nsets0,
+                                                          int prev, int upd_n, TsluWork tw, TsluFinal fin)
+{
+    tslu_panel_body(A, lda, n, c, pw, nsets0, prev, upd_n, tw, fin, blockIdx.x);
+}
+
 // After a tournament: workgroups [0, nsw) move the planned rows in the columns
 // [a0, a1) and [b0, b1) (16 columns each, all sources read before any
 // destination is written); the others finish the panel rows r >= c: rows
 // c .. c+pw become L11 \ U11, every other row x (its pre-move panel entries,
 // from oldtop when it was a panel row) becomes L21 = x U11^-1.
-__global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, int n, int c, int pw, const float2 *top,
-                                                    const float2 *oldtop, const int *plan, int a0, int a1, int b0,
-                                                    int b1, int nsw)
+__device__ __forceinline__ void tslu_apply_body(float2 *A, long long lda, int n, int c, int pw, const float2 *top,
+                                                const float2 *oldtop, const int *plan, int a0, int a1, int b0, int b1,
+                                                int nsw, int bid)
 {
     __shared__ int s_dst[kPlanMax], s_src[kPlanMax], s_m;
     __shared__ float2 s_buf[kPlanMax][16];
@@ -632,9 +640,9 @@ __global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, in
     }
     __syncthreads();
     const int m = s_m;
-    if ((int)blockIdx.x < nsw) {
+    if (bid < nsw) {
         const int cl = tid & 15, rl = tid >> 4;
-        const int t = blockIdx.x * 16 + cl, w1 = a1 - a0;
+        const int t = bid * 16 + cl, w1 = a1 - a0;
         int col = -1;
         if (t < w1) col = a0 + t;
         else if (t < w1 + (b1 - b0)) col = b0 + (t - w1);
@@ -643,7 +651,7 @@ __global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, in
         __syncthreads();
         for (int s = rl; s < m; s += 16)
             if (col >= 0) A[(long long)s_dst[s] * lda + col] = s_buf[s][cl];
-        if ((int)blockIdx.x * 16 + 16 <= w1 || b1 <= b0) return;   // no columns right of the panel
+        if (bid * 16 + 16 <= w1 || b1 <= b0) return;   // no columns right of the panel
         // the panel's block row in the columns to its right: U12 = L11^-1 (moved rows c .. c+pw),
         // L11 the unit lower part of top; one thread per column, the column in registers
         for (int idx = tid; idx < kPW * kPW; idx += 256) U[idx / kPW][idx % kPW] = top[idx];
@@ -668,7 +676,7 @@ __global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, in
     }
     for (int idx = tid; idx < kPW * kPW; idx += 256) U[idx / kPW][idx % kPW] = top[idx];
     __syncthreads();
-    const long long r = (long long)c + (long long)(blockIdx.x - nsw) * 256 + tid;
+    const long long r = (long long)c + (long long)(bid - nsw) * 256 + tid;
     if (r >= n) return;
     float2 *row = A + r * lda + c;
     if (r < c + pw) {
@@ -690,6 +698,13 @@ __global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, in
 #pragma unroll
     for (int cc = 0; cc < kPW; ++cc)
         if (cc < pw) row[cc] = x[cc];
+}
+
+__global__ __launch_bounds__(256) void k_tslu_apply(float2 *A, long long lda, int n, int c, int pw, const float2 *top,
+                                                    const float2 *oldtop, const int *plan, int a0, int a1, int b0,
+                                                    int b1, int nsw)
+{
+    tslu_apply_body(A, lda, n, c, pw, top, oldtop, plan, a0, a1, b0, b1, nsw, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- row interchanges
@@ -716,7 +731,7 @@ inline int fp32_block_depth(int n, int k)
     return n - k > kNBDeepAbove ? kNBFP32 : kNBFP32Small;
 }
 
-__global__ __launch_bounds__(256) void k_swap_plan(const long long *ipiv, int k1, int k2, int *plan)
+__device__ __forceinline__ void swap_plan_body(const long long *ipiv, int k1, int k2, int *plan)
 {
     __shared__ int s_piv[kMaxSwaps], s_partner[kMaxSwaps], s_src[2 * kMaxSwaps];
     const int len = k2 - k1, tid = threadIdx.x;
@@ -751,6 +766,11 @@ __global__ __launch_bounds__(256) void k_swap_plan(const long long *ipiv, int k1
     for (int s = tid; s < 2 * len; s += 256) plan[s] = s_src[s];
 }
 
+__global__ __launch_bounds__(256) void k_swap_plan(const long long *ipiv, int k1, int k2, int *plan)
+{
+    swap_plan_body(ipiv, k1, k2, plan);
+}
+
 // Apply a swap plan of 2 len slots to columns [c0, c1) and [c2, c3): CW
 // columns per workgroup (64 KiB of LDS at most), every moved row read into LDS
 // before any is written.
@@ -758,15 +778,15 @@ template <typename T>
 constexpr int laswp_cols() { return sizeof(T) == 4 ? 16 : 8; }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_laswp_apply(typename CT<T>::t *A, long long lda, const int *plan, int len,
-                                                     int c0, int c1, int c2, int c3)
+__device__ __forceinline__ void laswp_apply_body(typename CT<T>::t *A, long long lda, const int *plan, int len, int c0,
+                                                 int c1, int c2, int c3, int bid)
 {
     using V = typename CT<T>::t;
     constexpr int CW = laswp_cols<T>(), RL = 256 / CW;
     extern __shared__ char smem_raw[];
     V *buf = (V *)smem_raw;   // [2 len][CW]
     const int tid = threadIdx.x, cl = tid % CW, rl = tid / CW;
-    const int t = blockIdx.x * CW + cl, w1 = c1 - c0;
+    const int t = bid * CW + cl, w1 = c1 - c0;
     int col = -1;
     if (t < w1) col = c0 + t;
     else if (t < w1 + (c3 - c2)) col = c2 + (t - w1);
@@ -780,6 +800,13 @@ __global__ __launch_bounds__(256) void k_laswp_apply(typename CT<T>::t *A, long 
         const int p = pos[s], q = src[s];
         if (col >= 0 && p >= 0 && p != q) A[(long long)p * lda + col] = buf[s * CW + cl];
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp_apply(typename CT<T>::t *A, long long lda, const int *plan, int len,
+                                                     int c0, int c1, int c2, int c3)
+{
+    laswp_apply_body<T>(A, lda, plan, len, c0, c1, c2, c3, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- small triangular solve
@@ -1132,9 +1159,9 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo)
 }
 
 // A (rows x K, row-major complex, lda) -> P[4][rows][K] bf16
-__global__ void k_split_rows(const float2 *__restrict__ A, long long lda, int rows, int K, __bf16 *__restrict__ P)
+__device__ __forceinline__ void split_rows_body(const float2 *__restrict__ A, long long lda, int rows, int K,
+                                                __bf16 *__restrict__ P, long long idx)
 {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)rows * K) return;
     const long long r = idx / K, k = idx % K;
     const float2 v = A[r * lda + k];
@@ -1143,12 +1170,17 @@ __global__ void k_split_rows(const float2 *__restrict__ A, long long lda, int ro
     split_bf16(v.y, P[2 * plane + idx], P[3 * plane + idx]);
 }
 
+__global__ void k_split_rows(const float2 *__restrict__ A, long long lda, int rows, int K, __bf16 *__restrict__ P)
+{
+    split_rows_body(A, lda, rows, K, P, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // B (K x cols, row-major complex, ldb) -> P[4][cols][K] bf16 (transposed: k contiguous)
-__global__ __launch_bounds__(256) void k_split_cols(const float2 *__restrict__ B, long long ldb, int K, int cols,
-                                                    __bf16 *__restrict__ P)
+__device__ __forceinline__ void split_cols_body(const float2 *__restrict__ B, long long ldb, int K, int cols,
+                                                __bf16 *__restrict__ P, int bx, int by)
 {
     __shared__ float2 t[32][33];
-    const int c0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int c0 = bx * 32, k0 = by * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
     for (int r = ty; r < 32; r += 8) {
         const int k = k0 + r, c = c0 + tx;
@@ -1167,15 +1199,20 @@ __global__ __launch_bounds__(256) void k_split_cols(const float2 *__restrict__ B
     }
 }
 
+__global__ __launch_bounds__(256) void k_split_cols(const float2 *__restrict__ B, long long ldb, int K, int cols,
+                                                    __bf16 *__restrict__ P)
+{
+    split_cols_body(B, ldb, K, cols, P, blockIdx.x, blockIdx.y);
+}
+
 // C[M x N] -= A B from split planes PA[4][M][K], PB[4][N][K]; K a multiple of 32.
 // WM x WN waves, wave tile (16 TM) x (16 TN) complex.  Two LDS stages (160 KB for the
 // 128 x 128 tile): the global loads of step k+1 are in flight while step k's products run
 // from the other stage, one barrier per step; the first row block of the C tile is read
 // during the last step's products.
 template <int TM, int TN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *__restrict__ PA,
-                                                                 const __bf16 *__restrict__ PB, float2 *C,
-                                                                 long long ldc, int M, int N, int K)
+__device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA, const __bf16 *__restrict__ PB,
+                                                  float2 *C, long long ldc, int M, int N, int K, int tile)
 {
     constexpr int NT = 64 * WM * WN;
     constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN, BK = kSplitBK, RS = BK + kSplitPad;
@@ -1184,7 +1221,7 @@ __global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv / WN, wn = wv % WN;
     int tm, tn;
-    gemm_tile(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    gemm_tile(tile, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
     const long long m0 = (long long)tm * BM, n0 = (long long)tn * BN;
     const long long planeA = (long long)M * K, planeB = (long long)N * K;
     // 16-byte chunks (8 bf16) per chunk of K: A has 4 * BM * 4, B has 4 * BN * 4
@@ -1325,6 +1362,14 @@ __global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *
             }
 }
 
+template <int TM, int TN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN, 2) void k_cgemm_bf16x3(const __bf16 *__restrict__ PA,
+                                                                 const __bf16 *__restrict__ PB, float2 *C,
+                                                                 long long ldc, int M, int N, int K)
+{
+    cgemm_bf16x3_body<TM, TN, WM, WN>(PA, PB, C, ldc, M, N, K, blockIdx.x);
+}
+
 // ---------------------------------------------------------------- conversions
 template <typename T>
 __global__ void k_to_work(const double2 *__restrict__ src, int n, typename CT<T>::t *dst, long long lda)
@@ -1463,14 +1508,14 @@ constexpr int kTrinvParts = 4;   // lanes per column in k_trinv
 // kTrinvParts threads per column share each row's dot product (partial sums
 // combined through LDS); rows past the end of the matrix become identity rows.
 template <bool LOWER>
-__global__ __launch_bounds__(kIB * kTrinvParts) void k_trinv(const float2 *A, long long lda, int n, float2 *Dinv)
+__device__ __forceinline__ void trinv_body(const float2 *A, long long lda, int n, float2 *Dinv, int bid)
 {
     __shared__ float2 S[kIB * kIB];
     __shared__ float2 red[kTrinvParts][kIB];
     // part = t / kIB: the waves of a part read whole rows of S (consecutive columns,
     // no bank conflicts); part p sums the terms k = p mod kTrinvParts of each dot product
     const int t = threadIdx.x, col = t % kIB, part = t / kIB;
-    const int b0 = blockIdx.x * kIB, bs = min(kIB, n - b0);
+    const int b0 = bid * kIB, bs = min(kIB, n - b0);
     for (int idx = t; idx < kIB * kIB; idx += kIB * kTrinvParts) {
         const int i = idx / kIB, j = idx % kIB;
         float2 v{0, 0};
@@ -1515,8 +1560,14 @@ __global__ __launch_bounds__(kIB * kTrinvParts) void k_trinv(const float2 *A, lo
         }
         __syncthreads();
     }
-    float2 *out = Dinv + (size_t)blockIdx.x * kIB * kIB;
+    float2 *out = Dinv + (size_t)bid * kIB * kIB;
     for (int idx = t; idx < kIB * kIB; idx += kIB * kTrinvParts) out[idx] = S[idx];
+}
+
+template <bool LOWER>
+__global__ __launch_bounds__(kIB * kTrinvParts) void k_trinv(const float2 *A, long long lda, int n, float2 *Dinv)
+{
+    trinv_body<LOWER>(A, lda, n, Dinv, blockIdx.x);
 }
 
 // out[i] = (SUB ? out[i] : 0) - / + sum_c Mx[i][c] xin[c] for rows i < rs, columns
@@ -1593,16 +1644,16 @@ __device__ __forceinline__ void blk_matvec(const float2 *Mx, long long ld, int r
 // (i, y) forms rows [16 y, 16 y + 16) of G[i]; thread t: row t / 16, columns
 // 8 (t % 16) .. + 8.
 template <bool LOWER>
-__global__ __launch_bounds__(256) void k_blk_coupling(const float2 *A, long long lda, int n, const float2 *Dinv,
-                                                      float2 *G)
+__device__ __forceinline__ void blk_coupling_body(const float2 *A, long long lda, int n, const float2 *Dinv, float2 *G,
+                                                  int bx, int by)
 {
     __shared__ float2 Ds[16][kIB];
     __shared__ float2 Ms[32][kIB];
     const int nb = (n + kIB - 1) / kIB;
-    const int i = blockIdx.x, j = LOWER ? i - 1 : i + 1;
+    const int i = bx, j = LOWER ? i - 1 : i + 1;
     if (j < 0 || j >= nb) return;   // uniform
     const int r0 = i * kIB, rs = min(kIB, n - r0), c0 = j * kIB, cs = min(kIB, n - c0);
-    const int tid = threadIdx.x, rr = tid >> 4, cg = (tid & 15) * 8, y0 = blockIdx.y * 16;
+    const int tid = threadIdx.x, rr = tid >> 4, cg = (tid & 15) * 8, y0 = by * 16;
     for (int idx = tid; idx < 16 * kIB; idx += 256) {
         const int r = idx / kIB, k = idx % kIB;
         Ds[r][k] = (y0 + r < rs && k < rs) ? Dinv[(size_t)i * kIB * kIB + (size_t)(y0 + r) * kIB + k] : float2{0, 0};
@@ -1630,6 +1681,13 @@ __global__ __launch_bounds__(256) void k_blk_coupling(const float2 *A, long long
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) G[(size_t)i * kIB * kIB + (size_t)(y0 + rr) * kIB + cg + c] = acc[c];
+}
+
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_blk_coupling(const float2 *A, long long lda, int n, const float2 *Dinv,
+                                                      float2 *G)
+{
+    blk_coupling_body<LOWER>(A, lda, n, Dinv, G, blockIdx.x, blockIdx.y);
 }
 
 // One launch per triangle.  Workgroups take their block in the order they
@@ -2448,9 +2506,470 @@ void lu_solve_inv(hipStream_t s, const float2 *A, long long lda, int n, const fl
 
 long long padded_ld(int64_t n) { return (n + 63) / 64 * 64; }
 
+// ---------------------------------------------------------------- batched FP32 factorisation
+// Several matrices factored in shared launches: every launch of the single-matrix
+// schedule (512-deep block columns: tournament panels, their apply, interchanges, block
+// rows, split-bf16 trailing updates; then the diagonal-block inverses of the solves) is
+// issued once for all matrices that have it at that step.  A workgroup finds its matrix in
+// the table of each matrix's first workgroup and runs the single-matrix kernel's body on
+// that matrix's arguments, so each matrix gets the factors (and pivots) it gets alone.
+struct PanelBA {
+    float2 *A;
+    long long lda;
+    int n, c, pw, nsets0, prev, upd_n;
+    TsluWork tw;
+    TsluFinal fin;
+};
+struct ApplyBA {
+    float2 *A;
+    long long lda;
+    int n, c, pw;
+    const float2 *top, *oldtop;
+    const int *plan;
+    int a0, a1, b0, b1, nsw;
+};
+struct SwapBA {
+    const long long *ipiv;
+    int k1, k2;
+    int *plan;
+};
+struct LaswpBA {
+    float2 *A;
+    long long lda;
+    const int *plan;
+    int len, c0, c1, c2, c3;
+};
+struct TrinvBA {
+    const float2 *A;
+    long long lda;
+    int n;
+    float2 *Dinv;
+};
+struct ToEBA {
+    float2 *E;
+    int nblk;
+};
+struct CgemmBA {
+    const float2 *A;
+    long long lda;
+    const float2 *B;
+    long long ldb;
+    float2 *C;
+    long long ldc;
+    int M, N, K;
+};
+struct SplitRowsBA {
+    const float2 *A;
+    long long lda;
+    int rows, K;
+    __bf16 *P;
+};
+struct SplitColsBA {
+    const float2 *B;
+    long long ldb;
+    int K, cols, gx;
+    __bf16 *P;
+};
+struct Bf16BA {
+    const __bf16 *PA, *PB;
+    float2 *C;
+    long long ldc;
+    int M, N, K;
+};
+struct CouplingBA {
+    const float2 *A;
+    long long lda;
+    int n, gx;
+    const float2 *Dinv;
+    float2 *G;
+};
+
+#define BH_BATCH_ITEM(ARGS)                              \
+    const int r_ = batch_find(first, count, blockIdx.x); \
+    const auto &a = ARGS[r_];                            \
+    const int bid = (int)blockIdx.x - first[r_];
+
+__global__ __launch_bounds__(kTsluT, 2) void k_tslu_panel_batch(const PanelBA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    tslu_panel_body(a.A, a.lda, a.n, a.c, a.pw, a.nsets0, a.prev, a.upd_n, a.tw, a.fin, bid);
+}
+
+__global__ __launch_bounds__(256) void k_tslu_apply_batch(const ApplyBA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    tslu_apply_body(a.A, a.lda, a.n, a.c, a.pw, a.top, a.oldtop, a.plan, a.a0, a.a1, a.b0, a.b1, a.nsw, bid);
+}
+
+__global__ __launch_bounds__(256) void k_swap_plan_batch(const SwapBA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    (void)bid;
+    swap_plan_body(a.ipiv, a.k1, a.k2, a.plan);
+}
+
+__global__ __launch_bounds__(256) void k_laswp_apply_batch(const LaswpBA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    laswp_apply_body<float>(a.A, a.lda, a.plan, a.len, a.c0, a.c1, a.c2, a.c3, bid);
+}
+
+template <bool LOWER>
+__global__ __launch_bounds__(kIB * kTrinvParts) void k_trinv_batch(const TrinvBA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    trinv_body<LOWER>(a.A, a.lda, a.n, a.Dinv, bid);
+}
+
+__global__ __launch_bounds__(256) void k_inv_to_e_batch(const ToEBA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    const long long idx = (long long)bid * 256 + threadIdx.x;
+    if (idx >= (long long)a.nblk * kIB * kIB) return;
+    const int i = (int)(idx / kIB % kIB), j = (int)(idx % kIB);
+    const float2 v = a.E[idx];
+    a.E[idx] = i == j ? float2{0, 0} : float2{-v.x, -v.y};
+}
+
+__global__ __launch_bounds__(256, 2) void k_cgemm_batch(const CgemmBA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    cgemm_tile<float, 4, 2, true>(bid, a.A, a.lda, a.B, a.ldb, a.C, a.ldc, a.M, a.N, a.K);
+}
+
+__global__ __launch_bounds__(256) void k_split_rows_batch(const SplitRowsBA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    split_rows_body(a.A, a.lda, a.rows, a.K, a.P, (long long)bid * 256 + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_split_cols_batch(const SplitColsBA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    split_cols_body(a.B, a.ldb, a.K, a.cols, a.P, bid % a.gx, bid / a.gx);
+}
+
+__global__ __launch_bounds__(512, 2) void k_cgemm_bf16x3_batch(const Bf16BA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    cgemm_bf16x3_body<2, 4, 4, 2>(a.PA, a.PB, a.C, a.ldc, a.M, a.N, a.K, bid);
+}
+
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_blk_coupling_batch(const CouplingBA *args, const int *first, int count)
+{
+    BH_BATCH_ITEM(args)
+    blk_coupling_body<LOWER>(a.A, a.lda, a.n, a.Dinv, a.G, bid % a.gx, bid / a.gx);
+}
+
+#undef BH_BATCH_ITEM
+
+// Largest matrix the batched schedule takes: every block column 512 deep (the single
+// schedule's deeper blocks start above this trailing size).
+constexpr int kBatchMaxN = kNBDeepAbove;
+
+namespace {
+
+enum BatchKind : int {
+    kBkPanel, kBkApply, kBkSwap, kBkLaswp, kBkTrinvL, kBkTrinvU, kBkToE, kBkCgemm, kBkSplitRows, kBkSplitCols,
+    kBkBf16, kBkCoupleL, kBkCoupleU
+};
+
+struct BatchLaunch {
+    int kind;
+    int count;            // matrices in this launch
+    int blocks;           // total workgroups
+    size_t args, first;   // offsets into the uploaded table
+    size_t shm;
+    double work;          // for the profiler: the single-matrix kernels' work units, summed
+};
+
+// Host-side schedule: launches in order and one table of their arguments.
+struct BatchPlan {
+    std::vector<char> blob;
+    std::vector<BatchLaunch> launches;
+    template <typename S>
+    size_t put(const std::vector<S> &v) {
+        size_t off = (blob.size() + 15) & ~(size_t)15;
+        blob.resize(off + v.size() * sizeof(S));
+        std::memcpy(blob.data() + off, v.data(), v.size() * sizeof(S));
+        return off;
+    }
+    // items: (args, blocks) per matrix; matrices with no blocks are left out
+    template <typename S>
+    void add(int kind, const std::vector<std::pair<S, long long>> &items, size_t shm = 0, double work = 0) {
+        std::vector<S> a;
+        std::vector<int> first{0};
+        for (const auto &it : items) {
+            if (it.second <= 0) continue;
+            BH_REQUIRE((long long)first.back() + it.second < (1LL << 31), "batched launch too large");
+            a.push_back(it.first);
+            first.push_back(first.back() + (int)it.second);
+        }
+        if (a.empty()) return;
+        BatchLaunch L{kind, (int)a.size(), first.back(), put(a), 0, shm, work};
+        L.first = put(first);
+        launches.push_back(L);
+    }
+};
+
+struct BatchMat {
+    bh_matrix *M;
+    int n;
+    long long lda;
+    float2 *A;
+    long long *ipiv;
+    // workspace (device)
+    int *tcnt, *tidx, *tplan, *plan, *info;
+    float2 *tval, *top, *oldtop, *eblk, *dinv;
+    __bf16 *pa, *pb;
+};
+
+void plan_factor(BatchPlan &P, std::vector<BatchMat> &mats)
+{
+    int nmax = 0;
+    for (auto &m : mats) nmax = std::max(nmax, m.n);
+    for (int k = 0; k < nmax; k += kNBInner) {
+        // the block column [k, k + kb) of every matrix that has it: its panels
+        for (int sp = 0; sp < kNBInner; sp += kPW) {
+            std::vector<std::pair<PanelBA, long long>> pan;
+            std::vector<std::pair<ApplyBA, long long>> app;
+            double wp = 0, wa = 0;
+            for (auto &m : mats) {
+                if (m.n <= k) continue;
+                const int kb = std::min(kNBInner, m.n - k);
+                if (sp >= kb) continue;
+                const int c = k + sp, pw = std::min(kPW, kb - sp), prev = sp > 0;
+                const int nsets0 = (m.n - c + kTsluT - 1) / kTsluT;
+                const int upd_n = prev ? std::max(0, k + kb - (c + kPW)) : 0;
+                const long long tiles = upd_n > 0 ? (long long)((m.n - c + 127) / 128) * ((upd_n + 63) / 64) : 0;
+                PanelBA a{m.A, m.lda, m.n, c, pw, nsets0, prev, upd_n, TsluWork{m.tcnt, m.tidx, m.tval},
+                          TsluFinal{m.ipiv, m.top, m.oldtop, m.tplan, m.info}};
+                pan.push_back({a, nsets0 + tiles});
+                wp += (double)(m.n - c) * pw * sizeof(float2) * (prev ? 3 : 1);
+                const int w = (c - k) + (k + kb - (c + pw));
+                const int nsw = (w + 15) / 16;
+                ApplyBA b{m.A, m.lda, m.n, c, pw, m.top, m.oldtop, m.tplan, k, c, c + pw, k + kb, nsw};
+                app.push_back({b, nsw + (m.n - c + 255) / 256});
+                wa += (double)(m.n - c) * pw * sizeof(float2) * 2;
+            }
+            if (pan.empty()) break;
+            P.add(kBkPanel, pan, 0, wp);
+            P.add(kBkApply, app, 0, wa);
+        }
+        // interchanges of the block reach the columns left of it, and right of it when
+        // it is not the last block (pieces of kMaxSwaps, in order)
+        for (int q0 = 0; q0 < kNBInner; q0 += kMaxSwaps) {
+            std::vector<std::pair<SwapBA, long long>> sw;
+            std::vector<std::pair<LaswpBA, long long>> la;
+            int lmax = 0;
+            double wl = 0;
+            for (auto &m : mats) {
+                if (m.n <= k) continue;
+                const int kb = std::min(kNBInner, m.n - k), nk = k + kb;
+                const int q = k + q0;
+                if (q >= nk) continue;
+                const int len = std::min(kMaxSwaps, nk - q);
+                const int c2 = nk < m.n ? nk : 0, c3 = nk < m.n ? m.n : 0;
+                const int w = k + (c3 - c2);
+                if (w <= 0) continue;
+                sw.push_back({SwapBA{m.ipiv, q, q + len, m.plan}, 1});
+                la.push_back({LaswpBA{m.A, m.lda, m.plan, len, 0, k, c2, c3}, (w + 15) / 16});
+                lmax = std::max(lmax, len);
+                wl += 2.0 * 2 * len * w * sizeof(float2);
+            }
+            P.add(kBkSwap, sw);
+            P.add(kBkLaswp, la, (size_t)2 * lmax * 16 * sizeof(float2), wl);
+        }
+        // block row U12 = L11^-1 A12 and the trailing update, for matrices with columns right
+        // of the block: inverses of L11's 128-row diagonal blocks, then per 128-row block
+        // one product with I - inv(L_bb) and one K = 128 update of the rows below
+        std::vector<BatchMat *> rest;
+        for (auto &m : mats)
+            if (m.n > k + kNBInner) rest.push_back(&m);
+        if (rest.empty()) continue;
+        const int nk = k + kNBInner, nbb = kNBInner / kIB;
+        {
+            std::vector<std::pair<TrinvBA, long long>> ti;
+            std::vector<std::pair<ToEBA, long long>> te;
+            for (auto *m : rest) {
+                ti.push_back({TrinvBA{m->A + (long long)k * m->lda + k, m->lda, kNBInner, m->eblk}, nbb});
+                te.push_back({ToEBA{m->eblk, nbb}, ((long long)nbb * kIB * kIB + 255) / 256});
+            }
+            P.add(kBkTrinvL, ti, 0, 4.0 * kIB * (double)kIB * kIB * nbb / 3.0 * rest.size());
+            P.add(kBkToE, te);
+        }
+        for (int b = 0; b < nbb; ++b) {
+            const int r = k + b * kIB;
+            std::vector<std::pair<CgemmBA, long long>> ap, up;
+            double wap = 0, wup = 0;
+            for (auto *m : rest) {
+                const int N = m->n - nk;
+                float2 *C = m->A + (long long)r * m->lda + nk;
+                ap.push_back({CgemmBA{m->eblk + (size_t)b * kIB * kIB, kIB, C, m->lda, C, m->lda, kIB, N, kIB},
+                              (N + 63) / 64});
+                wap += 8.0 * kIB * (double)N * kIB;
+                if (b + 1 < nbb) {
+                    const int Mr = nk - (r + kIB);
+                    up.push_back({CgemmBA{m->A + (long long)(r + kIB) * m->lda + r, m->lda, C, m->lda,
+                                          m->A + (long long)(r + kIB) * m->lda + nk, m->lda, Mr, N, kIB},
+                                  (long long)((N + 63) / 64) * ((Mr + 127) / 128)});
+                    wup += 8.0 * Mr * (double)N * kIB;
+                }
+            }
+            P.add(kBkCgemm, ap, 0, wap);
+            P.add(kBkCgemm, up, 0, wup);
+        }
+        {
+            std::vector<std::pair<SplitRowsBA, long long>> sr;
+            std::vector<std::pair<SplitColsBA, long long>> sc;
+            std::vector<std::pair<Bf16BA, long long>> gm;
+            double wsplit = 0, wg = 0;
+            for (auto *m : rest) {
+                const int Mt = m->n - nk, Nt = m->n - nk, K = kNBInner;
+                sr.push_back({SplitRowsBA{m->A + (long long)nk * m->lda + k, m->lda, Mt, K, m->pa},
+                              ((long long)Mt * K + 255) / 256});
+                const int gx = (Nt + 31) / 32;
+                sc.push_back({SplitColsBA{m->A + (long long)k * m->lda + nk, m->lda, K, Nt, gx, m->pb},
+                              (long long)gx * ((K + 31) / 32)});
+                gm.push_back({Bf16BA{m->pa, m->pb, m->A + (long long)nk * m->lda + nk, m->lda, Mt, Nt, K},
+                              (long long)((Nt + 127) / 128) * ((Mt + 127) / 128)});
+                wsplit += 8.0 * ((double)Mt + Nt) * K * 2.0;
+                wg += 24.0 * Mt * (double)Nt * K;
+            }
+            P.add(kBkSplitRows, sr, 0, wsplit);
+            P.add(kBkSplitCols, sc);
+            P.add(kBkBf16, gm, 0, wg);
+        }
+    }
+    // the solves' diagonal-block inverses and couplings
+    std::vector<std::pair<TrinvBA, long long>> tl, tu;
+    std::vector<std::pair<CouplingBA, long long>> cl, cu;
+    for (auto &m : mats) {
+        const int nb = (m.n + kIB - 1) / kIB;
+        const size_t bsz = (size_t)nb * kIB * kIB;
+        tl.push_back({TrinvBA{m.A, m.lda, m.n, m.dinv}, nb});
+        tu.push_back({TrinvBA{m.A, m.lda, m.n, m.dinv + bsz}, nb});
+        cl.push_back({CouplingBA{m.A, m.lda, m.n, nb, m.dinv, m.dinv + 2 * bsz}, (long long)nb * (kIB / 16)});
+        cu.push_back({CouplingBA{m.A, m.lda, m.n, nb, m.dinv + bsz, m.dinv + 3 * bsz}, (long long)nb * (kIB / 16)});
+    }
+    P.add(kBkTrinvL, tl);
+    P.add(kBkTrinvU, tu);
+    P.add(kBkCoupleL, cl);
+    P.add(kBkCoupleU, cu);
+}
+
+const char *batch_kernel_name(int kind)
+{
+    switch (kind) {
+    case kBkPanel: return "k_tslu_panel";
+    case kBkApply: return "k_tslu_apply";
+    case kBkSwap: return "k_swap_plan";
+    case kBkLaswp: return "k_laswp";
+    case kBkTrinvL:
+    case kBkTrinvU: return "k_trinv";
+    case kBkToE: return "k_inv_to_e";
+    case kBkCgemm: return "k_cgemm";
+    case kBkSplitRows:
+    case kBkSplitCols: return "k_split";
+    case kBkBf16: return "k_cgemm_bf16x3";
+    default: return "k_blk_coupling";
+    }
+}
+
+void run_plan(Ctx &ctx, const BatchPlan &P, const char *dtab)
+{
+    hipStream_t s = ctx.stream;
+    for (const auto &L : P.launches) {
+        const unsigned g = (unsigned)L.blocks;
+        const int *first = (const int *)(dtab + L.first);
+        const void *a = dtab + L.args;
+        ProfScope prof(ctx, batch_kernel_name(L.kind), L.work, s);
+        switch (L.kind) {
+        case kBkPanel: k_tslu_panel_batch<<<g, kTsluT, 0, s>>>((const PanelBA *)a, first, L.count); break;
+        case kBkApply: k_tslu_apply_batch<<<g, 256, 0, s>>>((const ApplyBA *)a, first, L.count); break;
+        case kBkSwap: k_swap_plan_batch<<<g, 256, 0, s>>>((const SwapBA *)a, first, L.count); break;
+        case kBkLaswp: k_laswp_apply_batch<<<g, 256, L.shm, s>>>((const LaswpBA *)a, first, L.count); break;
+        case kBkTrinvL:
+            k_trinv_batch<true><<<g, kIB * kTrinvParts, 0, s>>>((const TrinvBA *)a, first, L.count);
+            break;
+        case kBkTrinvU:
+            k_trinv_batch<false><<<g, kIB * kTrinvParts, 0, s>>>((const TrinvBA *)a, first, L.count);
+            break;
+        case kBkToE: k_inv_to_e_batch<<<g, 256, 0, s>>>((const ToEBA *)a, first, L.count); break;
+        case kBkCgemm: k_cgemm_batch<<<g, 256, 0, s>>>((const CgemmBA *)a, first, L.count); break;
+        case kBkSplitRows: k_split_rows_batch<<<g, 256, 0, s>>>((const SplitRowsBA *)a, first, L.count); break;
+        case kBkSplitCols: k_split_cols_batch<<<g, 256, 0, s>>>((const SplitColsBA *)a, first, L.count); break;
+        case kBkBf16: k_cgemm_bf16x3_batch<<<g, 512, 0, s>>>((const Bf16BA *)a, first, L.count); break;
+        case kBkCoupleL:
+            k_blk_coupling_batch<true><<<g, 256, 0, s>>>((const CouplingBA *)a, first, L.count);
+            break;
+        case kBkCoupleU:
+            k_blk_coupling_batch<false><<<g, 256, 0, s>>>((const CouplingBA *)a, first, L.count);
+            break;
+        default: BH_FAIL(BH_ERR_INVALID, "unknown batched launch %d", L.kind);
+        }
+        check(batch_kernel_name(L.kind));
+    }
+}
+
+}  // namespace
+
 }  // namespace bh
 
 using namespace bh;
+
+namespace {
+
+// FP32 factorisation inputs: the matrix rounded to FP32 in lu32 (unless the assembly wrote
+// it there) and the buffer of the solves' diagonal-block inverses.
+void fp32_prepare(Ctx &ctx, bh_matrix *M, long long lda)
+{
+    const int64_t n = M->n;
+    pool_release(ctx, M->lu64);
+    if (M->lu32.n != (size_t)lda * n * 2) {
+        pool_alloc(ctx, M->lu32, (size_t)lda * n * 2);
+        M->lu32_fresh = false;
+    }
+    if (!M->lu32_fresh) {
+        const unsigned cgrid = (unsigned)(((long long)n * n + 255) / 256);
+        k_to_work<float><<<cgrid, 256, 0, ctx.stream>>>((const double2 *)M->a64.p, (int)n, (float2 *)M->lu32.p, lda);
+        check("k_to_work");
+    }
+    M->lu32_fresh = false;   // it now holds the factors
+    const size_t nb = (size_t)(n + kIB - 1) / kIB;
+    pool_alloc(ctx, M->dinv32, 4 * nb * kIB * kIB * 2);   // L and U diagonal inverses, then their couplings
+}
+
+// After a factorisation: the FP32 solves' vector and block flags (so that solving
+// allocates nothing), the permutation (P b)[i] = b[perm[i]] from the sequential
+// interchanges, and the matrix state.
+void factor_finish(Ctx &ctx, bh_matrix *M, long long lda, int precision, int info)
+{
+    const int64_t n = M->n;
+    if (precision != BH_FP64) {
+        if (M->x32.n < (size_t)n * kMaxRhs * 2) M->x32.alloc((size_t)n * kMaxRhs * 2);
+        const size_t nflags = (size_t)((n + kIB - 1) / kIB) + 2;
+        if (M->tsync.n < nflags) {
+            M->tsync.alloc(nflags);
+            k_fill_int<<<(unsigned)((nflags + 255) / 256), 256, 0, ctx.stream>>>(M->tsync.p, (long long)nflags);
+            check("k_fill_int");
+            M->tsync_tickets = 0;
+            M->tsync_epoch = 0;
+        }
+    }
+    std::vector<long long> ip(n);
+    BH_CHECK_HIP(hipMemcpy(ip.data(), M->ipiv.p, sizeof(long long) * n, hipMemcpyDeviceToHost));
+    std::vector<int> perm(n);
+    for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
+    for (int64_t i = 0; i < n; ++i) std::swap(perm[i], perm[ip[i]]);
+    M->perm.upload(perm.data(), n, ctx.stream);
+    BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
+    M->lda = lda;
+    M->factored = info ? -1 : (precision == BH_FP64 ? BH_FP64 : BH_FP32);
+}
+
+}  // namespace
 
 extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
 {
@@ -2468,42 +2987,20 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
     BH_CHECK_HIP(hipEventCreate(&e0));
     BH_CHECK_HIP(hipEventCreate(&e1));
     int info = 0;
-    const unsigned cgrid = (unsigned)(((long long)n * n + 255) / 256);
     BH_CHECK_HIP(hipEventRecord(e0, ctx.stream));
     if (precision == BH_FP64) {
         pool_release(ctx, M->lu32);
         M->lu32_fresh = false;
         pool_release(ctx, M->dinv32);
         pool_alloc(ctx, M->lu64, (size_t)lda * n * 2);
+        const unsigned cgrid = (unsigned)(((long long)n * n + 255) / 256);
         k_to_work<double><<<cgrid, 256, 0, ctx.stream>>>((const double2 *)M->a64.p, (int)n, (double2 *)M->lu64.p, lda);
         check("k_to_work");
         lu_factor<double>(ctx, (double2 *)M->lu64.p, lda, (int)n, M->ipiv.p, &info);
     } else {
-        pool_release(ctx, M->lu64);
-        if (M->lu32.n != (size_t)lda * n * 2) {
-            pool_alloc(ctx, M->lu32, (size_t)lda * n * 2);
-            M->lu32_fresh = false;
-        }
-        if (!M->lu32_fresh) {   // else the assembly wrote the FP32 copy
-            k_to_work<float><<<cgrid, 256, 0, ctx.stream>>>((const double2 *)M->a64.p, (int)n, (float2 *)M->lu32.p,
-                                                            lda);
-            check("k_to_work");
-        }
-        M->lu32_fresh = false;   // it now holds the factors
-        const size_t nb = (size_t)(n + kIB - 1) / kIB;
-        pool_alloc(ctx, M->dinv32, 4 * nb * kIB * kIB * 2);   // L and U diagonal inverses, then their couplings
+        fp32_prepare(ctx, M, lda);
         lu_factor<float>(ctx, (float2 *)M->lu32.p, lda, (int)n, M->ipiv.p, &info, precision == BH_FP32_SPLIT,
                          (float2 *)M->dinv32.p);
-        // the solves' FP32 vector and their block flags, here so that solving allocates nothing
-        if (M->x32.n < (size_t)n * kMaxRhs * 2) M->x32.alloc((size_t)n * kMaxRhs * 2);
-        const size_t nflags = (size_t)((n + kIB - 1) / kIB) + 2;
-        if (M->tsync.n < nflags) {
-            M->tsync.alloc(nflags);
-            k_fill_int<<<(unsigned)((nflags + 255) / 256), 256, 0, ctx.stream>>>(M->tsync.p, (long long)nflags);
-            check("k_fill_int");
-            M->tsync_tickets = 0;
-            M->tsync_epoch = 0;
-        }
     }
     BH_CHECK_HIP(hipEventRecord(e1, ctx.stream));
     BH_CHECK_HIP(hipEventSynchronize(e1));
@@ -2512,17 +3009,131 @@ extern "C" int bh_matrix_factor(bh_matrix *M, int precision)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     M->seconds_factor = ms * 1e-3;
-    // permutation: (P b)[i] = b[perm[i]], from the sequential interchanges
-    std::vector<long long> ip(n);
-    BH_CHECK_HIP(hipMemcpy(ip.data(), M->ipiv.p, sizeof(long long) * n, hipMemcpyDeviceToHost));
-    std::vector<int> perm(n);
-    for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
-    for (int64_t i = 0; i < n; ++i) std::swap(perm[i], perm[ip[i]]);
-    M->perm.upload(perm.data(), n, ctx.stream);
-    BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
-    M->lda = lda;
-    M->factored = info ? -1 : (precision == BH_FP64 ? BH_FP64 : BH_FP32);
+    factor_finish(ctx, M, lda, precision, info);
     if (info) BH_FAIL(BH_ERR_SINGULAR, "SingularException(%d): exactly zero pivot", info);
+    BH_API_END
+}
+
+extern "C" int bh_matrix_factor_batch(bh_ctx *ctxh, int64_t count, bh_matrix *const *mats, int precision)
+{
+    BH_API_BEGIN
+    BH_REQUIRE(ctxh != nullptr && (count == 0 || mats != nullptr), "NULL argument");
+    BH_REQUIRE(count >= 0 && count <= (1 << 20), "bad batch size %lld", (long long)count);
+    BH_REQUIRE(precision == BH_FP64 || precision == BH_FP32 || precision == BH_FP32_SPLIT,
+               "precision must be BH_FP64, BH_FP32 or BH_FP32_SPLIT");
+    Ctx &ctx = *ctxh;
+    for (int64_t i = 0; i < count; ++i) {
+        BH_REQUIRE(mats[i] != nullptr, "NULL matrix %lld", (long long)i);
+        BH_REQUIRE(mats[i]->ctx == &ctx, "matrix %lld belongs to another context", (long long)i);
+        BH_REQUIRE(mats[i]->n > 0 && mats[i]->n < (int64_t(1) << 31), "bad matrix size");
+        for (int64_t j = 0; j < i; ++j) BH_REQUIRE(mats[j] != mats[i], "matrix %lld appears twice", (long long)i);
+    }
+    // the shared schedule takes FP32 split-bf16 factorisations of up to kBatchMaxN rows;
+    // the others are factored one by one, as bh_matrix_factor does
+    std::vector<bh_matrix *> shared, alone;
+    for (int64_t i = 0; i < count; ++i)
+        (precision == BH_FP32_SPLIT && mats[i]->n <= kBatchMaxN ? shared : alone).push_back(mats[i]);
+    int first_error = BH_OK;
+    std::string first_msg;
+    for (bh_matrix *M : alone) {
+        const int rc = bh_matrix_factor(M, precision);
+        if (rc != BH_OK && first_error == BH_OK) {
+            first_error = rc;
+            first_msg = bh_last_error();
+        }
+    }
+    if (!shared.empty()) {
+        std::vector<std::unique_lock<std::mutex>> locks;
+        for (bh_matrix *M : shared) locks.emplace_back(M->mu);
+        // workspace of each matrix, one allocation: counters and info first (cleared together)
+        std::vector<BatchMat> bm(shared.size());
+        size_t ints = 0, rest = 0;
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        std::vector<size_t> sets(shared.size());
+        for (size_t i = 0; i < shared.size(); ++i) {
+            const int n = (int)shared[i]->n;
+            sets[i] = 2 * ((size_t)(n + kTsluT - 1) / kTsluT) + 8;
+            ints += al((sets[i] + 1) * sizeof(int));
+            const size_t trail = n > kNBInner ? (size_t)(n - kNBInner) : 0;
+            rest += al(sets[i] * kPW * sizeof(int)) + al((1 + 2 * kPlanMax) * sizeof(int)) +
+                    al(4 * kMaxSwaps * sizeof(int)) + al(sets[i] * kPW * kPW * sizeof(float2)) +
+                    2 * al(kPW * kPW * sizeof(float2)) + al((size_t)(kNBInner / kIB) * kIB * kIB * sizeof(float2)) +
+                    2 * al(4 * trail * kNBInner * sizeof(__bf16));
+        }
+        DevBuf<char> ws;
+        ws.alloc(ints + rest);
+        k_fill_int<<<(unsigned)((ints / sizeof(int) + 255) / 256), 256, 0, ctx.stream>>>((int *)ws.p,
+                                                                                        (long long)(ints / sizeof(int)));
+        check("k_fill_int");
+        size_t oi = 0, orest = ints;
+        auto take = [&](size_t &off, size_t bytes) {
+            char *p = ws.p + off;
+            off += al(bytes);
+            return p;
+        };
+        hipEvent_t e0, e1;
+        BH_CHECK_HIP(hipEventCreate(&e0));
+        BH_CHECK_HIP(hipEventCreate(&e1));
+        BH_CHECK_HIP(hipEventRecord(e0, ctx.stream));
+        for (size_t i = 0; i < shared.size(); ++i) {
+            bh_matrix *M = shared[i];
+            const int n = (int)M->n;
+            const long long lda = padded_ld(n);
+            M->ipiv.alloc(n);
+            fp32_prepare(ctx, M, lda);
+            BatchMat &b = bm[i];
+            b.M = M;
+            b.n = n;
+            b.lda = lda;
+            b.A = (float2 *)M->lu32.p;
+            b.ipiv = M->ipiv.p;
+            b.dinv = (float2 *)M->dinv32.p;
+            b.tcnt = (int *)take(oi, (sets[i] + 1) * sizeof(int));
+            b.info = b.tcnt + sets[i];
+            const size_t trail = n > kNBInner ? (size_t)(n - kNBInner) : 0;
+            b.tidx = (int *)take(orest, sets[i] * kPW * sizeof(int));
+            b.tplan = (int *)take(orest, (1 + 2 * kPlanMax) * sizeof(int));
+            b.plan = (int *)take(orest, 4 * kMaxSwaps * sizeof(int));
+            b.tval = (float2 *)take(orest, sets[i] * kPW * kPW * sizeof(float2));
+            b.top = (float2 *)take(orest, kPW * kPW * sizeof(float2));
+            b.oldtop = (float2 *)take(orest, kPW * kPW * sizeof(float2));
+            b.eblk = (float2 *)take(orest, (size_t)(kNBInner / kIB) * kIB * kIB * sizeof(float2));
+            b.pa = (__bf16 *)take(orest, 4 * trail * kNBInner * sizeof(__bf16));
+            b.pb = (__bf16 *)take(orest, 4 * trail * kNBInner * sizeof(__bf16));
+        }
+        BatchPlan plan;
+        plan_factor(plan, bm);
+        DevBuf<char> tab;
+        tab.alloc(std::max<size_t>(plan.blob.size(), 16));
+        struct SyncOnExit {   // no workspace is freed under running kernels, also on errors
+            hipStream_t s;
+            ~SyncOnExit() { (void)hipStreamSynchronize(s); }
+        } sync_on_exit{ctx.stream};
+        BH_CHECK_HIP(hipMemcpyAsync(tab.p, plan.blob.data(), plan.blob.size(), hipMemcpyHostToDevice, ctx.stream));
+        run_plan(ctx, plan, tab.p);
+        BH_CHECK_HIP(hipEventRecord(e1, ctx.stream));
+        std::vector<int> infos(shared.size());
+        for (size_t i = 0; i < shared.size(); ++i)
+            BH_CHECK_HIP(hipMemcpyAsync(&infos[i], bm[i].info, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
+        BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
+        float ms = 0;
+        BH_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        for (size_t i = 0; i < shared.size(); ++i) {
+            bh_matrix *M = shared[i];
+            M->seconds_factor = ms * 1e-3;   // the whole batch's factorisation time
+            factor_finish(ctx, M, bm[i].lda, precision, infos[i]);
+            if (infos[i] && first_error == BH_OK) {
+                first_error = BH_ERR_SINGULAR;
+                char buf[160];
+                std::snprintf(buf, sizeof buf, "SingularException(%d): exactly zero pivot (batch matrix of n = %lld)",
+                              infos[i], (long long)M->n);
+                first_msg = buf;
+            }
+        }
+    }
+    if (first_error != BH_OK) BH_FAIL(first_error, "%s", first_msg.c_str());
     BH_API_END
 }
 

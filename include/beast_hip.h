@@ -255,6 +255,15 @@ int bh_matrix_download(bh_matrix *M, double *A, int64_t ld);
  * factorisation's workspace (about 1 GB at N = 40k) stays allocated in the context
  * for the next factorisation until the context is destroyed. */
 int bh_matrix_factor(bh_matrix *M, int precision);
+/* Several resident matrices (all of ctx) factored in shared launches (no counterpart in the
+ * reference, which factors one matrix per `\` or LU call, src/solvers/lusolver.jl:26-34):
+ * with BH_FP32_SPLIT, the matrices of up to 24,576 rows follow the schedule of
+ * bh_matrix_factor with every launch issued once for all matrices that have it at that
+ * step, so a batch of small matrices fills the GPU; each gets the factors and pivots
+ * bh_matrix_factor gives it alone.  Larger matrices and the other precisions are factored
+ * one by one.  Each matrix records the batch's factorisation time.  A singular matrix is
+ * left unfactored and reported (BH_ERR_SINGULAR) after the others are done. */
+int bh_matrix_factor_batch(bh_ctx *ctx, int64_t count, bh_matrix *const *mats, int precision);
 /* Solve A X = B for nrhs column-major right-hand sides (ld = n).  With FP64
  * factors: getrs.  With FP32 factors: FP64-residual iterative refinement with
  * GMRES (GMRES-IR), stopped at ||b - A x|| / ||b|| <= tol or maxit outer steps. */
