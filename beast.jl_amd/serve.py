@@ -26,6 +26,7 @@ from collections import OrderedDict
 import numpy as np
 
 from . import assembly as asm
+from ._native import BHError
 from .mesh import meshcube, meshellipsoid, meshsphere
 from .operators import Identity, Maxwell3D, cross, curl, n as nvec
 from .rt import raviartthomas
@@ -115,7 +116,9 @@ class Server:
     whose matrix is resident (cache hits, solved at once) and new requests admitted against
     the memory budget, up to ``max_batch`` of them and ``batch_dofs2`` summed N^2.  The new
     requests' matrices are assembled together (bh_matrix_assemble_batch: shared launches
-    for their classification, near blocks and far tiles), then each is factored and solved.
+    for their classification, near blocks and far tiles) and factored together
+    (bh_matrix_factor_batch: shared launches for their panels, interchanges, block rows and
+    trailing updates), then each is solved.
     While one worker assembles a batch the other factors and solves its own, so the device
     overlaps requests that arrived at different times.
 
@@ -273,9 +276,14 @@ class Server:
                 self._release(r, None, nb)
             raise
         errors = []
+        try:
+            # shared launches for the batch's factorisations; a singular matrix is left
+            # unfactored (its solve below reports it) while the others are factored
+            asm.DeviceMatrix.factor_batch(mats, self.precision)
+        except BHError:
+            pass
         for r, M, nb in zip(builds, mats, nbytes):
             try:
-                M.factor(self.precision)
                 self._solve(r, M, ctx)
                 self._release(r, M, nb)
             except BaseException as exc:

@@ -68,6 +68,9 @@ def test_batches_share_builds_and_respect_the_budget(bj, monkeypatch):
         return [_FakeMatrix(X.numfunctions()) for _, X in requests]
 
     monkeypatch.setattr(asm.DeviceMatrix, "assemble_batch", staticmethod(fake_batch))
+    factored = []
+    monkeypatch.setattr(asm.DeviceMatrix, "factor_batch",
+                        staticmethod(lambda mats, precision: factored.append(len(mats)) or mats))
     monkeypatch.setattr(sv.Server, "_rhs", staticmethod(lambda X, kind, k, d, p, ctx: np.ones(X.numfunctions())))
     reqs = sv.poisson_requests(seed=5, rate=1000.0, count=40, nmin=300, nmax=600, repeat_prob=0.4)
     budget = 6 * sv.Server.footprint(620)
@@ -86,5 +89,6 @@ def test_batches_share_builds_and_respect_the_budget(bj, monkeypatch):
     distinct = len({(r.shape, r.dofs, r.kind, r.wavenumber) for r in reqs})
     assert sum(built) >= distinct and sum(built) + st["cache_hits"] == 40
     assert max(built) > 1 and max(built) <= 8
+    assert factored == built            # each batch's matrices are factored together
     assert max(peak) <= budget
     assert all(np.array_equal(r.result, 2.0 * np.ones((r.n, len(r.rhs)))) for r in reqs)
