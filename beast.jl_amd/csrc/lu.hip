@@ -49,6 +49,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <vector>
 
@@ -1780,8 +1781,10 @@ __global__ __launch_bounds__(256) void k_trsv_flag(const float2 *A, long long ld
 }
 
 // r = b - A x in FP64 (A row-major n x n), one wave per row; x, b, r are n x nrhs row-major.
-__global__ __launch_bounds__(256) void k_residual(const double2 *__restrict__ A, int n, const double2 *__restrict__ x,
-                                                  const double2 *__restrict__ b, double2 *__restrict__ r, int nrhs)
+// r = b - A x (RESID) or r = A x, for nrhs columns (x, b, r row-major n x nrhs)
+template <bool RESID>
+__device__ __forceinline__ void apply_a_body(const double2 *__restrict__ A, int n, const double2 *__restrict__ x,
+                                             const double2 *__restrict__ b, double2 *__restrict__ r, int nrhs)
 {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
@@ -1819,90 +1822,99 @@ __global__ __launch_bounds__(256) void k_residual(const double2 *__restrict__ A,
             sy += __shfl_xor(sy, off, 64);
         }
         if (lane == 0) {
-            const double2 bv = b[(size_t)row * nrhs + q];
-            r[(size_t)row * nrhs + q] = {bv.x - sx, bv.y - sy};
+            if constexpr (RESID) {
+                const double2 bv = b[(size_t)row * nrhs + q];
+                r[(size_t)row * nrhs + q] = {bv.x - sx, bv.y - sy};
+            } else {
+                r[(size_t)row * nrhs + q] = {sx, sy};
+            }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_residual(const double2 *__restrict__ A, int n, const double2 *__restrict__ x,
+                                                  const double2 *__restrict__ b, double2 *__restrict__ r, int nrhs)
+{
+    apply_a_body<true>(A, n, x, b, r, nrhs);
+}
+
+__global__ __launch_bounds__(256) void k_matvec_cols(const double2 *__restrict__ A, int n,
+                                                     const double2 *__restrict__ x, double2 *__restrict__ y, int nrhs)
+{
+    apply_a_body<false>(A, n, x, nullptr, y, nrhs);
 }
 
 // y = A x for one vector (FP64), used by GMRES.
-__global__ __launch_bounds__(256) void k_matvec(const double2 *__restrict__ A, int n, const double2 *__restrict__ x,
-                                                double2 *__restrict__ y)
+// The same for q columns side by side (row-major n x q blocks): one Krylov space per
+// column, all advanced by the same launches.
+struct ColScale {
+    double s[kMaxRhs];
+};
+
+__global__ void k_scale_cols(const double2 *__restrict__ src, int n, int q, ColScale sc, double2 *__restrict__ dst)
 {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;
-    const int lane = threadIdx.x & 63;
-    const double2 *a = A + (size_t)row * n;
-    double sx = 0, sy = 0;
-    int c = lane;
-    // four row loads in flight per lane, summed in the same order as one at a time
-    for (; c + 192 < n; c += 256) {
-        double2 av[4], xv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) av[u] = a[c + 64 * u];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) xv[u] = x[c + 64 * u];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            sx += av[u].x * xv[u].x - av[u].y * xv[u].y;
-            sy += av[u].x * xv[u].y + av[u].y * xv[u].x;
-        }
-    }
-    for (; c < n; c += 64) {
-        const double2 av = a[c], xv = x[c];
-        sx += av.x * xv.x - av.y * xv.y;
-        sy += av.x * xv.y + av.y * xv.x;
-    }
-    for (int off = 32; off >= 1; off >>= 1) {
-        sx += __shfl_xor(sx, off, 64);
-        sy += __shfl_xor(sy, off, 64);
-    }
-    if (lane == 0) y[row] = {sx, sy};
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * q) return;
+    const double f = sc.s[i % q];
+    dst[i] = {src[i].x * f, src[i].y * f};
 }
 
-// dots[j] = sum_i conj(V[j][i]) w[i] for j < m, partial per block: out[block][j]
-__global__ __launch_bounds__(256) void k_dots(const double2 *__restrict__ Vb, int n, int m, const double2 *__restrict__ w,
-                                              double2 *__restrict__ out)
+// out[block][j][c] = sum over the block's rows i of conj(V_j[i][c]) w[i][c], j < m, c < q
+__global__ __launch_bounds__(256) void k_dots_cols(const double2 *__restrict__ Vb, int n, int m, int q,
+                                                   const double2 *__restrict__ w, double2 *__restrict__ out)
 {
-    __shared__ double sx[256], sy[256];
+    __shared__ double2 red[4][kMaxRhs];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int j = 0; j < m; ++j) {
-        const double2 *v = Vb + (size_t)j * n;
-        double ax = 0, ay = 0;
+        const double2 *v = Vb + (size_t)j * n * q;
+        double ax[kMaxRhs], ay[kMaxRhs];
+#pragma unroll
+        for (int c = 0; c < kMaxRhs; ++c) ax[c] = ay[c] = 0;
         for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-            const double2 a = v[i], b = w[i];
-            ax += a.x * b.x + a.y * b.y;
-            ay += a.x * b.y - a.y * b.x;
+#pragma unroll
+            for (int c = 0; c < kMaxRhs; ++c) {
+                if (c < q) {
+                    const double2 a = v[(size_t)i * q + c], b = w[(size_t)i * q + c];
+                    ax[c] += a.x * b.x + a.y * b.y;
+                    ay[c] += a.x * b.y - a.y * b.x;
+                }
+            }
         }
-        sx[threadIdx.x] = ax;
-        sy[threadIdx.x] = ay;
+#pragma unroll
+        for (int c = 0; c < kMaxRhs; ++c) {
+            if (c < q) {
+                double sx = ax[c], sy = ay[c];
+                for (int off = 32; off >= 1; off >>= 1) {
+                    sx += __shfl_xor(sx, off, 64);
+                    sy += __shfl_xor(sy, off, 64);
+                }
+                if (lane == 0) red[wid][c] = {sx, sy};
+            }
+        }
         __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) { sx[threadIdx.x] += sx[threadIdx.x + s]; sy[threadIdx.x] += sy[threadIdx.x + s]; }
-            __syncthreads();
+        if ((int)threadIdx.x < q) {
+            const int c = threadIdx.x;
+            out[((size_t)blockIdx.x * m + j) * q + c] = {red[0][c].x + red[1][c].x + red[2][c].x + red[3][c].x,
+                                                        red[0][c].y + red[1][c].y + red[2][c].y + red[3][c].y};
         }
-        if (threadIdx.x == 0) out[(size_t)blockIdx.x * m + j] = {sx[0], sy[0]};
         __syncthreads();
     }
 }
 
-// w -= sum_j h[j] V[j]
-__global__ void k_axpys(const double2 *__restrict__ Vb, int n, int m, const double2 *__restrict__ h, double2 *w)
+// w[i][c] -= sum_j h[j][c] V_j[i][c]
+__global__ void k_axpys_cols(const double2 *__restrict__ Vb, int n, int m, int q, const double2 *__restrict__ h,
+                             double2 *w)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * q) return;
+    const int c = (int)(i % q);
     double2 acc = w[i];
     for (int j = 0; j < m; ++j) {
-        const double2 v = Vb[(size_t)j * n + i], c = h[j];
-        acc.x -= c.x * v.x - c.y * v.y;
-        acc.y -= c.x * v.y + c.y * v.x;
+        const double2 v = Vb[(size_t)j * n * q + i], cf = h[(size_t)j * q + c];
+        acc.x -= cf.x * v.x - cf.y * v.y;
+        acc.y -= cf.x * v.y + cf.y * v.x;
     }
     w[i] = acc;
-}
-
-__global__ void k_scale_copy(const double2 *__restrict__ src, int n, double s, double2 *__restrict__ dst)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = {src[i].x * s, src[i].y * s};
 }
 
 template <typename T>
@@ -3181,23 +3193,6 @@ double host_norm(const std::vector<double2> &v, int n, int nrhs, int q)
     return std::sqrt(s);
 }
 
-// Sum of 256 per-block partial dot products (k_dots output, column j of m).
-std::complex<double> sum_partials(const std::vector<double2> &hp, int m, int j)
-{
-    double ax = 0, ay = 0;
-    for (int b = 0; b < 256; ++b) { ax += hp[(size_t)b * m + j].x; ay += hp[(size_t)b * m + j].y; }
-    return {ax, ay};
-}
-
-// GMRES on A d = r0 (one right-hand side) right-preconditioned by the factors,
-// A M^-1 u = r0, d = M^-1 u, with the preconditioned basis Z_k = M^-1 V_k kept
-// (flexible form, so d = Z y needs no further solve).  FP64 vectors and FP64
-// matrix products; classical Gram-Schmidt applied twice; Givens rotations.
-// The Arnoldi residual estimate is the true residual ||r0 - A d|| in exact
-// arithmetic, so the iteration stops at ||r0 - A d|| <= atol.  Restarts use
-// the recomputed true residual.  Returns the number of iterations.
-// Device workspace of GMRES (restart r, n unknowns): Vb (r + 1) n, Zb r n, w n, r n,
-// part 256 (r + 1), hd r + 1 complex doubles.
 struct GmresWork {
     double2 *Vb, *Zb, *w, *r, *part, *hd;
 };
@@ -3208,122 +3203,178 @@ __global__ void k_zero(double2 *x, long long cnt)
     if (i < cnt) x[i] = double2{0.0, 0.0};
 }
 
-int gmres_right(bh_matrix *M, const double2 *r0, double r0norm, double2 *d, double atol, int restart, int maxinner,
-                const GmresWork &G)
+// GMRES on A D = R0 for the q columns of R0 (row-major n x q) at once: one right-
+// preconditioned flexible GMRES per column (A M^-1 u = r0, d = M^-1 u, with the
+// preconditioned basis Z_k = M^-1 V_k kept, so d = Z y needs no further solve), FP64
+// vectors and matrix products, classical Gram-Schmidt applied twice, Givens rotations,
+// restarts from the recomputed true residual.  The columns share every launch (the
+// factors and A are read once per step for all of them); each column's arithmetic is that
+// of its own GMRES.  Column c stops at ||r0_c - A d_c|| <= atol[c] (a column starting
+// there does no step) or after maxinner steps.  Returns the steps summed over the columns.
+// Device workspace (restart r): Vb (r + 1) n q, Zb r n q, w n q, r n q, part 256 (r + 1) q,
+// hd (r + 1) q complex doubles.
+int gmres_block(bh_matrix *M, const double2 *R0, int q, const double *r0norm, double2 *D, const double *atol,
+                int restart, int maxinner, const GmresWork &G)
 {
     using C = std::complex<double>;
     hipStream_t s = M->ctx->stream;
     const int n = (int)M->n;
-    const unsigned g = (unsigned)((n + 255) / 256);
+    const long long nq = (long long)n * q;
+    const unsigned gq = (unsigned)((nq + 255) / 256);
     double2 *const Vb = G.Vb, *const Zb = G.Zb, *const w = G.w, *const r = G.r, *const part = G.part,
                    *const hd = G.hd;
-    std::vector<double2> hp((size_t)256 * (restart + 1));
-    k_zero<<<g, 256, 0, s>>>(d, n);
+    std::vector<double2> hp((size_t)256 * (restart + 1) * q);
+    auto sum_cols = [&](int m, int j, int c) {   // sum of the 256 block partials of (j, c)
+        double ax = 0, ay = 0;
+        for (int b = 0; b < 256; ++b) {
+            const double2 v = hp[((size_t)b * m + j) * q + c];
+            ax += v.x;
+            ay += v.y;
+        }
+        return C(ax, ay);
+    };
+    k_zero<<<gq, 256, 0, s>>>(D, nq);
     check("k_zero");
-    double beta = r0norm;
-    const double2 *rcur = r0;
-    int total = 0;
-    while (beta > atol && total < maxinner) {
-        k_scale_copy<<<g, 256, 0, s>>>(rcur, n, 1.0 / beta, Vb);
-        check("k_scale_copy");
-        std::vector<C> H((size_t)(restart + 1) * restart, 0.0), cs(restart), sn(restart), gvec(restart + 1, 0.0);
-        gvec[0] = beta;
-        int k = 0;
-        bool done = false;
-        for (; k < restart && total < maxinner; ++k, ++total) {
-            double2 *zk = Zb + (size_t)k * n;
-            apply_factors(M, Vb + (size_t)k * n, zk, 1);
+    std::vector<double> beta(r0norm, r0norm + q);
+    std::vector<int> total(q, 0);
+    std::vector<char> live(q);
+    for (int c = 0; c < q; ++c) live[c] = beta[c] > atol[c];
+    const double2 *rcur = R0;
+    for (;;) {
+        bool any = false;
+        for (int c = 0; c < q; ++c) {
+            if (live[c] && total[c] >= maxinner) live[c] = 0;
+            any = any || live[c];
+        }
+        if (!any) break;
+        ColScale sc{};
+        for (int c = 0; c < q; ++c) sc.s[c] = live[c] ? 1.0 / beta[c] : 0.0;
+        k_scale_cols<<<gq, 256, 0, s>>>(rcur, n, q, sc, Vb);
+        check("k_scale_cols");
+        std::vector<std::vector<C>> H(q, std::vector<C>((size_t)(restart + 1) * restart, 0.0)), cs(q, std::vector<C>(restart)),
+            sn(q, std::vector<C>(restart)), gvec(q, std::vector<C>(restart + 1, 0.0));
+        std::vector<int> kc(q, 0);          // steps of this cycle per column
+        std::vector<char> run(live);        // columns still stepping in this cycle
+        std::vector<char> conv(q, 0);
+        for (int c = 0; c < q; ++c) gvec[c][0] = beta[c];
+        for (int k = 0; k < restart; ++k) {
+            bool stepping = false;
+            for (int c = 0; c < q; ++c) stepping = stepping || run[c];
+            if (!stepping) break;
+            double2 *zk = Zb + (size_t)k * nq;
+            apply_factors(M, Vb + (size_t)k * nq, zk, q);
             {
                 ProfScope prof(*M->ctx, "k_matvec", 16.0 * n * (double)n);
-                k_matvec<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, zk, w);
+                k_matvec_cols<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, zk, w, q);
             }
             check("k_matvec");
-            std::vector<C> h(k + 1, 0.0);
+            std::vector<std::vector<C>> h(q, std::vector<C>(k + 1, 0.0));
             for (int pass = 0; pass < 2; ++pass) {
-                k_dots<<<256, 256, 0, s>>>(Vb, n, k + 1, w, part);
+                k_dots_cols<<<256, 256, 0, s>>>(Vb, n, k + 1, q, w, part);
                 check("k_dots");
-                BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double2) * 256 * (k + 1), hipMemcpyDeviceToHost, s));
+                BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double2) * 256 * (k + 1) * q,
+                                            hipMemcpyDeviceToHost, s));
                 BH_CHECK_HIP(hipStreamSynchronize(s));
-                std::vector<double2> hh(k + 1);
-                for (int j = 0; j <= k; ++j) {
-                    const C v = sum_partials(hp, k + 1, j);
-                    hh[j] = {v.real(), v.imag()};
-                    h[j] += v;
+                std::vector<double2> hh((size_t)(k + 1) * q, double2{0, 0});
+                for (int c = 0; c < q; ++c) {
+                    if (!run[c]) continue;
+                    for (int j = 0; j <= k; ++j) {
+                        const C v = sum_cols(k + 1, j, c);
+                        hh[(size_t)j * q + c] = {v.real(), v.imag()};
+                        h[c][j] += v;
+                    }
                 }
-                BH_CHECK_HIP(hipMemcpyAsync(hd, hh.data(), sizeof(double2) * (k + 1), hipMemcpyHostToDevice, s));
-                k_axpys<<<g, 256, 0, s>>>(Vb, n, k + 1, hd, w);
+                BH_CHECK_HIP(hipMemcpyAsync(hd, hh.data(), sizeof(double2) * (k + 1) * q, hipMemcpyHostToDevice, s));
+                k_axpys_cols<<<gq, 256, 0, s>>>(Vb, n, k + 1, q, hd, w);
                 check("k_axpys");
             }
-            k_dots<<<256, 256, 0, s>>>(w, n, 1, w, part);
+            k_dots_cols<<<256, 256, 0, s>>>(w, n, 1, q, w, part);
             check("k_dots");
-            BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double2) * 256, hipMemcpyDeviceToHost, s));
+            BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double2) * 256 * q, hipMemcpyDeviceToHost, s));
             BH_CHECK_HIP(hipStreamSynchronize(s));
-            const double hn = std::sqrt(std::max(0.0, sum_partials(hp, 1, 0).real()));
-            for (int j = 0; j <= k; ++j) H[(size_t)j * restart + k] = h[j];
-            H[(size_t)(k + 1) * restart + k] = hn;
-            for (int j = 0; j < k; ++j) {
-                const C x = H[(size_t)j * restart + k], y = H[(size_t)(j + 1) * restart + k];
-                H[(size_t)j * restart + k] = std::conj(cs[j]) * x + std::conj(sn[j]) * y;
-                H[(size_t)(j + 1) * restart + k] = -sn[j] * x + cs[j] * y;
+            ColScale nx{};
+            for (int c = 0; c < q; ++c) {
+                if (!run[c]) continue;
+                const double hn = std::sqrt(std::max(0.0, sum_cols(1, 0, c).real()));
+                auto &Hc = H[c];
+                for (int j = 0; j <= k; ++j) Hc[(size_t)j * restart + k] = h[c][j];
+                Hc[(size_t)(k + 1) * restart + k] = hn;
+                for (int j = 0; j < k; ++j) {
+                    const C x = Hc[(size_t)j * restart + k], y = Hc[(size_t)(j + 1) * restart + k];
+                    Hc[(size_t)j * restart + k] = std::conj(cs[c][j]) * x + std::conj(sn[c][j]) * y;
+                    Hc[(size_t)(j + 1) * restart + k] = -sn[c][j] * x + cs[c][j] * y;
+                }
+                const C x = Hc[(size_t)k * restart + k], y = Hc[(size_t)(k + 1) * restart + k];
+                const double den = std::sqrt(std::norm(x) + std::norm(y));
+                cs[c][k] = den == 0 ? C(1.0) : x / den;
+                sn[c][k] = den == 0 ? C(0.0) : y / den;
+                Hc[(size_t)k * restart + k] = den;
+                Hc[(size_t)(k + 1) * restart + k] = 0.0;
+                gvec[c][k + 1] = -sn[c][k] * gvec[c][k];
+                gvec[c][k] = std::conj(cs[c][k]) * gvec[c][k];
+                kc[c] = k + 1;
+                ++total[c];
+                if (std::abs(gvec[c][k + 1]) <= atol[c] || hn == 0) {
+                    conv[c] = 1;
+                    run[c] = 0;
+                } else if (total[c] >= maxinner) {
+                    run[c] = 0;
+                } else {
+                    nx.s[c] = 1.0 / hn;
+                }
             }
-            const C x = H[(size_t)k * restart + k], y = H[(size_t)(k + 1) * restart + k];
-            const double den = std::sqrt(std::norm(x) + std::norm(y));
-            cs[k] = den == 0 ? C(1.0) : x / den;
-            sn[k] = den == 0 ? C(0.0) : y / den;
-            H[(size_t)k * restart + k] = den;
-            H[(size_t)(k + 1) * restart + k] = 0.0;
-            gvec[k + 1] = -sn[k] * gvec[k];
-            gvec[k] = std::conj(cs[k]) * gvec[k];
-            if (hn != 0 && k + 1 < restart) {
-                k_scale_copy<<<g, 256, 0, s>>>(w, n, 1.0 / hn, Vb + (size_t)(k + 1) * n);
-                check("k_scale_copy");
+            if (k + 1 < restart) {
+                k_scale_cols<<<gq, 256, 0, s>>>(w, n, q, nx, Vb + (size_t)(k + 1) * nq);
+                check("k_scale_cols");
             }
-            if (std::abs(gvec[k + 1]) <= atol || hn == 0) { ++k; ++total; done = true; break; }
         }
-        // d += Z y with H y = g (upper triangular k x k)
-        std::vector<C> y(k, 0.0);
-        for (int i = k - 1; i >= 0; --i) {
-            C t = gvec[i];
-            for (int j = i + 1; j < k; ++j) t -= H[(size_t)i * restart + j] * y[j];
-            y[i] = H[(size_t)i * restart + i] == 0.0 ? C(0.0) : t / H[(size_t)i * restart + i];
+        // D[:, c] += Z y_c with H_c y_c = g_c (upper triangular kc x kc)
+        int kmax = 0;
+        for (int c = 0; c < q; ++c) kmax = std::max(kmax, kc[c]);
+        if (kmax > 0) {
+            std::vector<double2> yn((size_t)kmax * q, double2{0, 0});
+            for (int c = 0; c < q; ++c) {
+                const int k = kc[c];
+                const auto &Hc = H[c];
+                std::vector<C> y(k, 0.0);
+                for (int i = k - 1; i >= 0; --i) {
+                    C t = gvec[c][i];
+                    for (int j = i + 1; j < k; ++j) t -= Hc[(size_t)i * restart + j] * y[j];
+                    y[i] = Hc[(size_t)i * restart + i] == 0.0 ? C(0.0) : t / Hc[(size_t)i * restart + i];
+                }
+                for (int j = 0; j < k; ++j) yn[(size_t)j * q + c] = {-y[j].real(), -y[j].imag()};
+            }
+            BH_CHECK_HIP(hipMemcpyAsync(hd, yn.data(), sizeof(double2) * kmax * q, hipMemcpyHostToDevice, s));
+            k_axpys_cols<<<gq, 256, 0, s>>>(Zb, n, kmax, q, hd, D);
+            check("k_axpys");
         }
-        std::vector<double2> yn(k);
-        for (int j = 0; j < k; ++j) yn[j] = {-y[j].real(), -y[j].imag()};
-        BH_CHECK_HIP(hipMemcpyAsync(hd, yn.data(), sizeof(double2) * k, hipMemcpyHostToDevice, s));
-        k_axpys<<<g, 256, 0, s>>>(Zb, n, k, hd, d);
-        check("k_axpys");
-        if (done || total >= maxinner) break;
-        // restart from the true residual r0 - A d
+        bool again = false;
+        for (int c = 0; c < q; ++c) {
+            if (conv[c]) live[c] = 0;
+            if (live[c] && total[c] >= maxinner) live[c] = 0;
+            again = again || live[c];
+        }
+        if (!again) break;
+        // restart from the true residuals R0 - A D
         {
             ProfScope prof(*M->ctx, "k_residual", 16.0 * n * (double)n);
-            k_residual<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, d, r0, r, 1);
+            k_residual<<<(n + 3) / 4, 256, 0, s>>>((const double2 *)M->a64.p, n, D, R0, r, q);
         }
         check("k_residual");
-        k_dots<<<256, 256, 0, s>>>(r, n, 1, r, part);
+        k_dots_cols<<<256, 256, 0, s>>>(r, n, 1, q, r, part);
         check("k_dots");
-        BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double2) * 256, hipMemcpyDeviceToHost, s));
+        BH_CHECK_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double2) * 256 * q, hipMemcpyDeviceToHost, s));
         BH_CHECK_HIP(hipStreamSynchronize(s));
-        beta = std::sqrt(std::max(0.0, sum_partials(hp, 1, 0).real()));
+        for (int c = 0; c < q; ++c) {
+            beta[c] = std::sqrt(std::max(0.0, sum_cols(1, 0, c).real()));
+            if (live[c] && beta[c] <= atol[c]) live[c] = 0;
+        }
         rcur = r;
     }
     BH_CHECK_HIP(hipStreamSynchronize(s));
-    return total;
-}
-
-// Column r of a row-major n x q block: out[i] = X[i q + r] (get) or X[i q + r] += in[i] (add).
-__global__ void k_get_col(const double2 *__restrict__ X, int n, int q, int r, double2 *__restrict__ out)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = X[(size_t)i * q + r];
-}
-
-__global__ void k_add_col(double2 *__restrict__ X, int n, int q, int r, const double2 *__restrict__ in)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        X[(size_t)i * q + r].x += in[i].x;
-        X[(size_t)i * q + r].y += in[i].y;
-    }
+    int sum = 0;
+    for (int c = 0; c < q; ++c) sum += total[c];
+    return sum;
 }
 
 }  // namespace
@@ -3334,8 +3385,18 @@ constexpr int kGmresRestart = 30;
 
 // Device workspace of one solve: db, dx, dr (n x nrhs each), rcol, dcol (n each) and the
 // GMRES workspace, each piece 256-byte aligned inside one buffer.
+// x += d elementwise
+__global__ void k_add_to(double2 *__restrict__ x, long long cnt, const double2 *__restrict__ d)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) {
+        x[i].x += d[i].x;
+        x[i].y += d[i].y;
+    }
+}
+
 struct SolveWork {
-    double2 *db, *dx, *dr, *rcol, *dcol;
+    double2 *db, *dx, *dr, *dd;
     GmresWork g;
 };
 
@@ -3352,14 +3413,13 @@ size_t solve_work_layout(int n, int nrhs, char *base, SolveWork *W)
     w.db = take(nq);
     w.dx = take(nq);
     w.dr = take(nq);
-    w.rcol = take(n);
-    w.dcol = take(n);
-    w.g.Vb = take((r + 1) * n);
-    w.g.Zb = take(r * n);
-    w.g.w = take(n);
-    w.g.r = take(n);
-    w.g.part = take(256 * (r + 1));
-    w.g.hd = take(r + 1);
+    w.dd = take(nq);
+    w.g.Vb = take((r + 1) * nq);
+    w.g.Zb = take(r * nq);
+    w.g.w = take(nq);
+    w.g.r = take(nq);
+    w.g.part = take(256 * (r + 1) * (size_t)nrhs);
+    w.g.hd = take((r + 1) * (size_t)nrhs);
     if (W) *W = w;
     return off;
 }
@@ -3429,17 +3489,16 @@ void solve_impl(bh_matrix *M, int64_t nrhs, const double *B, double *X, double t
             const double target = tol > 0 ? tol : 1e-12;
             if (res <= target || iters >= maxit_) break;
             ++iters;
-            const unsigned gn = (unsigned)((n + 255) / 256);
+            // solve A D = R for the columns above the target, each to 1/2 of its remaining
+            // target, then X += D (columns already there have D = 0)
+            std::vector<double> rn(q), atol(q);
             for (int r = 0; r < q; ++r) {
-                if (rel[r] <= target) continue;
-                // solve A d = r_r to 1/2 of the remaining target, then x_r += d
-                k_get_col<<<gn, 256, 0, s>>>(W.dr, n, q, r, W.rcol);
-                check("k_get_col");
-                const double rn = rel[r] * bn[r];
-                inner += gmres_right(M, W.rcol, rn, W.dcol, 0.5 * target * bn[r], kGmresRestart, 200, W.g);
-                k_add_col<<<gn, 256, 0, s>>>(W.dx, n, q, r, W.dcol);
-                check("k_add_col");
+                rn[r] = rel[r] * bn[r];
+                atol[r] = rel[r] <= target ? std::numeric_limits<double>::infinity() : 0.5 * target * bn[r];
             }
+            inner += gmres_block(M, W.dr, q, rn.data(), W.dd, atol.data(), kGmresRestart, 200, W.g);
+            k_add_to<<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(W.dx, cnt, W.dd);
+            check("k_add_to");
         }
     }
     float ms = 0;

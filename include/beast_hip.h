@@ -273,8 +273,9 @@ int bh_matrix_solve(bh_matrix *M, int64_t nrhs, const double *B, double *X, doub
  * that cannot allocate device memory, e.g. code under HIP graph capture or a tight serving
  * loop of repeated right-hand sides on resident factors).  Same arguments, results and
  * errors as bh_matrix_solve; its workspace is a fixed 64 MiB scratch array per device instead
- * of a fresh allocation, which limits n * (3 nrhs + 63) * 16 bytes (plus alignment) to 64 MiB,
- * about n <= 60,000 with one right-hand side; larger solves are refused (BH_ERR_INVALID).
+ * of a fresh allocation, which limits 67 * n * nrhs * 16 bytes (plus alignment and 0.13 MB
+ * per right-hand side) to 64 MiB, about n * nrhs <= 62,000; larger solves are refused
+ * (BH_ERR_INVALID).
  * The factors' own solve buffers are allocated by bh_matrix_factor.  All calls on one device
  * share the array: calls that overlap in time on one GPU race (serialise them). */
 int bh_matrix_solve_noalloc(bh_matrix *M, int64_t nrhs, const double *B, double *X, double tol, int maxit,
