@@ -776,7 +776,7 @@ __global__ __launch_bounds__(256) void k_swap_plan(const long long *ipiv, int k1
 // columns per workgroup (64 KiB of LDS at most), every moved row read into LDS
 // before any is written.
 template <typename T>
-constexpr int laswp_cols() { return sizeof(T) == 4 ? 16 : 8; }
+constexpr int laswp_cols() { return sizeof(T) == 4 ? 32 : 8; }
 
 template <typename T>
 __device__ __forceinline__ void laswp_apply_body(typename CT<T>::t *A, long long lda, const int *plan, int len, int c0,
@@ -800,6 +800,45 @@ __device__ __forceinline__ void laswp_apply_body(typename CT<T>::t *A, long long
     for (int s = rl; s < 2 * len; s += RL) {
         const int p = pos[s], q = src[s];
         if (col >= 0 && p >= 0 && p != q) A[(long long)p * lda + col] = buf[s * CW + cl];
+    }
+}
+
+// FP32: a workgroup takes 32 columns, one per thread and eight slots at a time; every
+// moved entry is held in registers (all its loads in flight at once, no LDS staging),
+// then written after the barrier.
+template <>
+__device__ __forceinline__ void laswp_apply_body<float>(float2 *A, long long lda, const int *plan, int len, int c0,
+                                                        int c1, int c2, int c3, int bid)
+{
+    constexpr int CW = laswp_cols<float>(), RL = 256 / CW, MAXK = 2 * kMaxSwaps / RL;
+    __shared__ int s_src[2 * kMaxSwaps], s_pos[2 * kMaxSwaps];
+    const int tid = threadIdx.x, cl = tid % CW, rl = tid / CW;
+    for (int q = tid; q < 2 * len; q += 256) {
+        s_src[q] = plan[q];
+        s_pos[q] = plan[2 * len + q];
+    }
+    __syncthreads();
+    const int t = bid * CW + cl, w1 = c1 - c0;
+    int col = -1;
+    if (t < w1) col = c0 + t;
+    else if (t < w1 + (c3 - c2)) col = c2 + (t - w1);
+    float2 v[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) {
+        const int sl = rl + RL * k;
+        if (sl < 2 * len && col >= 0) {
+            const int p = s_pos[sl], q = s_src[sl];
+            if (p >= 0 && p != q) v[k] = A[(long long)q * lda + col];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) {
+        const int sl = rl + RL * k;
+        if (sl < 2 * len && col >= 0) {
+            const int p = s_pos[sl], q = s_src[sl];
+            if (p >= 0 && p != q) A[(long long)p * lda + col] = v[k];
+        }
     }
 }
 
@@ -2010,7 +2049,7 @@ void laswp(Ctx &ctx, hipStream_t s, typename CT<T>::t *A, long long lda, int k1,
     k_swap_plan<<<1, 256, 0, s>>>(ipiv, k1, k2, plan);
     check("k_swap_plan");
     constexpr int CW = laswp_cols<T>();
-    const size_t shm = (size_t)2 * len * CW * sizeof(typename CT<T>::t);
+    const size_t shm = sizeof(T) == 4 ? 0 : (size_t)2 * len * CW * sizeof(typename CT<T>::t);
     k_laswp_apply<T><<<(w + CW - 1) / CW, 256, shm, s>>>(A, lda, plan, len, c0, std::max(c0, c1), c2, std::max(c2, c3));
     check("k_laswp_apply");
 }
@@ -2774,7 +2813,6 @@ void plan_factor(BatchPlan &P, std::vector<BatchMat> &mats)
         for (int q0 = 0; q0 < kNBInner; q0 += kMaxSwaps) {
             std::vector<std::pair<SwapBA, long long>> sw;
             std::vector<std::pair<LaswpBA, long long>> la;
-            int lmax = 0;
             double wl = 0;
             for (auto &m : mats) {
                 if (m.n <= k) continue;
@@ -2786,12 +2824,12 @@ void plan_factor(BatchPlan &P, std::vector<BatchMat> &mats)
                 const int w = k + (c3 - c2);
                 if (w <= 0) continue;
                 sw.push_back({SwapBA{m.ipiv, q, q + len, m.plan}, 1});
-                la.push_back({LaswpBA{m.A, m.lda, m.plan, len, 0, k, c2, c3}, (w + 15) / 16});
-                lmax = std::max(lmax, len);
+                constexpr int cw = laswp_cols<float>();
+                la.push_back({LaswpBA{m.A, m.lda, m.plan, len, 0, k, c2, c3}, (w + cw - 1) / cw});
                 wl += 2.0 * 2 * len * w * sizeof(float2);
             }
             P.add(kBkSwap, sw);
-            P.add(kBkLaswp, la, (size_t)2 * lmax * 16 * sizeof(float2), wl);
+            P.add(kBkLaswp, la, 0, wl);
         }
         // block row U12 = L11^-1 A12 and the trailing update, for matrices with columns right
         // of the block: inverses of L11's 128-row diagonal blocks, then per 128-row block
