@@ -854,8 +854,13 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         ps = ps * u - (R)(1.0 / 6.0);
         const R ct = pc * u + (R)1, st = t * (ps * u + (R)1);
         const R amp = idist * (R)kInv4Pi;
-        // exp(-i gi R) = (cc - i sc)(ct - i st)
-        return {(cc * ct - sc * st) * amp, -(sc * ct + cc * st) * amp};
+        // exp(-i gi (R - Rc)) / (4 pi R); the pair's common factor exp(-i gi Rc) = cc - i sc
+        // multiplies its moments once (rot below), every quantity summed being linear in G
+        return {ct * amp, -st * amp};
+    };
+    // moments (or block entries) times exp(-i gi Rc) when the phases are shifted
+    auto rot = [&](cx<R> &v) {
+        if constexpr (shifted) v = {cc * v.r + sc * v.i, cc * v.i - sc * v.r};
     };
     const R *xq = (const R *)A.tq + (size_t)p * Qo * 4;
     const R *yq = (const R *)A.sq + (size_t)q * Qi * 4;
@@ -890,6 +895,13 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
             }
             T1.r += wp * (x[0] * S1[0].r + x[1] * S1[1].r + x[2] * S1[2].r);
             T1.i += wp * (x[0] * S1[0].i + x[1] * S1[1].i + x[2] * S1[2].i);
+        }
+        rot(T0);
+        rot(T1);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            rot(Tx[k]);
+            rot(Ty[k]);
         }
         R a[9], b[9];
         load_verts(a, b);
@@ -958,7 +970,10 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         }
         const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
 #pragma unroll
-        for (int k = 0; k < 9; ++k) z[k * stride] = {s1 * zz[k].r, s1 * zz[k].i};
+        for (int k = 0; k < 9; ++k) {
+            rot(zz[k]);
+            z[k * stride] = {s1 * zz[k].r, s1 * zz[k].i};
+        }
     } else {
         // M1 = sum w x.U, MU = sum w U, MX = sum w x x Rv, MR = sum w Rv with
         // U = xd x H1, Rv = xd H0 - H1, H0 = sum w h, H1 = sum w h y, h = -(gamma + 1/R) G / R
@@ -1004,6 +1019,13 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
                 MX[k].r += wp * XR[k].r; MX[k].i += wp * XR[k].i;
                 MR[k].r += wp * Rv[k].r; MR[k].i += wp * Rv[k].i;
             }
+        }
+        rot(M1);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            rot(MU[k]);
+            rot(MX[k]);
+            rot(MR[k]);
         }
         R a[9], b[9];
         load_verts(a, b);
