@@ -794,6 +794,9 @@ template <typename R, int OPK, int QO = 0, int QI = 0, bool SHIFT = false>
 __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> *z, int stride, bool maybe_near = true)
 {
     const int Qo = QO ? QO : A.Qo, Qi = QI ? QI : A.Qi;
+    // the test-point loop of the small default rules is unrolled too (independent point
+    // pairs of several test points in flight: measured 10% faster at 3 x 4 points)
+    constexpr int kUnrollO = (QO > 0 && QO <= 4) ? QO : 1;
     const double *tg = A.tgeo + (size_t)p * kGeo, *sg = A.sgeo + (size_t)q * kGeo;
     if (A.mode != kModeDoubleNum && maybe_near) {
         // the classification pass listed p's non-far trial cells in ascending
@@ -866,7 +869,7 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
     const R *yq = (const R *)A.sq + (size_t)q * Qi * 4;
     if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
         cx<R> T0{0, 0}, T1{0, 0}, Tx[3] = {{0, 0}, {0, 0}, {0, 0}}, Ty[3] = {{0, 0}, {0, 0}, {0, 0}};
-#pragma unroll 1
+#pragma unroll kUnrollO
         for (int ip = 0; ip < Qo; ++ip) {
             const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
             const R wp = xq[4 * ip + 3];
@@ -931,7 +934,7 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         cx<R> zz[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) zz[k] = {0, 0};
-#pragma unroll 1
+#pragma unroll kUnrollO
         for (int ip = 0; ip < Qo; ++ip) {
             const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
             const R wp = xq[4 * ip + 3];
@@ -979,7 +982,7 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         // U = xd x H1, Rv = xd H0 - H1, H0 = sum w h, H1 = sum w h y, h = -(gamma + 1/R) G / R
         cx<R> M1{0, 0}, MU[3] = {{0, 0}, {0, 0}, {0, 0}}, MX[3] = {{0, 0}, {0, 0}, {0, 0}},
                         MR[3] = {{0, 0}, {0, 0}, {0, 0}};
-#pragma unroll 1
+#pragma unroll kUnrollO
         for (int ip = 0; ip < Qo; ++ip) {
             const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
             const R wp = xq[4 * ip + 3];
