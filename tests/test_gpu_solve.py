@@ -207,3 +207,27 @@ def test_golden_efie_solution_gpu(bj, golden):
     u = bj.DeviceMatrix.assemble(t, X).factor("fp32split").solve(b, tol=1e-12)
     ref = np.sort(np.abs(u0))
     assert np.abs(np.sort(np.abs(u)) - ref).max() / ref.max() < 2e-3
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32split", 1e-12), ("fp32", 1e-12), ("fp64", 1e-13)])
+def test_right_hand_sides_in_lockstep_equal_one_at_a_time(bj, precision, tol):
+    """GMRES-IR refines all right-hand sides of a solve together (one Krylov space per
+    column, shared launches): each column equals, bit for bit, the solve of that column
+    alone, with columns converging after different numbers of steps; the iteration counts
+    add up over the columns."""
+    n = 2100
+    A = _rand(n, 77)
+    rng = np.random.default_rng(78)
+    B = rng.standard_normal((n, 5)) + 1j * rng.standard_normal((n, 5))
+    B[:, 2] *= 1e-6                                   # a column far smaller than the others
+    B[:, 4] = A @ (rng.standard_normal(n) + 0j)       # and one with a different spectrum mix
+    M = bj.DeviceMatrix.from_host(A).factor(precision)
+    X = M.solve(B, tol=tol, maxit=10)
+    inner_all = M.stats["inner"]
+    inner_sum = 0
+    for c in range(B.shape[1]):
+        x = M.solve(B[:, c], tol=tol, maxit=10)
+        inner_sum += M.stats["inner"]
+        assert np.array_equal(X[:, c], x), c
+        assert np.linalg.norm(A @ x - B[:, c]) / np.linalg.norm(B[:, c]) < 1e-11
+    assert inner_all == inner_sum
