@@ -940,10 +940,15 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
             const R wp = xq[4 * ip + 3];
             const R xd[3] = {x[0] + dc[0], x[1] + dc[1], x[2] + dc[2]};
             cx<R> Rs[3] = {{0, 0}, {0, 0}, {0, 0}}, Ss{0, 0};
+            // the trial points are read again for every test point (L1 hits) instead of being
+            // held in registers across the test-point loop (13 points: 104 VGPRs), which
+            // lets the 12 x 13-point tiles run two waves per SIMD
+            const R *yqi = yq;
+            asm volatile("" : "+v"(yqi));
 #pragma unroll
             for (int iq = 0; iq < Qi; ++iq) {
-                const R y[3] = {yq[4 * iq], yq[4 * iq + 1], yq[4 * iq + 2]};
-                const R wq = yq[4 * iq + 3];
+                const R y[3] = {yqi[4 * iq], yqi[4 * iq + 1], yqi[4 * iq + 2]};
+                const R wq = yqi[4 * iq + 3];
                 const R r[3] = {xd[0] - y[0], xd[1] - y[1], xd[2] - y[2]};
                 const R R2 = dot3(r, r);
                 R dist, idist;
@@ -1199,8 +1204,12 @@ __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
     }
 }
 
+// workgroups per CU the compiler must fit: two for n x K (see far_block), free otherwise
+template <int OPK>
+constexpr int far_min_blocks() { return OPK == BH_OP_MW_DOUBLELAYER_ROTATED ? 2 : 1; }
+
 template <typename R, int OPK, bool ROWMAJOR, int QO, int QI, bool SHIFT = false>
-__global__ __launch_bounds__(256) void k_far_tiles(TileArgs A)
+__global__ __launch_bounds__(256, far_min_blocks<OPK>()) void k_far_tiles(TileArgs A)
 {
     far_tiles_dev<R, OPK, ROWMAJOR, QO, QI, SHIFT>(A, blockIdx.y, blockIdx.x);
 }
@@ -1846,7 +1855,8 @@ __global__ __launch_bounds__(256) void k_wilton_batch(const BatchNear *reqs, con
 // dof tiles of several requests of one kernel configuration; tile t of a request is
 // (row t / column tiles, column t mod column tiles)
 template <typename R, int OPK, bool ROWMAJOR, int QO, int QI, bool SHIFT = false>
-__global__ __launch_bounds__(256) void k_far_tiles_batch(const TileArgs *args, const int *first, int n)
+__global__ __launch_bounds__(256, far_min_blocks<OPK>()) void k_far_tiles_batch(const TileArgs *args, const int *first,
+                                                                                 int n)
 {
     const int r = batch_find(first, n, blockIdx.x);
     const TileArgs A = args[r];
