@@ -727,9 +727,11 @@ constexpr int kNBFP32 = 1024, kNBFP32Small = 512, kNBDeepAbove = 24576;
 constexpr int kNBInner = 512;
 constexpr int kAheadFirstBelow = 20480;   // trailing width below which the next block column goes first
 
-inline int fp32_block_depth(int n, int k)
+// The 1024-deep block columns join their 512-column halves with split-bf16 products,
+// so the plain FP32 factorisation keeps 512 throughout.
+inline int fp32_block_depth(int n, int k, bool split)
 {
-    return n - k > kNBDeepAbove ? kNBFP32 : kNBFP32Small;
+    return split && n - k > kNBDeepAbove ? kNBFP32 : kNBFP32Small;
 }
 
 __device__ __forceinline__ void swap_plan_body(const long long *ipiv, int k1, int k2, int *plan)
@@ -1245,6 +1247,20 @@ __global__ __launch_bounds__(256) void k_split_cols(const float2 *__restrict__ B
     split_cols_body(B, ldb, K, cols, P, blockIdx.x, blockIdx.y);
 }
 
+// C is read and written once per tile with nontemporal accesses, so that the tile
+// rows of C streaming through do not evict the A and B planes, which the other
+// tiles of the launch re-read, from L2.
+__device__ __forceinline__ float2 ntload2(const float2 *p)
+{
+    return float2{__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y)};
+}
+
+__device__ __forceinline__ void ntstore2(float2 *p, float2 v)
+{
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
 // C[M x N] -= A B from split planes PA[4][M][K], PB[4][N][K]; K a multiple of 32.
 // WM x WN waves, wave tile (16 TM) x (16 TN) complex.  Two LDS stages (160 KB for the
 // 128 x 128 tile): the global loads of step k+1 are in flight while step k's products run
@@ -1367,7 +1383,7 @@ __device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA,
         for (int r = 0; r < 4; ++r) {
             long long gi, gj;
             cidx(0, b, r, gi, gj);
-            c0[b][r] = (gi < M && gj < N) ? C[gi * ldc + gj] : float2{0, 0};
+            c0[b][r] = (gi < M && gj < N) ? ntload2(C + gi * ldc + gj) : float2{0, 0};
         }
     compute(st);
     float2 cv[TM][TN][4];
@@ -1379,7 +1395,7 @@ __device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA,
             for (int r = 0; r < 4; ++r) {
                 long long gi, gj;
                 cidx(a, b, r, gi, gj);
-                cv[a][b][r] = (gi < M && gj < N) ? C[gi * ldc + gj] : float2{0, 0};
+                cv[a][b][r] = (gi < M && gj < N) ? ntload2(C + gi * ldc + gj) : float2{0, 0};
             }
 #pragma unroll
     for (int b = 0; b < TN; ++b)
@@ -1397,7 +1413,7 @@ __device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA,
                     float2 c = cv[a][b][r];
                     c.x -= arr[a][b][r] - aii[a][b][r];
                     c.y -= ari[a][b][r];
-                    C[gi * ldc + gj] = c;
+                    ntstore2(C + gi * ldc + gj, c);
                 }
             }
 }
@@ -2423,7 +2439,7 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
             BH_CHECK_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
             BH_CHECK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
         }
-        factor_block<T>(ctx, s, W, A, lda, n, 0, std::min(fp32_block_depth(n, 0), n), ipiv);
+        factor_block<T>(ctx, s, W, A, lda, n, 0, std::min(fp32_block_depth(n, 0, split), n), ipiv);
         // block k is factored; its interchanges reach the other columns (in pieces
         // of at most kMaxSwaps interchanges, applied in order) over [c0, c1) and [c2, c3)
         auto swaps = [&](int k, int kb, int c0, int c1, int c2, int c3) {
@@ -2431,12 +2447,12 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
                 laswp<T>(ctx, s, A, lda, q, std::min(q + kMaxSwaps, k + kb), ipiv, c0, c1, c2, c3, W.plan.p);
         };
         for (int k = 0, kb = 0; k < n; k += kb) {
-            kb = std::min(fp32_block_depth(n, k), n - k);
+            kb = std::min(fp32_block_depth(n, k, split), n - k);
             if (k + kb >= n) {
                 swaps(k, kb, 0, k, 0, 0);
                 break;
             }
-            const int nk = k + kb, nkb = std::min(fp32_block_depth(n, nk), n - nk);
+            const int nk = k + kb, nkb = std::min(fp32_block_depth(n, nk, split), n - nk);
             // near the end the block-column chain is the critical path: there the
             // next block column goes first on its own (interchanges, block row,
             // update) and the rest is brought up to date while it is factored;
@@ -2444,10 +2460,10 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
             const bool ahead_first = n - nk <= kAheadFirstBelow;
             if (ahead_first) {
                 swaps(k, kb, nk, nk + nkb, 0, 0);
-                block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, nk, nk + nkb, &PB);
+                block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, nk, nk + nkb, split ? &PB : nullptr);
             } else {
                 swaps(k, kb, 0, k, nk, n);
-                block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, nk, n, &PB);
+                block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, nk, n, split ? &PB : nullptr);
             }
             trailing(s, k, kb, nk, nk + nkb);
             BH_CHECK_HIP(hipEventRecord(ready, s));
@@ -2457,7 +2473,7 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
             // the rest, concurrently
             if (ahead_first) {
                 swaps(k, kb, 0, k, nk + nkb, n);
-                block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, nk + nkb, n, &PB, false);
+                block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, nk + nkb, n, split ? &PB : nullptr, false);
             }
             trailing(s, k, kb, nk + nkb, n);
             BH_CHECK_HIP(hipStreamWaitEvent(s, done, 0));
