@@ -787,11 +787,26 @@ struct FarArgs {
     OpParams op;
 };
 
+constexpr int kTileCells = 128;    // max cells per dof-tile side
+
+// The test cells of a dof tile staged in LDS field by field (kTileCells apart), so that
+// the lanes of a wave, which hold consecutive test cells, read consecutive words.
+template <typename R>
+struct RowStage {
+    const double *cen;    // [3][kTileCells] centroid
+    const R *av;          // [9][kTileCells] vertices relative to the centroid
+    const double *area;   // [kTileCells]
+    const R *pts;         // [Qo * 4][kTileCells] points relative to the centroid, weight
+    int rc;               // the test cell's place in the tile
+};
+
 // Far-field 3x3 block of (test p, trial q) by double Gauss, written to z[k*stride].
 // QO, QI: the point counts when fixed at compile time (0: read from A).
 // maybe_near = false: the caller knows the pair is not in p's non-far list.
-template <typename R, int OPK, int QO = 0, int QI = 0, bool SHIFT = false>
-__device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> *z, int stride, bool maybe_near = true)
+// STAGED: the test cell's data come from st (LDS) instead of A's arrays.
+template <typename R, int OPK, int QO = 0, int QI = 0, bool SHIFT = false, bool STAGED = false>
+__device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> *z, int stride, bool maybe_near = true,
+                                          RowStage<R> st = {})
 {
     const int Qo = QO ? QO : A.Qo, Qi = QI ? QI : A.Qi;
     // the test-point loop of the small default rules is unrolled too (independent point
@@ -817,13 +832,13 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
     }
     R dc[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) dc[k] = (R)(tg[kGeoC + k] - sg[kGeoC + k]);
+    for (int k = 0; k < 3; ++k) dc[k] = (R)((STAGED ? st.cen[k * kTileCells + st.rc] : tg[kGeoC + k]) - sg[kGeoC + k]);
     // vertices relative to the centroids are only needed once the moments are
     // summed; loading them late keeps them out of the point loops' registers
     auto load_verts = [&](R *a, R *b) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            a[k] = (R)(tg[k] - tg[kGeoC + k % 3]);
+            a[k] = STAGED ? st.av[k * kTileCells + st.rc] : (R)(tg[k] - tg[kGeoC + k % 3]);
             b[k] = (R)(sg[k] - sg[kGeoC + k % 3]);
         }
     };
@@ -865,14 +880,17 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
     auto rot = [&](cx<R> &v) {
         if constexpr (shifted) v = {cc * v.r + sc * v.i, cc * v.i - sc * v.r};
     };
-    const R *xq = (const R *)A.tq + (size_t)p * Qo * 4;
+    // test point v (coordinates 4 ip .. 4 ip + 2, weight 4 ip + 3)
+    const R *xq = STAGED ? st.pts + st.rc : (const R *)A.tq + (size_t)p * Qo * 4;
+    auto xpt = [&](int v) -> R { return xq[STAGED ? v * kTileCells : v]; };
+    auto t_area = [&]() -> double { return STAGED ? st.area[st.rc] : tg[kGeoA]; };   // read where used
     const R *yq = (const R *)A.sq + (size_t)q * Qi * 4;
     if constexpr (OPK == BH_OP_MW_SINGLELAYER) {
         cx<R> T0{0, 0}, T1{0, 0}, Tx[3] = {{0, 0}, {0, 0}, {0, 0}}, Ty[3] = {{0, 0}, {0, 0}, {0, 0}};
 #pragma unroll kUnrollO
         for (int ip = 0; ip < Qo; ++ip) {
-            const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
-            const R wp = xq[4 * ip + 3];
+            const R x[3] = {xpt(4 * ip), xpt(4 * ip + 1), xpt(4 * ip + 2)};
+            const R wp = xpt(4 * ip + 3);
             const R xd[3] = {x[0] + dc[0], x[1] + dc[1], x[2] + dc[2]};
             cx<R> S0{0, 0}, S1[3] = {{0, 0}, {0, 0}, {0, 0}};
 #pragma unroll
@@ -908,7 +926,7 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         }
         R a[9], b[9];
         load_verts(a, b);
-        const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
+        const R s1 = (R)(1.0 / (4.0 * t_area() * sg[kGeoA]));
         efie_block<R>(A.op, s1, a, b, T0, Tx, Ty, T1, z, stride);
     } else if constexpr (OPK == BH_OP_MW_DOUBLELAYER_ROTATED) {
         // On a flat cell paired with itself n . r = n . g_j = 0 for every point pair, so the
@@ -936,8 +954,8 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         for (int k = 0; k < 9; ++k) zz[k] = {0, 0};
 #pragma unroll kUnrollO
         for (int ip = 0; ip < Qo; ++ip) {
-            const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
-            const R wp = xq[4 * ip + 3];
+            const R x[3] = {xpt(4 * ip), xpt(4 * ip + 1), xpt(4 * ip + 2)};
+            const R wp = xpt(4 * ip + 3);
             const R xd[3] = {x[0] + dc[0], x[1] + dc[1], x[2] + dc[2]};
             cx<R> Rs[3] = {{0, 0}, {0, 0}, {0, 0}}, Ss{0, 0};
             // the trial points are read again for every test point (L1 hits) instead of being
@@ -976,7 +994,7 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
                 }
             }
         }
-        const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
+        const R s1 = (R)(1.0 / (4.0 * t_area() * sg[kGeoA]));
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             rot(zz[k]);
@@ -989,8 +1007,8 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
                         MR[3] = {{0, 0}, {0, 0}, {0, 0}};
 #pragma unroll kUnrollO
         for (int ip = 0; ip < Qo; ++ip) {
-            const R x[3] = {xq[4 * ip], xq[4 * ip + 1], xq[4 * ip + 2]};
-            const R wp = xq[4 * ip + 3];
+            const R x[3] = {xpt(4 * ip), xpt(4 * ip + 1), xpt(4 * ip + 2)};
+            const R wp = xpt(4 * ip + 3);
             const R xd[3] = {x[0] + dc[0], x[1] + dc[1], x[2] + dc[2]};
             cx<R> H0{0, 0}, H1[3] = {{0, 0}, {0, 0}, {0, 0}};
 #pragma unroll
@@ -1037,7 +1055,7 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
         }
         R a[9], b[9];
         load_verts(a, b);
-        const R s1 = (R)(1.0 / (4.0 * tg[kGeoA] * sg[kGeoA]));
+        const R s1 = (R)(1.0 / (4.0 * t_area() * sg[kGeoA]));
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -1058,8 +1076,8 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
 }
 
 constexpr int kTile = 64;          // dofs per tile side
-constexpr int kTileCells = 128;    // max cells per tile side
 constexpr int kPairChunk = 512;    // pair blocks staged in LDS at once (two per thread)
+constexpr int kPairChunkStaged = 256;   // the same when the test cells are staged too (one per thread)
 
 struct TileSide {
     const int *ptr;       // [ntiles+1] into cells
@@ -1081,19 +1099,71 @@ struct TileArgs {
     double scale_r, scale_i;
 };
 
+// v of lane l (l wave-uniform)
+__device__ __forceinline__ double readlane_r(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ float readlane_r(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// LDS of a far tile: the pair blocks of a chunk and, for the double layer's small default
+// rule, the tile's test cells (whose per-lane reads would otherwise touch one cache line
+// per lane), later the row-major transpose, in one buffer within 80 KB (two workgroups
+// per CU).  Staging halves the chunk, and so doubles the chunk barriers; measured on the
+// 26,664-cell sphere it pays for the double layer (78.4 -> 75.7 ms) and not for the
+// single layer (66.2 -> 68.5 ms), whose test cells are read through the cache.
+template <typename R, int OPK, int QO, bool ROWMAJOR>
+struct FarTileLds {
+    static constexpr bool kStage = OPK == BH_OP_MW_DOUBLELAYER && QO > 0 && QO <= 4;
+    static constexpr int kChunk = kStage ? kPairChunkStaged : kPairChunk;
+    static constexpr size_t kBlk = (size_t)kChunk * 9 * sizeof(cx<R>);
+    static constexpr size_t kStageBytes = kStage ? (size_t)kTileCells * (4 * sizeof(double) + (9 + 4 * QO) * sizeof(R)) : 0;
+    static constexpr size_t kT = ROWMAJOR ? (size_t)kTile * (kTile + 1) * sizeof(cx<R>) : 0;
+    static constexpr size_t kBytes = kBlk + kStageBytes > kT ? kBlk + kStageBytes : kT;
+};
+
 template <typename R, int OPK, bool ROWMAJOR, int QO, int QI, bool SHIFT = false>
 __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
 {
+    using L = FarTileLds<R, OPK, QO, ROWMAJOR>;
+    constexpr bool kStage = L::kStage;
+    constexpr int kChunk = L::kChunk;
     __shared__ int s_rc[kTileCells], s_cc[kTileCells];
     __shared__ int s_rslot[kTile][2], s_cslot[kTile][2];
     __shared__ R s_rcoef[kTile][2], s_ccoef[kTile][2];
-    __shared__ cx<R> s_blk[kPairChunk * 9];
+    __shared__ __align__(16) unsigned char s_raw[L::kBytes];
     __shared__ unsigned s_near[kTileCells * kTileCells / 32];   // bit rc * kTileCells + cc: non-far pair
+    cx<R> *s_blk = reinterpret_cast<cx<R> *>(s_raw);
+    double *s_cen = reinterpret_cast<double *>(s_raw + L::kBlk);   // [3][kTileCells], then area
+    double *s_area = s_cen + 3 * kTileCells;
+    R *s_av = reinterpret_cast<R *>(s_area + kTileCells);        // [9][kTileCells], then points
+    R *s_pts = s_av + 9 * kTileCells;
     const int tid = threadIdx.x;
     const int r0 = A.row.ptr[tr], nr = A.row.ptr[tr + 1] - r0;
     const int c0 = A.col.ptr[tc], nc = A.col.ptr[tc + 1] - c0;
     const int mbase = tr * kTile, nbase = tc * kTile;
-    for (int k = tid; k < nr; k += 256) s_rc[k] = A.row.cells[r0 + k];
+    for (int k = tid; k < nr; k += 256) {
+        const int p = A.row.cells[r0 + k];
+        s_rc[k] = p;
+        if constexpr (kStage) {
+            const double *tg = A.far.tgeo + (size_t)p * kGeo;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s_cen[c * kTileCells + k] = tg[kGeoC + c];
+            s_area[k] = tg[kGeoA];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) s_av[v * kTileCells + k] = (R)(tg[v] - tg[kGeoC + v % 3]);
+            const R *tq = (const R *)A.far.tq + (size_t)p * QO * 4;
+#pragma unroll
+            for (int v = 0; v < 4 * QO; ++v) s_pts[v * kTileCells + k] = tq[v];
+        }
+    }
     for (int k = tid; k < nc; k += 256) s_cc[k] = A.col.cells[c0 + k];
     if (tid < 2 * kTile) {
         const int l = tid >> 1, s = tid & 1;
@@ -1136,7 +1206,7 @@ __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
         __syncthreads();
     }
 
-    const int CC = nr > 0 ? max(1, kPairChunk / nr) : 1;
+    const int CC = nr > 0 ? max(1, kChunk / nr) : 1;
     for (int cb = 0; cb < nc; cb += CC) {
         const int cn = min(CC, nc - cb);
         // phase 1: pair blocks of (all row cells) x (column cells cb .. cb+cn)
@@ -1145,31 +1215,38 @@ __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
             const int rc = t % nr, cc = t / nr;
             const int bit = rc * kTileCells + cb + cc;
             const bool near = lists && ((s_near[bit >> 5] >> (bit & 31)) & 1u);
-            far_block<R, OPK, QO, QI, SHIFT>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kPairChunk, near);
+            far_block<R, OPK, QO, QI, SHIFT, kStage>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kChunk, near,
+                                                     RowStage<R>{s_cen, s_av, s_area, s_pts, rc});
         }
         __syncthreads();
-        // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184;
-        // the column dof (and so whether its cell is in this chunk) is uniform per wave
+        // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184.
+        // The wave's 32 column slots (its 16 column dofs, two cells each) are read at once,
+        // lane l < 32 holding slot l, and the slots whose cell lies in this chunk form a
+        // wave-uniform mask, so no slot waits on its own LDS read
+        {
+            const int sl = own & 31, nls = grp * 16 + (sl >> 1);
+            const int cs_l = s_cslot[nls][sl & 1];
+            const R bc_l = s_ccoef[nls][sl & 1];
+            const int lc_l = (cs_l >> 2) - cb;
+            const unsigned hit = (unsigned)__ballot(cs_l >= 0 && lc_l >= 0 && lc_l < cn);   // lanes 0..31
+            const int rs[2] = {s_rslot[own][0], s_rslot[own][1]};
+            const R ac[2] = {s_rcoef[own][0], s_rcoef[own][1]};
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const int nl = grp * 16 + e;
+            for (int e = 0; e < 16; ++e) {
 #pragma unroll
-            for (int sb = 0; sb < 2; ++sb) {
-                const int cslt = s_cslot[nl][sb];
-                if (cslt < 0) continue;
-                const int lc = (cslt >> 2) - cb;
-                if (lc < 0 || lc >= cn) continue;
-                const int j = cslt & 3;
-                const R bc = s_ccoef[nl][sb];
+                for (int sb = 0; sb < 2; ++sb) {
+                    const int slot = 2 * e + sb;
+                    if (!((hit >> slot) & 1u)) continue;
+                    const int cslt = __builtin_amdgcn_readlane(cs_l, slot);
+                    const R bc = readlane_r(bc_l, slot);
+                    const int lc = (cslt >> 2) - cb, j = cslt & 3;
 #pragma unroll
-                for (int sa = 0; sa < 2; ++sa) {
-                    const int rslt = s_rslot[own][sa];
-                    if (rslt < 0) continue;
-                    const int i = rslt & 3;
-                    const cx<R> zz = s_blk[(lc * nr + (rslt >> 2)) + (i + 3 * j) * kPairChunk];
-                    const R ac = s_rcoef[own][sa];
-                    acc[e].r += ac * (zz.r * bc);
-                    acc[e].i += ac * (zz.i * bc);
+                    for (int sa = 0; sa < 2; ++sa) {
+                        if (rs[sa] < 0) continue;
+                        const cx<R> zz = s_blk[(lc * nr + (rs[sa] >> 2)) + ((rs[sa] & 3) + 3 * j) * kChunk];
+                        acc[e].r += ac[sa] * (zz.r * bc);
+                        acc[e].i += ac[sa] * (zz.i * bc);
+                    }
                 }
             }
         }
@@ -1180,7 +1257,7 @@ __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
     cx<R> *Z = (cx<R> *)A.Z;
     const R sr = (R)A.scale_r, si = (R)A.scale_i;
     if constexpr (ROWMAJOR) {
-        static_assert(kTile * (kTile + 1) <= kPairChunk * 9, "transpose does not fit the pair-block space");
+        static_assert(kTile * (kTile + 1) * sizeof(cx<R>) <= L::kBytes, "transpose does not fit the LDS buffer");
         cx<R> *s_t = s_blk;
 #pragma unroll
         for (int e = 0; e < 16; ++e) s_t[own * (kTile + 1) + grp * 16 + e] = acc[e];
