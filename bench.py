@@ -231,6 +231,8 @@ def main():
                     help="skip timing the vendor-library GEMM of the same split-bf16 update")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device copies (calibrates HBM byte counters) and exit")
+    ap.add_argument("--assembly-only", type=int, default=0, metavar="K",
+                    help="only assemble the config's resident matrix K times (for kernel counters) and exit")
     ap.add_argument("--gemm-comparator-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gemm_comparator_only:
@@ -258,6 +260,17 @@ def main():
     mesh, X, t, e, k = build_problem(bj, args.config)
     N, F = X.numfunctions(), mesh.numcells
     log(f"bench: {args.config} F={F} N={N} k={k}")
+    if args.assembly_only > 0:
+        times = []
+        for _ in range(args.assembly_only):
+            ctx.synchronize()
+            ts = time.perf_counter()
+            M = bj.DeviceMatrix.assemble(t, X)
+            ctx.synchronize()
+            times.append(time.perf_counter() - ts)
+            del M
+        print(json.dumps({"assembly_only": {"config": args.config, "dofs": N, "assemble_s": times}}), flush=True)
+        return
     b = bj.assemble(e, X)
 
     # plain FP64 GPU path (also the reference solution for the error), timed like the
