@@ -1097,6 +1097,7 @@ struct TileArgs {
     int rowmajor;
     int accumulate;
     double scale_r, scale_i;
+    const void *rpack;    // the row tiles' test cells packed field by field (single layer, default rule)
 };
 
 // v of lane l (l wave-uniform)
@@ -1122,9 +1123,13 @@ __device__ __forceinline__ float readlane_r(float v, int l)
 template <typename R, int OPK, int QO, bool ROWMAJOR>
 struct FarTileLds {
     static constexpr bool kStage = OPK == BH_OP_MW_DOUBLELAYER && QO > 0 && QO <= 4;
+    // the single layer reads the same field-major layout from a packed copy in global memory
+    // (A.rpack: consecutive lanes read consecutive words), keeping its 512-pair chunks
+    static constexpr bool kPacked = OPK == BH_OP_MW_SINGLELAYER && QO > 0 && QO <= 4;
+    static constexpr size_t kRowBytes = (size_t)kTileCells * (4 * sizeof(double) + (9 + 4 * (QO > 0 ? QO : 1)) * sizeof(R));
     static constexpr int kChunk = kStage ? kPairChunkStaged : kPairChunk;
     static constexpr size_t kBlk = (size_t)kChunk * 9 * sizeof(cx<R>);
-    static constexpr size_t kStageBytes = kStage ? (size_t)kTileCells * (4 * sizeof(double) + (9 + 4 * QO) * sizeof(R)) : 0;
+    static constexpr size_t kStageBytes = kStage ? kRowBytes : 0;
     static constexpr size_t kT = ROWMAJOR ? (size_t)kTile * (kTile + 1) * sizeof(cx<R>) : 0;
     static constexpr size_t kBytes = kBlk + kStageBytes > kT ? kBlk + kStageBytes : kT;
 };
@@ -1215,8 +1220,16 @@ __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
             const int rc = t % nr, cc = t / nr;
             const int bit = rc * kTileCells + cb + cc;
             const bool near = lists && ((s_near[bit >> 5] >> (bit & 31)) & 1u);
-            far_block<R, OPK, QO, QI, SHIFT, kStage>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kChunk, near,
-                                                     RowStage<R>{s_cen, s_av, s_area, s_pts, rc});
+            if constexpr (L::kPacked) {
+                const unsigned char *pk = (const unsigned char *)A.rpack + (size_t)tr * L::kRowBytes;
+                const double *pc = (const double *)pk;
+                const R *pv = (const R *)(pc + 4 * kTileCells);
+                far_block<R, OPK, QO, QI, SHIFT, true>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kChunk, near,
+                                                       RowStage<R>{pc, pv, pc + 3 * kTileCells, pv + 9 * kTileCells, rc});
+            } else {
+                far_block<R, OPK, QO, QI, SHIFT, kStage>(A.far, s_rc[rc], s_cc[cb + cc], s_blk + t, kChunk, near,
+                                                         RowStage<R>{s_cen, s_av, s_area, s_pts, rc});
+            }
         }
         __syncthreads();
         // phase 2: signed gather, store(a * (z * b), m, n) order of integralop.jl:173-184.
@@ -1279,6 +1292,27 @@ __device__ void far_tiles_dev(const TileArgs &A, int tr, int tc)
             if (A.Z32) ((float2 *)A.Z32)[(size_t)m * A.ld32 + n] = float2{(float)v.r, (float)v.i};
         }
     }
+}
+
+// The test cells of every row tile, packed in the layout of RowStage: per tile, centroid
+// [3][kTileCells], area [kTileCells] (doubles), then vertices relative to the centroid
+// [9][kTileCells] and points [4 Qo][kTileCells] in R.  Block = one row tile.
+template <typename R>
+__global__ __launch_bounds__(kTileCells) void k_pack_rows(const int *__restrict__ ptr, const int *__restrict__ cells,
+                                                          const double *__restrict__ geo, const R *__restrict__ tq,
+                                                          int Qo, unsigned char *pack, size_t tile_bytes)
+{
+    const int tile = blockIdx.x, k = threadIdx.x;
+    const int r0 = ptr[tile], nr = ptr[tile + 1] - r0;
+    if (k >= nr) return;
+    const int p = cells[r0 + k];
+    double *pc = (double *)(pack + (size_t)tile * tile_bytes);
+    R *pv = (R *)(pc + 4 * kTileCells);
+    const double *g = geo + (size_t)p * kGeo;
+    for (int c = 0; c < 3; ++c) pc[c * kTileCells + k] = g[kGeoC + c];
+    pc[3 * kTileCells + k] = g[kGeoA];
+    for (int v = 0; v < 9; ++v) pv[v * kTileCells + k] = (R)(g[v] - g[kGeoC + v % 3]);
+    for (int v = 0; v < 4 * Qo; ++v) pv[(9 + v) * kTileCells + k] = tq[(size_t)p * Qo * 4 + v];
 }
 
 // workgroups per CU the compiler must fit: two for n x K (see far_block), free otherwise
@@ -1673,9 +1707,21 @@ void make_plan(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, Mesh &tm
     F.op = P.op;
 }
 
+// The row tiles' test cells packed for the single layer's default-rule tiles (k_pack_rows).
+template <typename R>
+const void *pack_rows(hipStream_t s, const Space &X, const FarArgs &F, DevBuf<char> &buf)
+{
+    const size_t tb = (size_t)kTileCells * (4 * sizeof(double) + (9 + 4 * F.Qo) * sizeof(R));
+    buf.ensure(tb * (size_t)X.ntiles);
+    k_pack_rows<R><<<(unsigned)X.ntiles, kTileCells, 0, s>>>(X.t_ptr.p, X.t_cells.p, F.tgeo, (const R *)F.tq, F.Qo,
+                                                             (unsigned char *)buf.p, tb);
+    check_launch("k_pack_rows");
+    return buf.p;
+}
+
 template <typename R>
 void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long long ld, bool rowmajor, bool accumulate,
-                  double sr, double si, void *Z32 = nullptr, long long ld32 = 0)
+                  double sr, double si, DevBuf<char> &rpack, void *Z32 = nullptr, long long ld32 = 0)
 {
     TileArgs A{};
     A.far = P.far;
@@ -1701,6 +1747,7 @@ void launch_tiles(Ctx &ctx, const Plan &P, Space &X, Space &Y, void *Z, long lon
     const bool dflt = P.far.Qo == 3 && P.far.Qi == 4;
     // the centroid-shifted phases (FP64, lossless, small cells) for the default rules
     const bool shift = sizeof(R) == 8 && dflt && P.far.shifted;
+    if (sl && dflt) A.rpack = pack_rows<R>(ctx.stream, X, P.far, rpack);
 #define BH_TILES(OPK, RM)                                                                    \
     do {                                                                                     \
         if (shift) k_far_tiles<R, OPK, RM, 3, 4, true><<<grid, 256, 0, ctx.stream>>>(A);    \
@@ -1844,8 +1891,9 @@ bool assemble_device(Ctx &ctx, const bh_operator *op, const bh_quadstrat *qs, bo
     build_tiles(Y, ctx.stream);
     if (X.ndofs == 0 || Y.ndofs == 0) return false;
     const bool copy32 = dZ32 != nullptr && !fp32 && rowmajor;
-    if (fp32) launch_tiles<float>(ctx, P, X, Y, dZ, ld, rowmajor, accumulate, sr, si);
-    else launch_tiles<double>(ctx, P, X, Y, dZ, ld, rowmajor, accumulate, sr, si, copy32 ? dZ32 : nullptr, ld32);
+    DevBuf<char> rpack;   // held until the launch has ended (synchronised below)
+    if (fp32) launch_tiles<float>(ctx, P, X, Y, dZ, ld, rowmajor, accumulate, sr, si, rpack);
+    else launch_tiles<double>(ctx, P, X, Y, dZ, ld, rowmajor, accumulate, sr, si, rpack, copy32 ? dZ32 : nullptr, ld32);
     BH_CHECK_HIP(hipStreamSynchronize(ctx.stream));
     return copy32;
 }
@@ -2148,6 +2196,10 @@ void batch_far(Ctx &ctx, std::vector<BatchItem> &items)
             A.accumulate = 0;
             A.scale_r = 1.0;
             A.scale_i = 0.0;
+            if (g.opk == BH_OP_MW_SINGLELAYER && (g.variant == 1 || g.variant == 2)) {
+                keep.emplace_back();
+                A.rpack = pack_rows<double>(s, X, it.far, keep.back());
+            }
             BH_REQUIRE((long long)first[k] + X.ntiles * X.ntiles < (1LL << 31), "batch has too many dof tiles");
             first[k + 1] = first[k] + (int)(X.ntiles * X.ntiles);
             const double nf = (double)X.mesh->nf;
