@@ -118,14 +118,35 @@ __global__ __launch_bounds__(256) void k_classify_fill(const double *__restrict_
     classify_fill_dev(tgeo, sgeo, nfs, k2, mode, offsets, entries, blockIdx.x);
 }
 
+// Slot of this lane in list l (active lanes only): one atomic per distinct list of the
+// wave, the lanes of a list ranked by lane (the lists' order is not significant: each
+// entry's block is computed on its own and written to the entry's place)
+__device__ __forceinline__ int list_slot(int *counters, int l, bool active)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(active);
+    int slot = -1;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int lv = __shfl(l, leader, 64);
+        const unsigned long long m = __ballot(active && l == lv);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counters[lv], __popcll(m));
+        base = __shfl(base, leader, 64);
+        if (active && l == lv) slot = base + __popcll(m & ((1ull << lane) - 1ull));
+        todo &= ~m;
+    }
+    return slot;
+}
+
 __global__ void k_class_lists(const int4 *__restrict__ entries, int n, int *__restrict__ counters,
                               int *__restrict__ lists, int cap)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const int c = entries[e].z;   // 1..4
-    const int slot = atomicAdd(&counters[c], 1);
-    lists[(size_t)c * cap + slot] = e;
+    const bool active = e < n;
+    const int c = active ? entries[e].z : 0;   // 1..4
+    const int slot = list_slot(counters, c, active);
+    if (active) lists[(size_t)c * cap + slot] = e;
 }
 
 // Grid-restricted classification.  A pair without touching vertices is near
@@ -1948,11 +1969,15 @@ __global__ void k_class_lists_batch(const BatchNear *reqs, const int *first, int
                                     int2 *lists)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const int r = batch_find(first, n, g), e = g - first[r];
-    const int l = 5 * reqs[r].op.kind + reqs[r].entries[e].z;   // class 1..4
-    const int slot = atomicAdd(&counters[l], 1);
-    lists[(size_t)l * total + slot] = make_int2(r, e);
+    const bool active = g < total;
+    int r = 0, e = 0, l = 0;
+    if (active) {
+        r = batch_find(first, n, g);
+        e = g - first[r];
+        l = 5 * reqs[r].op.kind + reqs[r].entries[e].z;   // class 1..4
+    }
+    const int slot = list_slot(counters, l, active);
+    if (active) lists[(size_t)l * total + slot] = make_int2(r, e);
 }
 
 template <int OPK>
