@@ -319,14 +319,15 @@ __device__ __forceinline__ cx<R> green(R gr, R gi, R dist, R idist)
     return {c * amp, -s * amp};
 }
 
-// Block from EFIE moments: z_ij = s1 [alpha (Mxy - a_i.My - b_j.Mx + (a_i.b_j) M0) + 4 beta M0],
-// with a_i.My and b_j.Mx formed once per i and per j, and s1 folded into alpha and beta
+// Block from EFIE moments: z_ij = ph s1 [alpha (Mxy - a_i.My - b_j.Mx + (a_i.b_j) M0) + 4 beta M0],
+// with a_i.My and b_j.Mx formed once per i and per j, and s1 and the common phase ph of
+// all moments folded into alpha and beta
 template <typename R>
 __device__ __forceinline__ void efie_block(const OpParams &op, R s1, const R *a, const R *b, cx<R> M0, const cx<R> *Mx,
-                                          const cx<R> *My, cx<R> Mxy, cx<R> *z, int stride)
+                                          const cx<R> *My, cx<R> Mxy, cx<R> *z, int stride, cx<R> ph = {1, 0})
 {
-    const cx<R> sa{s1 * (R)op.ar, s1 * (R)op.ai};
-    const cx<R> bM0 = cmul(cx<R>{(R)op.br, (R)op.bi}, M0);
+    const cx<R> sa = cmul(ph, cx<R>{s1 * (R)op.ar, s1 * (R)op.ai});
+    const cx<R> bM0 = cmul(cmul(cx<R>{(R)op.br, (R)op.bi}, M0), ph);
     const cx<R> c4{(R)4 * s1 * bM0.r, (R)4 * s1 * bM0.i};
     cx<R> AY[3], BX[3];
 #pragma unroll
@@ -938,17 +939,11 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
             T1.r += wp * (x[0] * S1[0].r + x[1] * S1[1].r + x[2] * S1[2].r);
             T1.i += wp * (x[0] * S1[0].i + x[1] * S1[1].i + x[2] * S1[2].i);
         }
-        rot(T0);
-        rot(T1);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            rot(Tx[k]);
-            rot(Ty[k]);
-        }
         R a[9], b[9];
         load_verts(a, b);
         const R s1 = (R)(1.0 / (4.0 * t_area() * sg[kGeoA]));
-        efie_block<R>(A.op, s1, a, b, T0, Tx, Ty, T1, z, stride);
+        // the moments' common factor exp(-i gi Rc) (shifted phases) enters with alpha and beta
+        efie_block<R>(A.op, s1, a, b, T0, Tx, Ty, T1, z, stride, cx<R>{cc, -sc});
     } else if constexpr (OPK == BH_OP_MW_DOUBLELAYER_ROTATED) {
         // On a flat cell paired with itself n . r = n . g_j = 0 for every point pair, so the
         // block is exactly zero; in FP32 the rounding of those dot products times the 1/R^2
