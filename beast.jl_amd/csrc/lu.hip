@@ -1193,7 +1193,10 @@ __global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__r
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kSplitBK = 32;
-constexpr int kSplitPad = 8;   // bf16 padding per LDS row
+// LDS rows of a stage are unpadded (64 B); the 16-byte k group g of row r is stored at
+// g ^ swz(r), which keeps both the staging writes (whole rows) and the fragment reads
+// (16 rows, one k group) free of bank conflicts with 32 or 64 banks
+__device__ __forceinline__ int split_swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
 __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo)
 {
@@ -1266,8 +1269,8 @@ __device__ __forceinline__ void ntstore2(float2 *p, float2 v)
 }
 
 // C[M x N] -= A B from split planes PA[4][K/32][M][32], PB[4][K/32][N][32]; K a multiple of 32.
-// WM x WN waves, wave tile (16 TM) x (16 TN) complex.  Two LDS stages (160 KB for the
-// 128 x 128 tile): the global loads of step k+1 are in flight while step k's products run
+// WM x WN waves, wave tile (16 TM) x (16 TN) complex.  Two LDS stages (128 KB for the
+// 128 x 128 tile, unpadded rows with swizzled k groups): the global loads of step k+1 are in flight while step k's products run
 // from the other stage, one barrier per step; the first row block of the C tile is read
 // during the last step's products.
 template <int TM, int TN, int WM, int WN>
@@ -1275,7 +1278,7 @@ __device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA,
                                                   float2 *C, long long ldc, int M, int N, int K, int tile)
 {
     constexpr int NT = 64 * WM * WN;
-    constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN, BK = kSplitBK, RS = BK + kSplitPad;
+    constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN, BK = kSplitBK, RS = BK;
     __shared__ __align__(16) __bf16 As[2][4][BM][RS];
     __shared__ __align__(16) __bf16 Bs[2][4][BN][RS];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1316,7 +1319,7 @@ __device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA,
             const int c = tid + NT * q;
             if (c < ACH) {
                 const int p = c / (BM * 4), rem = c % (BM * 4), r = rem / 4, g = rem % 4;
-                *(uint4 *)&As[st][p][r][8 * g] = ra[q];
+                *(uint4 *)&As[st][p][r][8 * (g ^ split_swz(r))] = ra[q];
             }
         }
 #pragma unroll
@@ -1324,7 +1327,7 @@ __device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA,
             const int c = tid + NT * q;
             if (c < BCH) {
                 const int p = c / (BN * 4), rem = c % (BN * 4), r = rem / 4, g = rem % 4;
-                *(uint4 *)&Bs[st][p][r][8 * g] = rb[q];
+                *(uint4 *)&Bs[st][p][r][8 * (g ^ split_swz(r))] = rb[q];
             }
         }
     };
@@ -1338,18 +1341,19 @@ __device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA,
             ari[a][b] = f32x4{0, 0, 0, 0};
         }
     const int li = lane & 15, lg = lane >> 4;   // row/col within a 16-block, k group of 8
+    const int kofs = 8 * (lg ^ split_swz(li));    // the k group's place in LDS (rows start at multiples of 16)
     // products of one k-step from LDS stage st
     auto compute = [&](int st) {
         bf16x8 afr[TM][4];
 #pragma unroll
         for (int a = 0; a < TM; ++a)
 #pragma unroll
-            for (int p = 0; p < 4; ++p) afr[a][p] = *(const bf16x8 *)&As[st][p][wm * 16 * TM + a * 16 + li][8 * lg];
+            for (int p = 0; p < 4; ++p) afr[a][p] = *(const bf16x8 *)&As[st][p][wm * 16 * TM + a * 16 + li][kofs];
 #pragma unroll
         for (int b = 0; b < TN; ++b) {
             bf16x8 bfr[4];
 #pragma unroll
-            for (int p = 0; p < 4; ++p) bfr[p] = *(const bf16x8 *)&Bs[st][p][wn * 16 * TN + b * 16 + li][8 * lg];
+            for (int p = 0; p < 4; ++p) bfr[p] = *(const bf16x8 *)&Bs[st][p][wn * 16 * TN + b * 16 + li][kofs];
 #pragma unroll
             for (int a = 0; a < TM; ++a) {
                 arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[0], arr[a][b], 0, 0, 0);
