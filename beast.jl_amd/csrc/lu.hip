@@ -1192,6 +1192,7 @@ __global__ __launch_bounds__(256, WPE) void k_cgemm(const typename CT<T>::t *__r
 // contiguous, so the GEMM streams bf16 through LDS and does no conversions.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kSplitBK = 32;
 // LDS rows of a stage are unpadded (64 B); the 16-byte k group g of row r is stored at
 // g ^ swz(r), which keeps both the staging writes (whole rows) and the fragment reads
@@ -1331,13 +1332,14 @@ __device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA,
             }
         }
     };
-    f32x4 arr[TM][TN], aii[TM][TN], ari[TM][TN];
+    // real and imaginary parts of the tile; the real part takes Re A Re B and
+    // Im A (-Im B), the B fragments' imaginary planes negated by their sign bits
+    f32x4 arr[TM][TN], ari[TM][TN];
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
         for (int b = 0; b < TN; ++b) {
             arr[a][b] = f32x4{0, 0, 0, 0};
-            aii[a][b] = f32x4{0, 0, 0, 0};
             ari[a][b] = f32x4{0, 0, 0, 0};
         }
     const int li = lane & 15, lg = lane >> 4;   // row/col within a 16-block, k group of 8
@@ -1354,19 +1356,21 @@ __device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA,
             bf16x8 bfr[4];
 #pragma unroll
             for (int p = 0; p < 4; ++p) bfr[p] = *(const bf16x8 *)&Bs[st][p][wn * 16 * TN + b * 16 + li][kofs];
+            const bf16x8 nb2 = __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, bfr[2]) ^ 0x80008000u);
+            const bf16x8 nb3 = __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, bfr[3]) ^ 0x80008000u);
 #pragma unroll
             for (int a = 0; a < TM; ++a) {
                 arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[0], arr[a][b], 0, 0, 0);
-                aii[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], bfr[2], aii[a][b], 0, 0, 0);
                 ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[2], ari[a][b], 0, 0, 0);
+                arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], nb2, arr[a][b], 0, 0, 0);
                 ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], bfr[0], ari[a][b], 0, 0, 0);
                 arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[1], arr[a][b], 0, 0, 0);
-                aii[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], bfr[3], aii[a][b], 0, 0, 0);
                 ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][0], bfr[3], ari[a][b], 0, 0, 0);
+                arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], nb3, arr[a][b], 0, 0, 0);
                 ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][2], bfr[1], ari[a][b], 0, 0, 0);
                 arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][1], bfr[0], arr[a][b], 0, 0, 0);
-                aii[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][3], bfr[2], aii[a][b], 0, 0, 0);
                 ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][1], bfr[2], ari[a][b], 0, 0, 0);
+                arr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][3], nb2, arr[a][b], 0, 0, 0);
                 ari[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[a][3], bfr[0], ari[a][b], 0, 0, 0);
             }
         }
@@ -1421,7 +1425,7 @@ __device__ __forceinline__ void cgemm_bf16x3_body(const __bf16 *__restrict__ PA,
                 cidx(a, b, r, gi, gj);
                 if (gi < M && gj < N) {
                     float2 c = cv[a][b][r];
-                    c.x -= arr[a][b][r] - aii[a][b][r];
+                    c.x -= arr[a][b][r];
                     c.y -= ari[a][b][r];
                     ntstore2(C + gi * ldc + gj, c);
                 }
