@@ -992,11 +992,17 @@ __device__ __forceinline__ void far_block(const FarArgs &A, int p, int q, cx<R> 
                 cx<R> h = cmul(cx<R>{gr + idist, gi}, G);
                 const R f = -idist * wq;
                 h.r *= f; h.i *= f;
-                const R nr = dot3(nrm, r);
-                Ss.r += h.r * nr; Ss.i += h.i * nr;
+                if constexpr (sizeof(R) == 4) {   // FP64 forms Ss = n . Rs after the loop
+                    const R nr = dot3(nrm, r);
+                    Ss.r += h.r * nr; Ss.i += h.i * nr;
+                }
 #pragma unroll
                 for (int k = 0; k < 3; ++k) { Rs[k].r += h.r * r[k]; Rs[k].i += h.i * r[k]; }
             }
+            // sum w h (n . r) = n . sum w h r: in FP64 the dot product with the normal is taken once
+            // per test point (in FP32 per point pair, where the rounding of n . r on flat cells
+            // stays at the level the coincident-cell case above zeroes)
+            if constexpr (sizeof(R) == 8) Ss = {dot3c_r(nrm, Rs), dot3c_i(nrm, Rs)};
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const R v[3] = {xd[0] - b[3 * j], xd[1] - b[3 * j + 1], xd[2] - b[3 * j + 2]};
