@@ -191,7 +191,9 @@ __global__ __launch_bounds__(256) void k_panel(typename CT<T>::t *A, long long l
             w.slot_row[par * G + g] = brow;
         }
         if (brow != INT_MAX && tid < kPW) w.cand[((size_t)par * G + g) * kPW + tid] = L[(brow - r0) * kPW + tid];
-        if (c >= r0 && c < r1 && tid >= 64 && tid < 64 + kPW) w.diag[par * kPW + tid - 64] = L[(c - r0) * kPW + tid - 64];
+        // wave 0 (lanes 32 .. 63) also writes the diagonal row: the barrier's release fence
+        // is thread 0's and orders only its own wave's stores
+        if (c >= r0 && c < r1 && tid >= 32 && tid < 32 + kPW) w.diag[par * kPW + tid - 32] = L[(c - r0) * kPW + tid - 32];
         grid_barrier(w.bar, G, gen);
         // global winner over the G candidates
         best = (T)-1;
@@ -504,18 +506,29 @@ __device__ __forceinline__ void tslu_panel_body(float2 *A, long long lda, int n,
     for (;;) {
         block_gepp(v, rowid < 0, pw, piv, s_key, s_row);
         if (nl == 1) break;
-        // publish this set: the owner of pivot j writes its row id and entries
+        // publish this set: the owner of pivot j stages its entries in LDS (s_u is free
+        // after the first stage), and wave 0 writes the row ids and entries, so that
+        // thread 0's release fence covers every store of the set (a barrier does not wait
+        // for other waves' stores, and a fence orders its own wave's)
         const size_t set = (size_t)(off + g);
         int j_of = -1;
         for (int j = 0; j < pw; ++j)
             if (piv[j] == tid) j_of = j;
         if (j_of >= 0) {
-            tw.idx[set * kPW + j_of] = rowid;
 #pragma unroll
-            for (int cc = 0; cc < kPW; ++cc) tw.val[(set * kPW + j_of) * kPW + cc] = orig[cc];
+            for (int cc = 0; cc < kPW; ++cc) s_u[j_of][cc] = orig[cc];
         }
-        if (tid < kPW && (tid >= pw || piv[tid] < 0)) tw.idx[set * kPW + tid] = -1;
         __syncthreads();
+        if (tid < 64) {
+            if (tid < kPW) {
+                const int p = tid < pw ? piv[tid] : -1;
+                tw.idx[set * kPW + tid] = p >= 0 ? s_rowid[p] : -1;
+            }
+            for (int e = tid; e < kPW * kPW; e += 64) {
+                const int j = e / kPW;
+                if (j < pw && piv[j] >= 0) tw.val[set * kPW * kPW + e] = s_u[j][e % kPW];
+            }
+        }
         if (tid == 0) {
             const int grp = g / kTsluFan, members = min(kTsluFan, nl - grp * kTsluFan);
             __threadfence();
@@ -1641,71 +1654,89 @@ __global__ __launch_bounds__(kIB * kTrinvParts) void k_trinv(const float2 *A, lo
 }
 
 // out[i] = (SUB ? out[i] : 0) - / + sum_c Mx[i][c] xin[c] for rows i < rs, columns
-// c < cs (xin, out: LDS, row-major x nrhs; they may alias).  256 threads: thread t
-// takes row t / 2 and half t % 2 of the columns, whose 64 entries it loads into
-// registers at once (one memory round trip per block instead of one per entry).
-constexpr int kBlkHalf = kIB / 2;
+// c < cs (xin, out: distinct LDS vectors, row-major x nrhs).  The kTrsvT threads of a
+// workgroup hold the 128 x 128 block in registers, loaded at once (one memory round
+// trip per block instead of one per entry): thread t (wave w, lane l) takes the rows
+// (kTrsvT / 8) p + 8 w + l / 8 (p < kBlkRows) and of each the 16-byte chunks
+// l % 8 + 8 j (j < 8), i.e. the columns 2 (l % 8 + 8 j) and the one after.  Eight
+// lanes read one 128-byte line of a row, so each load of a wave touches 8 cache
+// lines (a layout with a row per lane touches 64, and the vector memory pipeline
+// then takes four times as long over the block); the eight lanes of a row then sum
+// their parts by DPP within the half-row.
+constexpr int kTrsvT = 512, kBlkRowStep = kTrsvT / 8, kBlkRows = kIB / kBlkRowStep, kBlkChunks = 8;
+static_assert(kBlkRows * kBlkRowStep == kIB, "rows of a block split evenly over the threads");
+using BlkRegs = float4[kBlkRows][kBlkChunks];
 
-__device__ __forceinline__ void blk_load(const float2 *Mx, long long ld, int rs, int cs, float2 (&a)[kBlkHalf])
+__device__ __forceinline__ void blk_load(const float2 *Mx, long long ld, int rs, int cs, BlkRegs &a)
 {
-    constexpr int H = kBlkHalf;
-    const int t = threadIdx.x, i = t >> 1, h = t & 1;
-    const float2 *row = Mx + (long long)min(i, max(rs - 1, 0)) * ld + h * H;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, rl = 8 * w + (l >> 3), c8 = l & 7;
     if (rs == kIB && cs == kIB) {
-        const float4 *row4 = (const float4 *)row;   // 16-byte aligned: ld and block origins are multiples of 64
 #pragma unroll
-        for (int c = 0; c < H / 2; ++c) {
-            const float4 v = row4[c];
-            a[2 * c] = {v.x, v.y};
-            a[2 * c + 1] = {v.z, v.w};
+        for (int p = 0; p < kBlkRows; ++p) {
+            const float4 *row4 = (const float4 *)(Mx + (long long)(kBlkRowStep * p + rl) * ld);   // 16-byte aligned: ld and block origins are multiples of 64
+#pragma unroll
+            for (int j = 0; j < kBlkChunks; ++j) a[p][j] = row4[c8 + 8 * j];
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < H; ++c) a[c] = (i < rs && h * H + c < cs) ? row[c] : float2{0, 0};
+        for (int p = 0; p < kBlkRows; ++p) {
+            const int r = kBlkRowStep * p + rl;
+            const float2 *row = Mx + (long long)min(r, max(rs - 1, 0)) * ld;
+#pragma unroll
+            for (int j = 0; j < kBlkChunks; ++j) {
+                const int c = 2 * (c8 + 8 * j);
+                const float2 u = (r < rs && c < cs) ? row[c] : float2{0, 0};
+                const float2 v = (r < rs && c + 1 < cs) ? row[c + 1] : float2{0, 0};
+                a[p][j] = float4{u.x, u.y, v.x, v.y};
+            }
+        }
     }
 }
 
-template <bool SUB, int NRHS>
-__device__ __forceinline__ void blk_apply(const float2 (&a)[kBlkHalf], int rs, const float2 *xin, float2 *out)
+// Sum over the eight lanes 8 m .. 8 m + 7 of a value: DPP quad_perm 1,0,3,2 and
+// 2,3,0,1 make each quad uniform, then the half-row mirror adds the other quad.
+__device__ __forceinline__ float sum8_lanes(float v)
 {
-    constexpr int H = kBlkHalf;
-    const int t = threadIdx.x, i = t >> 1, h = t & 1;
-    float2 acc[NRHS];
-#pragma unroll
-    for (int r = 0; r < NRHS; ++r) acc[r] = {0, 0};
-#pragma unroll
-    for (int c = 0; c < H; ++c) {
-#pragma unroll
-        for (int r = 0; r < NRHS; ++r) {
-            const float2 x = xin[(h * H + c) * NRHS + r];
-            acc[r].x += a[c].x * x.x - a[c].y * x.y;
-            acc[r].y += a[c].x * x.y + a[c].y * x.x;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < NRHS; ++r) {
-        acc[r].x += __shfl_xor(acc[r].x, 1, 64);
-        acc[r].y += __shfl_xor(acc[r].y, 1, 64);
-    }
-    __syncthreads();
-    if (i < rs && h == 0) {
-#pragma unroll
-        for (int r = 0; r < NRHS; ++r) {
-            float2 v = acc[r];
-            if (SUB) v = {out[i * NRHS + r].x - v.x, out[i * NRHS + r].y - v.y};
-            out[i * NRHS + r] = v;
-        }
-    }
-    __syncthreads();
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v), __builtin_bit_cast(int, v),
+                                                               0xB1, 0xF, 0xF, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v), __builtin_bit_cast(int, v),
+                                                               0x4E, 0xF, 0xF, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v), __builtin_bit_cast(int, v),
+                                                               0x141, 0xF, 0xF, false));
+    return v;
 }
 
 template <bool SUB, int NRHS>
-__device__ __forceinline__ void blk_matvec(const float2 *Mx, long long ld, int rs, int cs, const float2 *xin,
-                                           float2 *out)
+__device__ __forceinline__ void blk_apply(const BlkRegs &a, int rs, const float2 *xin, float2 *out)
 {
-    float2 a[kBlkHalf];
-    blk_load(Mx, ld, rs, cs, a);
-    blk_apply<SUB, NRHS>(a, rs, xin, out);
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, rl = 8 * w + (l >> 3), c8 = l & 7;
+#pragma unroll
+    for (int q = 0; q < NRHS; ++q) {
+        float2 acc[kBlkRows];
+#pragma unroll
+        for (int p = 0; p < kBlkRows; ++p) acc[p] = {0, 0};
+#pragma unroll
+        for (int j = 0; j < kBlkChunks; ++j) {
+            const int c = 2 * (c8 + 8 * j);
+            const float2 x0 = xin[c * NRHS + q], x1 = xin[(c + 1) * NRHS + q];
+#pragma unroll
+            for (int p = 0; p < kBlkRows; ++p) {   // explicit FMAs: the same bits for every NRHS
+                const float4 m = a[p][j];
+                acc[p].x = fmaf(-m.w, x1.y, fmaf(m.z, x1.x, fmaf(-m.y, x0.y, fmaf(m.x, x0.x, acc[p].x))));
+                acc[p].y = fmaf(m.w, x1.x, fmaf(m.z, x1.y, fmaf(m.y, x0.x, fmaf(m.x, x0.y, acc[p].y))));
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < kBlkRows; ++p) {
+            const float2 v{sum8_lanes(acc[p].x), sum8_lanes(acc[p].y)};
+            const int i = kBlkRowStep * p + rl;
+            if (c8 == 0 && i < rs) {
+                float2 &o = out[i * NRHS + q];
+                o = SUB ? float2{o.x - v.x, o.y - v.y} : v;
+            }
+        }
+    }
+    __syncthreads();
 }
 
 // G[i] = Dinv[i] M[i][i -/+ 1] for the blocks with a neighbour (LOWER: the
@@ -1770,11 +1801,14 @@ __global__ __launch_bounds__(256) void k_blk_coupling(const float2 *A, long long
 constexpr long long kTrsvSpin = 1LL << 24;
 
 template <bool LOWER, int NRHS>
-__global__ __launch_bounds__(256) void k_trsv_flag(const float2 *A, long long lda, int n, const float2 *Dinv,
-                                                   const float2 *G, float2 *X, int *sync, int ticket0, int epoch)
+__global__ __launch_bounds__(kTrsvT) void k_trsv_flag(const float2 *A, long long lda, int n, const float2 *Dinv,
+                                                      const float2 *G, float2 *X, int *sync, int ticket0, int epoch)
 {
     __shared__ float2 xs[kIB * NRHS];
     __shared__ float2 xb[kIB * NRHS];
+    // the inverted diagonal block, each thread's share at a place of its own (read back
+    // by the thread that wrote it: no barrier, no bank conflicts)
+    __shared__ float4 dsh[kBlkRows][kBlkChunks][kTrsvT];
     __shared__ int s_idx, s_ready;
     int *counter = sync, *err = sync + 1, *flag = sync + 2;
     const int nblk = (n + kIB - 1) / kIB;
@@ -1785,6 +1819,7 @@ __global__ __launch_bounds__(256) void k_trsv_flag(const float2 *A, long long ld
     if (idx < 0 || idx >= nblk) return;   // cannot happen: one ticket per workgroup
     const int blk = LOWER ? idx : nblk - 1 - idx;
     const int r0 = blk * kIB, rs = min(kIB, n - r0);
+    const int jadj = LOWER ? blk - 1 : blk + 1, cadj = jadj * kIB, csadj = min(kIB, n - cadj);
     auto wait_flag = [&](int j) {   // thread 0
         long long it = 0;
         while (__hip_atomic_load(flag + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
@@ -1795,11 +1830,27 @@ __global__ __launch_bounds__(256) void k_trsv_flag(const float2 *A, long long ld
             __builtin_amdgcn_s_sleep(1);
         }
     };
+    // the block's own operands, held for the whole solve: the inverted diagonal block
+    // (LDS) and the coupling G to the adjacent block (registers), so that neither is
+    // loaded after a wait
+    BlkRegs a, g;
+    blk_load(Dinv + (size_t)blk * kIB * kIB, kIB, rs, rs, a);
+    if (idx > 0) blk_load(G + (size_t)blk * kIB * kIB, kIB, rs, csadj, g);
+#pragma unroll
+    for (int p = 0; p < kBlkRows; ++p)
+#pragma unroll
+        for (int j = 0; j < kBlkChunks; ++j) dsh[p][j][threadIdx.x] = a[p][j];
     for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x) xs[t] = t < rs * NRHS ? X[(long long)r0 * NRHS + t] : float2{0, 0};
     __syncthreads();
-    // every dependency but the adjacent block, taken as their ready flags appear
+    // every dependency but the adjacent block, taken as their ready flags appear; the
+    // next block of L (or U) is loaded before its flag is waited for
     const int early = max(idx - 1, 0);
     int done = 0;
+    auto load_early = [&](int d) {
+        const int c0 = (LOWER ? d : nblk - 1 - d) * kIB;
+        blk_load(A + (long long)r0 * lda + c0, lda, rs, min(kIB, n - c0), a);
+    };
+    if (early > 0) load_early(0);
     while (done < early) {
         if (threadIdx.x == 0) {
             wait_flag(LOWER ? done : nblk - 1 - done);
@@ -1820,29 +1871,31 @@ __global__ __launch_bounds__(256) void k_trsv_flag(const float2 *A, long long ld
             for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x)
                 xb[t] = t < cs * NRHS ? X[(long long)c0 * NRHS + t] : float2{0, 0};
             __syncthreads();
-            blk_matvec<true, NRHS>(A + (long long)r0 * lda + c0, lda, rs, cs, xb, xs);
+            blk_apply<true, NRHS>(a, rs, xb, xs);
+            if (done + 1 < early) load_early(done + 1);
         }
     }
-    // y = Dinv x without the adjacent block; then x = y - G x_adjacent, with G in
-    // registers before the wait so only the product follows it
-    blk_matvec<false, NRHS>(Dinv + (size_t)blk * kIB * kIB, kIB, rs, rs, xs, xs);
+    // y = Dinv x without the adjacent block (into xb); then x = y - G x_adjacent
+#pragma unroll
+    for (int p = 0; p < kBlkRows; ++p)
+#pragma unroll
+        for (int j = 0; j < kBlkChunks; ++j) a[p][j] = dsh[p][j][threadIdx.x];
+    blk_apply<false, NRHS>(a, rs, xs, xb);
     if (idx > 0) {
-        const int j = LOWER ? blk - 1 : blk + 1;
-        const int c0 = j * kIB, cs = min(kIB, n - c0);
-        float2 a[kBlkHalf];
-        blk_load(G + (size_t)blk * kIB * kIB, kIB, rs, cs, a);
         if (threadIdx.x == 0) {
-            wait_flag(j);
+            wait_flag(jadj);
             __threadfence();   // acquire: the adjacent block's rows
         }
         __syncthreads();
         for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x)
-            xb[t] = t < cs * NRHS ? X[(long long)c0 * NRHS + t] : float2{0, 0};
+            xs[t] = t < csadj * NRHS ? X[(long long)cadj * NRHS + t] : float2{0, 0};
         __syncthreads();
-        blk_apply<true, NRHS>(a, rs, xb, xs);
+        blk_apply<true, NRHS>(g, rs, xs, xb);
     }
-    for (int t = threadIdx.x; t < rs * NRHS; t += blockDim.x) X[(long long)r0 * NRHS + t] = xs[t];
-    __syncthreads();
+    // wave 0 writes the rows, so that thread 0's release fence covers every store (a
+    // barrier does not wait for other waves' stores, and a fence orders its own wave's)
+    if (threadIdx.x < 64)
+        for (int t = threadIdx.x; t < rs * NRHS; t += 64) X[(long long)r0 * NRHS + t] = xb[t];
     if (threadIdx.x == 0) {
         __threadfence();   // release: this block's rows
         __hip_atomic_store(flag + blk, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2563,8 +2616,8 @@ void lu_solve_inv_n(hipStream_t s, const float2 *A, long long lda, int n, const 
         }
         const int ticket0 = (int)ts.tickets, epoch = ++ts.epoch;
         ts.tickets += nb;
-        if (tri == 0) k_trsv_flag<true, NRHS><<<nb, 256, 0, s>>>(A, lda, n, DL, GL, X, ts.words, ticket0, epoch);
-        else k_trsv_flag<false, NRHS><<<nb, 256, 0, s>>>(A, lda, n, DU, GU, X, ts.words, ticket0, epoch);
+        if (tri == 0) k_trsv_flag<true, NRHS><<<nb, kTrsvT, 0, s>>>(A, lda, n, DL, GL, X, ts.words, ticket0, epoch);
+        else k_trsv_flag<false, NRHS><<<nb, kTrsvT, 0, s>>>(A, lda, n, DU, GU, X, ts.words, ticket0, epoch);
         check("k_trsv_flag");
     }
 }
