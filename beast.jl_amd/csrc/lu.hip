@@ -1791,13 +1791,15 @@ __global__ __launch_bounds__(256) void k_blk_coupling(const float2 *A, long long
     blk_coupling_body<LOWER>(A, lda, n, Dinv, G, blockIdx.x, blockIdx.y);
 }
 
-// One launch per triangle.  Workgroups take their block in the order they
-// start (a ticket from an atomic counter, so every block a workgroup waits for
-// belongs to a workgroup that has already started); block i subtracts the
-// products with the blocks it depends on as their ready flags appear, applies
-// its inverted diagonal block and publishes its rows with a ready flag (agent
-// scope release / acquire).  A wait that exceeds kTrsvSpin polls sets the
-// error word and gives up, so every workgroup ends.
+// One launch per triangle, one workgroup per CU.  A workgroup takes a block in the
+// order of a ticket counter, solves it and takes the next, until none is left (so a
+// block never waits for a workgroup that has not started: every block it waits for
+// has an earlier ticket, held by a running workgroup).  Block i subtracts the
+// products with the blocks it depends on as their ready flags appear, applies its
+// inverted diagonal block and publishes its rows with a ready flag (agent scope
+// release / acquire).  A wait that exceeds kTrsvSpin polls sets the error word and
+// gives up, so every workgroup ends.  Tickets handed out per launch: the blocks plus
+// one final ticket per workgroup.
 constexpr long long kTrsvSpin = 1LL << 24;
 
 template <bool LOWER, int NRHS>
@@ -1812,93 +1814,96 @@ __global__ __launch_bounds__(kTrsvT) void k_trsv_flag(const float2 *A, long long
     __shared__ int s_idx, s_ready;
     int *counter = sync, *err = sync + 1, *flag = sync + 2;
     const int nblk = (n + kIB - 1) / kIB;
-    if (threadIdx.x == 0)
-        s_idx = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - ticket0;
-    __syncthreads();
-    const int idx = s_idx;
-    if (idx < 0 || idx >= nblk) return;   // cannot happen: one ticket per workgroup
-    const int blk = LOWER ? idx : nblk - 1 - idx;
-    const int r0 = blk * kIB, rs = min(kIB, n - r0);
-    const int jadj = LOWER ? blk - 1 : blk + 1, cadj = jadj * kIB, csadj = min(kIB, n - cadj);
-    auto wait_flag = [&](int j) {   // thread 0
-        long long it = 0;
-        while (__hip_atomic_load(flag + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-            if (++it > kTrsvSpin) {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
+    for (;;) {
+        __syncthreads();   // the previous block's LDS reads are done
+        if (threadIdx.x == 0)
+            s_idx = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - ticket0;
+        __syncthreads();
+        const int idx = s_idx;
+        if (idx < 0 || idx >= nblk) return;   // no block left
+        const int blk = LOWER ? idx : nblk - 1 - idx;
+        const int r0 = blk * kIB, rs = min(kIB, n - r0);
+        const int jadj = LOWER ? blk - 1 : blk + 1, cadj = jadj * kIB, csadj = min(kIB, n - cadj);
+        auto wait_flag = [&](int j) {   // thread 0
+            long long it = 0;
+            while (__hip_atomic_load(flag + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                if (++it > kTrsvSpin) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    };
-    // the block's own operands, held for the whole solve: the inverted diagonal block
-    // (LDS) and the coupling G to the adjacent block (registers), so that neither is
-    // loaded after a wait
-    BlkRegs a, g;
-    blk_load(Dinv + (size_t)blk * kIB * kIB, kIB, rs, rs, a);
-    if (idx > 0) blk_load(G + (size_t)blk * kIB * kIB, kIB, rs, csadj, g);
+        };
+        // the block's own operands, held for the whole solve: the inverted diagonal block
+        // (LDS) and the coupling G to the adjacent block (registers), so that neither is
+        // loaded after a wait
+        BlkRegs a, g;
+        blk_load(Dinv + (size_t)blk * kIB * kIB, kIB, rs, rs, a);
+        if (idx > 0) blk_load(G + (size_t)blk * kIB * kIB, kIB, rs, csadj, g);
 #pragma unroll
-    for (int p = 0; p < kBlkRows; ++p)
+        for (int p = 0; p < kBlkRows; ++p)
 #pragma unroll
-        for (int j = 0; j < kBlkChunks; ++j) dsh[p][j][threadIdx.x] = a[p][j];
-    for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x) xs[t] = t < rs * NRHS ? X[(long long)r0 * NRHS + t] : float2{0, 0};
-    __syncthreads();
-    // every dependency but the adjacent block, taken as their ready flags appear; the
-    // next block of L (or U) is loaded before its flag is waited for
-    const int early = max(idx - 1, 0);
-    int done = 0;
-    auto load_early = [&](int d) {
-        const int c0 = (LOWER ? d : nblk - 1 - d) * kIB;
-        blk_load(A + (long long)r0 * lda + c0, lda, rs, min(kIB, n - c0), a);
-    };
-    if (early > 0) load_early(0);
-    while (done < early) {
-        if (threadIdx.x == 0) {
-            wait_flag(LOWER ? done : nblk - 1 - done);
-            // blocks publish in order, so every further flag already set is usable too
-            int ready = done + 1;
-            while (ready < early &&
-                   __hip_atomic_load(flag + (LOWER ? ready : nblk - 1 - ready), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) == epoch)
-                ++ready;
-            __threadfence();   // acquire: the rows of those blocks
-            s_ready = ready;
-        }
+            for (int j = 0; j < kBlkChunks; ++j) dsh[p][j][threadIdx.x] = a[p][j];
+        for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x) xs[t] = t < rs * NRHS ? X[(long long)r0 * NRHS + t] : float2{0, 0};
         __syncthreads();
-        const int ready = s_ready;
-        for (; done < ready; ++done) {
-            const int j = LOWER ? done : nblk - 1 - done;
-            const int c0 = j * kIB, cs = min(kIB, n - c0);
-            for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x)
-                xb[t] = t < cs * NRHS ? X[(long long)c0 * NRHS + t] : float2{0, 0};
+        // every dependency but the adjacent block, taken as their ready flags appear; the
+        // next block of L (or U) is loaded before its flag is waited for
+        const int early = max(idx - 1, 0);
+        int done = 0;
+        auto load_early = [&](int d) {
+            const int c0 = (LOWER ? d : nblk - 1 - d) * kIB;
+            blk_load(A + (long long)r0 * lda + c0, lda, rs, min(kIB, n - c0), a);
+        };
+        if (early > 0) load_early(0);
+        while (done < early) {
+            if (threadIdx.x == 0) {
+                wait_flag(LOWER ? done : nblk - 1 - done);
+                // blocks publish in order, so every further flag already set is usable too
+                int ready = done + 1;
+                while (ready < early &&
+                       __hip_atomic_load(flag + (LOWER ? ready : nblk - 1 - ready), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT) == epoch)
+                    ++ready;
+                __threadfence();   // acquire: the rows of those blocks
+                s_ready = ready;
+            }
             __syncthreads();
-            blk_apply<true, NRHS>(a, rs, xb, xs);
-            if (done + 1 < early) load_early(done + 1);
+            const int ready = s_ready;
+            for (; done < ready; ++done) {
+                const int j = LOWER ? done : nblk - 1 - done;
+                const int c0 = j * kIB, cs = min(kIB, n - c0);
+                for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x)
+                    xb[t] = t < cs * NRHS ? X[(long long)c0 * NRHS + t] : float2{0, 0};
+                __syncthreads();
+                blk_apply<true, NRHS>(a, rs, xb, xs);
+                if (done + 1 < early) load_early(done + 1);
+            }
         }
-    }
-    // y = Dinv x without the adjacent block (into xb); then x = y - G x_adjacent
+        // y = Dinv x without the adjacent block (into xb); then x = y - G x_adjacent
 #pragma unroll
-    for (int p = 0; p < kBlkRows; ++p)
+        for (int p = 0; p < kBlkRows; ++p)
 #pragma unroll
-        for (int j = 0; j < kBlkChunks; ++j) a[p][j] = dsh[p][j][threadIdx.x];
-    blk_apply<false, NRHS>(a, rs, xs, xb);
-    if (idx > 0) {
+            for (int j = 0; j < kBlkChunks; ++j) a[p][j] = dsh[p][j][threadIdx.x];
+        blk_apply<false, NRHS>(a, rs, xs, xb);
+        if (idx > 0) {
+            if (threadIdx.x == 0) {
+                wait_flag(jadj);
+                __threadfence();   // acquire: the adjacent block's rows
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x)
+                xs[t] = t < csadj * NRHS ? X[(long long)cadj * NRHS + t] : float2{0, 0};
+            __syncthreads();
+            blk_apply<true, NRHS>(g, rs, xs, xb);
+        }
+        // wave 0 writes the rows, so that thread 0's release fence covers every store (a
+        // barrier does not wait for other waves' stores, and a fence orders its own wave's)
+        if (threadIdx.x < 64)
+            for (int t = threadIdx.x; t < rs * NRHS; t += 64) X[(long long)r0 * NRHS + t] = xb[t];
         if (threadIdx.x == 0) {
-            wait_flag(jadj);
-            __threadfence();   // acquire: the adjacent block's rows
+            __threadfence();   // release: this block's rows
+            __hip_atomic_store(flag + blk, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __syncthreads();
-        for (int t = threadIdx.x; t < kIB * NRHS; t += blockDim.x)
-            xs[t] = t < csadj * NRHS ? X[(long long)cadj * NRHS + t] : float2{0, 0};
-        __syncthreads();
-        blk_apply<true, NRHS>(g, rs, xs, xb);
-    }
-    // wave 0 writes the rows, so that thread 0's release fence covers every store (a
-    // barrier does not wait for other waves' stores, and a fence orders its own wave's)
-    if (threadIdx.x < 64)
-        for (int t = threadIdx.x; t < rs * NRHS; t += 64) X[(long long)r0 * NRHS + t] = xb[t];
-    if (threadIdx.x == 0) {
-        __threadfence();   // release: this block's rows
-        __hip_atomic_store(flag + blk, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -2599,6 +2604,7 @@ struct TrsvSync {
     int *words;           // [0] ticket counter, [1] error flag, [2 ..] ready flags
     long long &tickets;   // tickets handed out so far
     int &epoch;           // ready-flag value of the last triangle
+    int cus;              // workgroups of a launch at most (one per CU)
 };
 
 template <int NRHS>
@@ -2608,16 +2614,17 @@ void lu_solve_inv_n(hipStream_t s, const float2 *A, long long lda, int n, const 
     const size_t bsz = (size_t)nb * kIB * kIB;
     const float2 *DL = Dinv, *DU = Dinv + bsz, *GL = Dinv + 2 * bsz, *GU = Dinv + 3 * bsz;
     for (int tri = 0; tri < 2; ++tri) {
-        if (ts.tickets + nb >= (1LL << 30) || ts.epoch >= (1 << 30)) {   // restart the counters
+        if (ts.tickets + 2LL * nb >= (1LL << 30) || ts.epoch >= (1 << 30)) {   // restart the counters
             k_fill_int<<<(unsigned)((nb + 2 + 255) / 256), 256, 0, s>>>(ts.words, nb + 2);
             check("k_fill_int");
             ts.tickets = 0;
             ts.epoch = 0;
         }
+        const int grid = std::max(1, std::min(nb, ts.cus));
         const int ticket0 = (int)ts.tickets, epoch = ++ts.epoch;
-        ts.tickets += nb;
-        if (tri == 0) k_trsv_flag<true, NRHS><<<nb, kTrsvT, 0, s>>>(A, lda, n, DL, GL, X, ts.words, ticket0, epoch);
-        else k_trsv_flag<false, NRHS><<<nb, kTrsvT, 0, s>>>(A, lda, n, DU, GU, X, ts.words, ticket0, epoch);
+        ts.tickets += nb + grid;
+        if (tri == 0) k_trsv_flag<true, NRHS><<<grid, kTrsvT, 0, s>>>(A, lda, n, DL, GL, X, ts.words, ticket0, epoch);
+        else k_trsv_flag<false, NRHS><<<grid, kTrsvT, 0, s>>>(A, lda, n, DU, GU, X, ts.words, ticket0, epoch);
         check("k_trsv_flag");
     }
 }
@@ -3297,7 +3304,7 @@ void apply_factors(bh_matrix *M, const double2 *B, double2 *Xd, int nrhs)
                 M->tsync_tickets = 0;
                 M->tsync_epoch = 0;
             }
-            TrsvSync ts{M->tsync.p, M->tsync_tickets, M->tsync_epoch};
+            TrsvSync ts{M->tsync.p, M->tsync_tickets, M->tsync_epoch, M->ctx->num_cus};
             ProfScope prof(*M->ctx, "k_trsv_inv", 8.0 * n * (double)n);   // one read of both FP32 factors
             lu_solve_inv(s, (const float2 *)M->lu32.p, M->lda, n, (const float2 *)M->dinv32.p, x32, nrhs, ts);
         }
