@@ -289,10 +289,20 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v)
 // their factored entries (L left of the diagonal, U from it).  The columns run
 // in eight phases of four: phase P updates the columns [4P, kPW), those of the
 // phase under a predicate (a full unroll of the 32 x 32 elimination does not fit
-// the instruction cache).
+// the instruction cache).  The elimination has no branch (rows that take no part
+// subtract a zero multiple of the pivot row), and the next column's key and its
+// DPP reduction follow its own update, so they overlap the other columns' updates.
+// The slots must hold finite values before the first column (a column without a
+// pivot reads wave 0's slot).
+__device__ __forceinline__ unsigned pivot_key(float2 x, bool used, int tid)
+{
+    const float mag = fabsf(x.x) + fabsf(x.y);
+    return used ? 0u : ((__float_as_uint(mag) & ~2047u) | (unsigned)(2047 - tid));
+}
+
 template <int P>
 __device__ __forceinline__ void gepp_phase(float2 (&v)[kPW], bool &used, int pw, int *piv, unsigned (*s_key)[4],
-                                           float2 (*s_row)[4][kPW])
+                                           float2 (*s_row)[4][kPW], unsigned &key, unsigned &wmax)
 {
     constexpr int C0 = 4 * P, NW = kTsluT / 64;
     static_assert(NW <= 4, "one key slot per wave");
@@ -301,17 +311,7 @@ __device__ __forceinline__ void gepp_phase(float2 (&v)[kPW], bool &used, int pw,
 #pragma unroll 1
     for (int j = C0; j < jend; ++j) {
         const int par = j & 1;
-        // column j of this row (selects per component: a conditional struct copy
-        // makes the compiler move v to scratch memory)
-        float2 vj = v[C0];
-#pragma unroll
-        for (int q = 1; q < 4; ++q) {
-            vj.x = (j == C0 + q) ? v[C0 + q].x : vj.x;
-            vj.y = (j == C0 + q) ? v[C0 + q].y : vj.y;
-        }
-        const float mag = fabsf(vj.x) + fabsf(vj.y);
-        const unsigned key = used ? 0u : ((__float_as_uint(mag) & ~2047u) | (unsigned)(2047 - tid));
-        const unsigned wmax = wave_max_u32(key);
+        // key and wmax hold column j's key of this row and the wave's best
         if (key == wmax && wmax != 0u) {
 #pragma unroll
             for (int cc = C0; cc < kPW; ++cc) s_row[par][wid][cc] = v[cc];
@@ -329,33 +329,49 @@ __device__ __forceinline__ void gepp_phase(float2 (&v)[kPW], bool &used, int pw,
         const int p = kmax ? 2047 - (int)(kmax & 2047u) : -1;
         if (tid == 0) piv[j] = p;
         if (tid == p) used = true;
-        if (!used && p >= 0) {
-            float2 pr[kPW];   // the pivot row, read in one batch
+        float2 pr[kPW];   // the pivot row, read in one batch
 #pragma unroll
-            for (int cc = C0; cc < kPW; ++cc) pr[cc] = s_row[par][wbest][cc];
-            float2 d = pr[C0];
+        for (int cc = C0; cc < kPW; ++cc) pr[cc] = s_row[par][wbest][cc];
+        float2 d = pr[C0], vj = v[C0];
+        // column j of this row and of the pivot row (selects per component: a
+        // conditional struct copy makes the compiler move v to scratch memory)
 #pragma unroll
-            for (int q = 1; q < 4; ++q) {
-                d.x = (j == C0 + q) ? pr[C0 + q].x : d.x;
-                d.y = (j == C0 + q) ? pr[C0 + q].y : d.y;
+        for (int q = 1; q < 4; ++q) {
+            d.x = (j == C0 + q) ? pr[C0 + q].x : d.x;
+            d.y = (j == C0 + q) ? pr[C0 + q].y : d.y;
+            vj.x = (j == C0 + q) ? v[C0 + q].x : vj.x;
+            vj.y = (j == C0 + q) ? v[C0 + q].y : vj.y;
+        }
+        const float den = fmaf(d.x, d.x, d.y * d.y);
+        const bool elim = !used && p >= 0 && den != 0.f;
+        // multiplier vj / d through the hardware reciprocal of |d|^2 (zero for rows
+        // that take no part)
+        const float rden = elim ? __builtin_amdgcn_rcpf(den) : 0.f;
+        const float2 l{fmaf(vj.x, d.x, vj.y * d.y) * rden, fmaf(vj.y, d.x, -vj.x * d.y) * rden};
+#pragma unroll
+        for (int cc = C0; cc < kPW; ++cc) {
+            const float ux = fmaf(l.y, pr[cc].y, fmaf(-l.x, pr[cc].x, v[cc].x));
+            const float uy = fmaf(-l.y, pr[cc].x, fmaf(-l.x, pr[cc].y, v[cc].y));
+            if (cc < C0 + 4) {
+                v[cc].x = cc == j ? (elim ? l.x : v[cc].x) : (cc > j ? ux : v[cc].x);
+                v[cc].y = cc == j ? (elim ? l.y : v[cc].y) : (cc > j ? uy : v[cc].y);
+            } else {
+                v[cc].x = ux;
+                v[cc].y = uy;
             }
-            if (d.x != 0.f || d.y != 0.f) {
-                // multiplier vj / d through the hardware reciprocal of |d|^2
-                const float rden = __builtin_amdgcn_rcpf(fmaf(d.x, d.x, d.y * d.y));
-                const float2 l{fmaf(vj.x, d.x, vj.y * d.y) * rden, fmaf(vj.y, d.x, -vj.x * d.y) * rden};
+        }
+        // the next column's key
+        if (j + 1 < pw) {
+            float2 vn = v[C0 + 1];
 #pragma unroll
-                for (int cc = C0; cc < kPW; ++cc) {
-                    const float ux = fmaf(l.y, pr[cc].y, fmaf(-l.x, pr[cc].x, v[cc].x));
-                    const float uy = fmaf(-l.y, pr[cc].x, fmaf(-l.x, pr[cc].y, v[cc].y));
-                    if (cc < C0 + 4) {
-                        v[cc].x = cc == j ? l.x : (cc > j ? ux : v[cc].x);
-                        v[cc].y = cc == j ? l.y : (cc > j ? uy : v[cc].y);
-                    } else {
-                        v[cc].x = ux;
-                        v[cc].y = uy;
-                    }
+            for (int q = 2; q <= 4; ++q) {
+                if (C0 + q < kPW) {
+                    vn.x = (j + 1 == C0 + q) ? v[C0 + q].x : vn.x;
+                    vn.y = (j + 1 == C0 + q) ? v[C0 + q].y : vn.y;
                 }
             }
+            key = pivot_key(vn, used, tid);
+            wmax = wave_max_u32(key);
         }
     }
 }
@@ -363,14 +379,16 @@ __device__ __forceinline__ void gepp_phase(float2 (&v)[kPW], bool &used, int pw,
 __device__ __forceinline__ void block_gepp(float2 (&v)[kPW], bool used, int pw, int *piv, unsigned (*s_key)[4],
                                            float2 (*s_row)[4][kPW])
 {
-    gepp_phase<0>(v, used, pw, piv, s_key, s_row);
-    gepp_phase<1>(v, used, pw, piv, s_key, s_row);
-    gepp_phase<2>(v, used, pw, piv, s_key, s_row);
-    gepp_phase<3>(v, used, pw, piv, s_key, s_row);
-    gepp_phase<4>(v, used, pw, piv, s_key, s_row);
-    gepp_phase<5>(v, used, pw, piv, s_key, s_row);
-    gepp_phase<6>(v, used, pw, piv, s_key, s_row);
-    gepp_phase<7>(v, used, pw, piv, s_key, s_row);
+    unsigned key = pivot_key(v[0], used, threadIdx.x);
+    unsigned wmax = wave_max_u32(key);
+    gepp_phase<0>(v, used, pw, piv, s_key, s_row, key, wmax);
+    gepp_phase<1>(v, used, pw, piv, s_key, s_row, key, wmax);
+    gepp_phase<2>(v, used, pw, piv, s_key, s_row, key, wmax);
+    gepp_phase<3>(v, used, pw, piv, s_key, s_row, key, wmax);
+    gepp_phase<4>(v, used, pw, piv, s_key, s_row, key, wmax);
+    gepp_phase<5>(v, used, pw, piv, s_key, s_row, key, wmax);
+    gepp_phase<6>(v, used, pw, piv, s_key, s_row, key, wmax);
+    gepp_phase<7>(v, used, pw, piv, s_key, s_row, key, wmax);
     __syncthreads();
 }
 
@@ -463,6 +481,8 @@ __device__ __forceinline__ void tslu_panel_body(float2 *A, long long lda, int n,
     __shared__ int s_win[kPW], s_at[kPW], s_pos[kPW];
     __shared__ __align__(16) float2 s_u[kPW][kPW];
     const int tid = threadIdx.x;
+    for (int e = tid; e < 2 * 4 * kPW; e += kTsluT) (&s_row[0][0][0])[e] = float2{0, 0};   // finite slots
+    __syncthreads();
     int g = bid, nl = nsets0, off = 0, coff = 0;
     int rowid = -1;
     float2 v[kPW], orig[kPW];
