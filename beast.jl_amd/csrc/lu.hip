@@ -1605,18 +1605,78 @@ __global__ __launch_bounds__(256) void k_trsv_step(const typename CT<T>::t *A, l
 constexpr int kIB = 128;
 constexpr int kTrinvParts = 4;   // lanes per column in k_trinv
 
+// Inverse of the unit-lower 128 x 128 block held in S (LDS, zero above the diagonal,
+// in place), by 32 x 32 blocks: the four diagonal blocks by forward substitution, a
+// thread per column with the column in registers (no barrier); then block row I = 1..3
+// as S_IJ = -S_II sum_{K=J..I-1} L_IK S_KJ (J < I), the sums first (into T, while
+// L_IJ is still in place), then the products with S_II.  512 threads.
+constexpr int kIB32 = 32;   // block of the blocked inverse
+__device__ __forceinline__ void unit_lower_inverse_blocked(float2 *S, float2 (*T)[kIB32][kIB32])
+{
+    const int t = threadIdx.x;
+    float2 x[kIB32];
+    const int blk = t / kIB32, c = t % kIB32;
+    if (t < kIB) {
+        // column c of the inverse of diagonal block blk: x_r = [r == c] - sum_{k<r} L[r][k] x_k
+        const float2 *L = S + (size_t)(kIB32 * blk) * kIB + kIB32 * blk;
+#pragma unroll
+        for (int r = 0; r < kIB32; ++r) {
+            float2 acc{0, 0};
+#pragma unroll
+            for (int k = 0; k < r; ++k) {
+                const float2 a = L[r * kIB + k];
+                acc.x = fmaf(a.x, x[k].x, fmaf(-a.y, x[k].y, acc.x));
+                acc.y = fmaf(a.x, x[k].y, fmaf(a.y, x[k].x, acc.y));
+            }
+            x[r] = r < c ? float2{0, 0} : (r == c ? float2{1, 0} : float2{-acc.x, -acc.y});
+        }
+    }
+    __syncthreads();   // every diagonal block has been read
+    if (t < kIB) {
+        float2 *L = S + (size_t)(kIB32 * blk) * kIB + kIB32 * blk;
+#pragma unroll
+        for (int r = 0; r < kIB32; ++r) L[r * kIB + c] = x[r];
+    }
+    __syncthreads();
+    for (int I = 1; I < kIB / kIB32; ++I) {
+        const float2 *rowI = S + (size_t)(kIB32 * I) * kIB;
+        for (int e = t; e < I * kIB32 * kIB32; e += kIB * kTrinvParts) {
+            const int J = e / (kIB32 * kIB32), r = e / kIB32 % kIB32, cc = e % kIB32;
+            float2 acc{0, 0};
+#pragma unroll 16
+            for (int k = kIB32 * J; k < kIB32 * I; ++k) {
+                const float2 a = rowI[r * kIB + k], b = S[(size_t)k * kIB + kIB32 * J + cc];
+                acc.x = fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x));
+                acc.y = fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y));
+            }
+            T[J][r][cc] = acc;
+        }
+        __syncthreads();
+        for (int e = t; e < I * kIB32 * kIB32; e += kIB * kTrinvParts) {
+            const int J = e / (kIB32 * kIB32), r = e / kIB32 % kIB32, cc = e % kIB32;
+            float2 acc{0, 0};
+#pragma unroll 8
+            for (int k = 0; k <= r; ++k) {
+                const float2 a = rowI[r * kIB + kIB32 * I + k], b = T[J][k][cc];
+                acc.x = fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x));
+                acc.y = fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y));
+            }
+            S[(size_t)(kIB32 * I + r) * kIB + kIB32 * J + cc] = float2{-acc.x, -acc.y};
+        }
+        __syncthreads();
+    }
+}
+
 // In-place inverse of diagonal block blockIdx.x of the unit-lower (LOWER) or
-// upper factor, row by row in LDS: X[i][t] = ([i == t] - sum_k T[i][k] X[k][t]) / T[i][i]
-// over the rows k already inverted (entries outside the triangle are zero).
-// kTrinvParts threads per column share each row's dot product (partial sums
-// combined through LDS); rows past the end of the matrix become identity rows.
+// upper factor in LDS; rows past the end of the matrix become identity rows.
+// LOWER: blocked (unit_lower_inverse_blocked).  Upper: row by row,
+// X[i][t] = ([i == t] - sum_k T[i][k] X[k][t]) / T[i][i] over the rows k already
+// inverted (entries outside the triangle are zero), kTrinvParts threads per column
+// sharing each row's dot product (partial sums combined through LDS).
 template <bool LOWER>
 __device__ __forceinline__ void trinv_body(const float2 *A, long long lda, int n, float2 *Dinv, int bid)
 {
     __shared__ float2 S[kIB * kIB];
-    __shared__ float2 red[kTrinvParts][kIB];
-    // part = t / kIB: the waves of a part read whole rows of S (consecutive columns,
-    // no bank conflicts); part p sums the terms k = p mod kTrinvParts of each dot product
     const int t = threadIdx.x, col = t % kIB, part = t / kIB;
     const int b0 = bid * kIB, bs = min(kIB, n - b0);
     for (int idx = t; idx < kIB * kIB; idx += kIB * kTrinvParts) {
@@ -1627,6 +1687,16 @@ __device__ __forceinline__ void trinv_body(const float2 *A, long long lda, int n
         S[idx] = v;
     }
     __syncthreads();
+    if constexpr (LOWER) {
+        __shared__ float2 T[kIB / kIB32 - 1][kIB32][kIB32];
+        unit_lower_inverse_blocked(S, T);
+        float2 *out = Dinv + (size_t)bid * kIB * kIB;
+        for (int idx = t; idx < kIB * kIB; idx += kIB * kTrinvParts) out[idx] = S[idx];
+        return;
+    }
+    __shared__ float2 red[kTrinvParts][kIB];
+    // part = t / kIB: the waves of a part read whole rows of S (consecutive columns,
+    // no bank conflicts); part p sums the terms k = p mod kTrinvParts of each dot product
     for (int s = LOWER ? 1 : 0; s < kIB; ++s) {
         const int i = LOWER ? s : kIB - 1 - s;
         const int k0 = LOWER ? part : i + 1 + part, k1 = LOWER ? i : kIB;
