@@ -2205,14 +2205,22 @@ void gemm_launch(Ctx &ctx, hipStream_t s, typename CT<T>::t *A, long long lda, i
 
 // Tile shapes chosen by measurement on MI355X (N = 12288 LU): FP32 3M products,
 // 128 x 64 complex tiles, two waves per SIMD; FP64 4M, 64 x 64, two waves per SIMD.
+// FP32 products of one tile row (M <= 128, the block rows' updates) take 128 x 32
+// tiles at three waves per SIMD: twice the tiles, so a 40k-wide row of them fills
+// the device in whole rounds (C3 block rows: 46.5 -> 42.0 ms of FP32 products; each
+// entry's sum over k runs in the same order as with 128 x 64 tiles).
 template <typename T>
 void gemm(Ctx &ctx, hipStream_t s, typename CT<T>::t *A, long long lda, int ar, int ac, int br, int bc, int cr, int cc,
           int M, int N, int K)
 {
     if (M <= 0 || N <= 0 || K <= 0) return;
     BH_REQUIRE(K % kBK == 0, "gemm K=%d not a multiple of %d", K, kBK);
-    if constexpr (sizeof(T) == 4) gemm_launch<T, 4, 2, true, 2>(ctx, s, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
-    else gemm_launch<T, 2, 2, false, 2>(ctx, s, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
+    if constexpr (sizeof(T) == 4) {
+        if (M <= 128) gemm_launch<T, 4, 1, true, 3>(ctx, s, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
+        else gemm_launch<T, 4, 2, true, 2>(ctx, s, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
+    } else {
+        gemm_launch<T, 2, 2, false, 2>(ctx, s, A, lda, ar, ac, br, bc, cr, cc, M, N, K);
+    }
 }
 
 template <typename T>
@@ -2383,14 +2391,15 @@ __global__ void k_inv_to_e(float2 *E, int nblk)
 // Rows [r, r + kIB) of a block row, columns [c0, c1): C <- inv(L_bb) C, formed as
 // C - E C with E = I - inv(L_bb).  The product has one tile row (M = kIB = BM), so
 // each workgroup reads, as B, exactly the columns of C it writes, and reads all of
-// them (main loop, then the epilogue's reads of C) before its first write.
+// them (main loop, then the epilogue's reads of C) before its first write.  128 x 32
+// tiles, as for the other one-row products.
 void blockrow_apply_inverse(Ctx &ctx, hipStream_t s, const float2 *E, float2 *A, long long lda, int r, int c0, int c1)
 {
     const int N = c1 - c0;
     static_assert(32 * 4 == kIB, "one tile row of the FP32 product must cover the block");
     ProfScope prof(ctx, "k_cgemm", 8.0 * kIB * (double)N * kIB, s);
     float2 *C = A + (long long)r * lda + c0;
-    k_cgemm<float, 4, 2, true, 2><<<(unsigned)((N + 63) / 64), 256, 0, s>>>(E, kIB, C, lda, C, lda, kIB, N, kIB);
+    k_cgemm<float, 4, 1, true, 3><<<(unsigned)((N + 31) / 32), 256, 0, s>>>(E, kIB, C, lda, C, lda, kIB, N, kIB);
     check("k_cgemm");
 }
 
