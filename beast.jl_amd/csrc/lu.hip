@@ -1921,6 +1921,8 @@ __global__ __launch_bounds__(kTrsvT) void k_trsv_flag(const float2 *A, long long
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
+                // once a wait has given up, the others do not wait out their own limits
+                if ((it & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
                 __builtin_amdgcn_s_sleep(1);
             }
         };
@@ -2638,6 +2640,11 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
             }
             trailing(s, k, kb, nk, nk + nkb);
             BH_CHECK_HIP(hipEventRecord(ready, s));
+            if (!ahead_first) {
+                // the rest of the update first in host order: the next block column's
+                // many short launches are then queued while the long product runs
+                trailing(s, k, kb, nk + nkb, n);
+            }
             BH_CHECK_HIP(hipStreamWaitEvent(s2, ready, 0));
             factor_block<T>(ctx, s2, W, A, lda, n, nk, nkb, ipiv);
             BH_CHECK_HIP(hipEventRecord(done, s2));
@@ -2645,8 +2652,8 @@ void lu_factor(Ctx &ctx, typename CT<T>::t *A, long long lda, int n, long long *
             if (ahead_first) {
                 swaps(k, kb, 0, k, nk + nkb, n);
                 block_row_u12<T>(ctx, s, W, A, lda, n, k, kb, nk + nkb, n, split ? &PB : nullptr, false);
+                trailing(s, k, kb, nk + nkb, n);
             }
-            trailing(s, k, kb, nk + nkb, n);
             BH_CHECK_HIP(hipStreamWaitEvent(s, done, 0));
         }
     }
